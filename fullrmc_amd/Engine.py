@@ -1,0 +1,394 @@
+"""Engine: the stochastic run loop of fullrmc's Engine.run for the pair path.
+
+Two ways to run the same trajectory:
+
+* :meth:`Engine.run` -- the reference's loop structure (Engine.py:3377-3516):
+  every move selects a group, generates the move, and asks each constraint
+  for its before/after histograms through the gfx950 drop-in extension
+  calls, then applies the accept rule.  One move per kernel launch.
+* :meth:`Engine.run_device` -- the state lives in HBM and k proposals are
+  evaluated per window against it (histogram deltas, G(r), S(Q),
+  chi-squares on the GPU); the first accepted proposal is committed and the
+  random streams are rewound to just after it, so the accepted-move
+  sequence, histograms and chi-squares equal :meth:`run`'s.
+
+Both use fullrmc's two global streams, Python's ``random`` and NumPy's
+legacy ``numpy.random``, exactly where the reference draws from them.
+"""
+import ctypes
+import random
+import time
+
+import numpy as np
+
+from . import _lib
+from .Core.boundary_conditions_collection import transform_coordinates
+from .Generators import RandomSelector, TranslationGenerator
+from .Globals import FLOAT_TYPE, INT_TYPE
+from .systems import box_coordinates, reciprocal, volume
+
+
+class Engine:
+    """Atomic configuration, groups, constraints and the run loop."""
+
+    def __init__(self):
+        self.realCoordinates = None
+        self.boxCoordinates = None
+        self.groups = []
+        self.generators = []
+        self.constraints = []
+        self.selector = RandomSelector()
+        self.tolerance = FLOAT_TYPE(0)
+        self.generated = self.tried = self.accepted = self.tolerated = 0
+        self.totalStandardError = None
+        self._dev = None
+        self._dev_kmax = 0
+        self._dev_synced = False
+
+    # ------------------------------------------------------------ system
+    def set_system(self, realCoordinates, allElements, moleculesIndex=None, basisVectors=None,
+                   reciprocalBasisVectors=None):
+        """Coordinates (n,3), element name per atom, molecule index per atom
+        and periodic cell vectors as rows (number = cube edge).  Replaces
+        Engine.set_pdb + set_boundary_conditions (Engine.py:2134-2298)."""
+        real = np.array(realCoordinates, dtype=FLOAT_TYPE)
+        if real.ndim != 2 or real.shape[1] != 3:
+            raise ValueError("realCoordinates must be (n,3)")
+        n = real.shape[0]
+        if len(allElements) != n:
+            raise ValueError("one element per atom is required")
+        if basisVectors is None:
+            raise NotImplementedError("infinite boundary conditions are not part of this path")
+        if np.isscalar(basisVectors):
+            basisVectors = np.diag([basisVectors] * 3)
+        self.basisVectors = np.array(basisVectors, dtype=FLOAT_TYPE)
+        self.reciprocalBasisVectors = (reciprocal(self.basisVectors) if reciprocalBasisVectors is None
+                                       else np.array(reciprocalBasisVectors, dtype=FLOAT_TYPE))
+        self.volume = volume(self.basisVectors)
+        self.isPBC = True
+        self.realCoordinates = real
+        self.boxCoordinates = box_coordinates(self.reciprocalBasisVectors, real)
+        self.allElements = list(allElements)
+        self.elements = sorted(set(self.allElements))
+        lut = {e: i for i, e in enumerate(self.elements)}
+        self.elementsIndex = np.array([lut[e] for e in self.allElements], dtype=INT_TYPE)
+        self.numberOfElements = len(self.elements)
+        self.numberOfAtomsPerElement = {}
+        for e in self.allElements:
+            self.numberOfAtomsPerElement[e] = self.numberOfAtomsPerElement.get(e, 0) + 1
+        self.moleculesIndex = (np.arange(n, dtype=INT_TYPE) if moleculesIndex is None
+                               else np.array(moleculesIndex, dtype=INT_TYPE))
+        self.numberOfAtoms = n
+        self.numberDensity = FLOAT_TYPE(n) / FLOAT_TYPE(self.volume)
+        self.set_groups_as_atoms()
+        self._drop_device()
+
+    def set_groups_as_atoms(self):
+        self.set_groups([np.array([i], dtype=INT_TYPE) for i in range(self.numberOfAtoms)])
+
+    def set_groups_as_molecules(self):
+        order = {}
+        for i, m in enumerate(self.moleculesIndex):
+            order.setdefault(int(m), []).append(i)
+        self.set_groups([np.array(v, dtype=INT_TYPE) for v in order.values()])
+
+    def set_groups(self, groups, generator=None):
+        self.groups = [np.array(g, dtype=INT_TYPE) for g in groups]
+        gen = generator if generator is not None else TranslationGenerator(amplitude=0.2)
+        self.generators = [gen] * len(self.groups)
+        self._drop_device()
+
+    def set_move_generator(self, generator):
+        """Use one move generator for every group."""
+        self.generators = [generator] * len(self.groups)
+
+    def set_tolerance(self, tolerance):
+        """Percentage [0,100] of worse moves allowed (Engine.py:1826-1842)."""
+        t = FLOAT_TYPE(tolerance)
+        if not 0 <= t <= 100:
+            raise ValueError("tolerance must be in [0,100]")
+        self.tolerance = FLOAT_TYPE(t / 100.)
+
+    def add_constraints(self, constraints):
+        for c in constraints if isinstance(constraints, (list, tuple)) else [constraints]:
+            c._attach(self)
+            self.constraints.append(c)
+        self._drop_device()
+
+    # ----------------------------------------------------- standard error
+    def compute_total_standard_error(self, constraints, current="standardError"):
+        """np.sum of standard errors over variance (Engine.py:3000-3029)."""
+        return np.sum([getattr(c, current) / c.varianceSquared for c in constraints])
+
+    def initialize_used_constraints(self, force=False):
+        for c in self.constraints:
+            if c.data is None or force:
+                c.compute_data()
+        self.totalStandardError = self.compute_total_standard_error(self.constraints)
+
+    # ------------------------------------------------------------- plain
+    def run(self, numberOfSteps=100, log=None):
+        """The reference step loop, one move per drop-in call sequence."""
+        self._pull_device()
+        self.initialize_used_constraints()
+        for _ in range(int(numberOfSteps)):
+            self.generated += 1
+            gi = self.selector.select_index(len(self.groups))
+            rel = self.groups[gi]
+            before = np.array(self.realCoordinates[rel], dtype=FLOAT_TYPE)
+            moved_real = self.generators[gi].move(before)
+            moved_box = transform_coordinates(self.reciprocalBasisVectors, moved_real)
+            self.tried += 1
+            for c in self.constraints:
+                c.compute_before_move(realIndexes=rel, relativeIndexes=rel)
+                c.compute_after_move(realIndexes=rel, relativeIndexes=rel, movedBoxCoordinates=moved_box)
+            new = self.compute_total_standard_error(self.constraints, current="afterMoveStandardError")
+            reject = False
+            if new > self.totalStandardError:
+                if random.random() > self.tolerance:
+                    reject = True
+                else:
+                    self.tolerated += 1
+                    self.totalStandardError = new
+            else:
+                self.totalStandardError = new
+            if reject:
+                for c in self.constraints:
+                    c.reject_move(realIndexes=rel, relativeIndexes=rel)
+            else:
+                self.accepted += 1
+                for c in self.constraints:
+                    c.accept_move(realIndexes=rel, relativeIndexes=rel)
+                self.realCoordinates[rel] = moved_real
+                self.boxCoordinates[rel] = moved_box
+            if log is not None:
+                log.append((int(gi), not reject, float(self.totalStandardError)))
+        self._dev_synced = False
+
+    # ------------------------------------------------------------ device
+    def _drop_device(self):
+        if self._dev is not None:
+            _lib.lib().frmc_engine_destroy(self._dev)
+        self._dev = None
+        self._dev_synced = False
+
+    def __del__(self):
+        try:
+            self._drop_device()
+        except Exception:
+            pass
+
+    def _device_engine(self, kmax):
+        """Create (or reuse) the device engine holding the current state."""
+        from . import _device
+        _device.require_gpu()
+        if self._dev is not None and self._dev_kmax >= kmax and self._dev_synced:
+            return self._dev
+        self._drop_device()
+        if not self.constraints:
+            raise ValueError("no constraints")
+        if len(self.constraints) > 4:
+            raise ValueError("at most 4 constraints run on the device")
+        self.initialize_used_constraints()
+        L = _lib.lib()
+        keep = []
+
+        def f32p(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            keep.append(a)
+            return a.ctypes.data_as(_lib.c_f32p)
+
+        def i32p(a):
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            keep.append(a)
+            return a.ctypes.data_as(_lib.c_i32p)
+
+        d = _lib.EngineDesc()
+        d.n, d.nel, d.isPBC = self.numberOfAtoms, self.numberOfElements, 1
+        d.box, d.real = f32p(self.boxCoordinates), f32p(self.realCoordinates)
+        d.mol, d.elem = i32p(self.moleculesIndex), i32p(self.elementsIndex)
+        for q in range(9):
+            d.basis[q] = float(self.basisVectors.reshape(-1)[q])
+            d.recip[q] = float(self.reciprocalBasisVectors.reshape(-1)[q])
+        offs = np.zeros(len(self.groups) + 1, dtype=np.int32)
+        offs[1:] = np.cumsum([len(g) for g in self.groups])
+        d.ngroups = len(self.groups)
+        d.group_offsets, d.group_atoms = i32p(offs), i32p(np.concatenate(self.groups))
+        d.nconstraints = len(self.constraints)
+        for ci, c in enumerate(self.constraints):
+            a = c._device_arrays()
+            cd = d.constraints[ci]
+            cd.kind, cd.histSize, cd.nQ = a["kind"], a["H"], a["nQ"]
+            cd.minDistance, cd.maxDistance, cd.bin = float(a["minD"]), float(a["maxD"]), float(a["bin"])
+            cd.pairWeights, cd.pairDensity = f32p(a["w"]), f32p(a["D"])
+            cd.shellVolumes, cd.prefactor = f32p(a["shellVolumes"]), f32p(a["prefactor"])
+            cd.experimental, cd.dataWeights = f32p(a["experimental"]), f32p(a["dataWeights"])
+            cd.sqMatrix, cd.scaleFactor = f32p(a["sqMatrix"]), float(a["scaleFactor"])
+        d.kmax = int(kmax)
+        d.stream = None
+        h = ctypes.c_void_p()
+        _lib.check(L.frmc_engine_create(ctypes.byref(h), ctypes.byref(d)))
+        self._dev, self._dev_kmax = h, int(kmax)
+        # device state from the constraints' current (slot-exact) histograms
+        for ci, c in enumerate(self.constraints):
+            slot = np.concatenate([c.data["intra"].reshape(-1), c.data["inter"].reshape(-1), [0]])
+            if np.any(slot != np.rint(slot)):
+                raise ValueError("constraint histograms are not integral")
+            s = np.ascontiguousarray(slot.astype(np.int32))
+            _lib.check(L.frmc_engine_set_counts(h, ci, s.ctypes.data_as(_lib.c_i32p)))
+        chi2 = (ctypes.c_float * len(self.constraints))()
+        tse = ctypes.c_float()
+        _lib.check(L.frmc_engine_refresh(h, chi2, ctypes.byref(tse)))
+        for ci, c in enumerate(self.constraints):
+            if np.float32(chi2[ci]).tobytes() != np.float32(c.standardError).tobytes():
+                raise _lib.FrmcError("device chi-square of constraint %d (%r) differs from the host chain (%r)"
+                                     % (ci, chi2[ci], c.standardError))
+        if np.float32(tse.value).tobytes() != np.float32(self.totalStandardError).tobytes():
+            raise _lib.FrmcError("device total %r differs from the host total %r"
+                                 % (tse.value, self.totalStandardError))
+        self._dev_synced = True
+        return h
+
+    def _pull_device(self):
+        """Copy the device state (coordinates, histograms, chi-squares) back
+        into the host mirror after device runs."""
+        if self._dev is None or not self._dev_synced or not getattr(self, "_dev_dirty", False):
+            return
+        L = _lib.lib()
+        box = np.empty((self.numberOfAtoms, 3), np.float32)
+        real = np.empty((self.numberOfAtoms, 3), np.float32)
+        chi2 = (ctypes.c_float * len(self.constraints))()
+        tse = ctypes.c_float()
+        for ci, c in enumerate(self.constraints):
+            nel, H = self.numberOfElements, int(c.histogramSize)
+            slot = np.empty(2 * nel * nel * H + 1, np.int32)
+            _lib.check(L.frmc_engine_get_state(self._dev, box.ctypes.data_as(_lib.c_f32p), ci,
+                                               slot.ctypes.data_as(_lib.c_i32p), None, chi2, ctypes.byref(tse)))
+            size = nel * nel * H
+            c.data = {"intra": slot[:size].reshape(nel, nel, H).astype(np.float32),
+                      "inter": slot[size:2 * size].reshape(nel, nel, H).astype(np.float32)}
+            c.standardError = np.float32(chi2[ci])
+        _lib.check(L.frmc_engine_get_real(self._dev, real.ctypes.data_as(_lib.c_f32p)))
+        self.boxCoordinates = box
+        self.realCoordinates = real
+        self.totalStandardError = np.float32(tse.value)
+        self._dev_dirty = False
+
+    def run_device(self, numberOfSteps=100, k=64, log=None, native=True):
+        """Device-resident run with k speculative proposals per window.
+
+        With translation moves on every group (the default generator) the
+        whole loop, random streams included, runs in the native library;
+        otherwise proposals come from the Python generators (the reference's
+        own NumPy calls) and only the evaluation runs on the device.
+        Returns a dict of counters and timings."""
+        h = self._device_engine(max(k, 1))
+        gen0 = self.generators[0]
+        nat = (native and all(g is gen0 for g in self.generators)
+               and getattr(gen0, "native_kind", None) == "translation")
+        t0 = time.perf_counter()
+        if nat:
+            stats = self._run_native(h, int(numberOfSteps), int(k), gen0, log)
+        else:
+            stats = self._run_python(h, int(numberOfSteps), int(k), log)
+        stats["seconds"] = time.perf_counter() - t0
+        self._dev_dirty = True
+        self._pull_device()
+        return stats
+
+    def _run_native(self, h, nsteps, k, gen, log):
+        L = _lib.lib()
+        py, npst = _lib.MT19937(), _lib.MT19937()
+        pys = random.getstate()
+        for q in range(624):
+            py.mt[q] = pys[1][q]
+        py.pos = pys[1][624]
+        nps = np.random.get_state()
+        if nps[3]:
+            raise ValueError("numpy.random holds a cached Gaussian; the native generator cannot carry it")
+        for q in range(624):
+            npst.mt[q] = int(nps[1][q])
+        npst.pos = int(nps[2])
+        st = _lib.RunStats()
+        glog = np.empty(nsteps, np.int32) if log is not None else None
+        alog = np.empty(nsteps, np.uint8) if log is not None else None
+        tlog = np.empty(nsteps, np.float32) if log is not None else None
+        _lib.check(L.frmc_engine_run_translation(
+            h, nsteps, k, float(gen.amplitude[0]), float(gen.amplitude[1]), float(self.tolerance),
+            ctypes.byref(py), ctypes.byref(npst), ctypes.byref(st),
+            glog.ctypes.data_as(_lib.c_i32p) if glog is not None else None,
+            alog.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)) if alog is not None else None,
+            tlog.ctypes.data_as(_lib.c_f32p) if tlog is not None else None))
+        random.setstate((pys[0], tuple(int(py.mt[q]) for q in range(624)) + (int(py.pos),), pys[2]))
+        np.random.set_state(("MT19937", np.array([npst.mt[q] for q in range(624)], dtype=np.uint32),
+                             int(npst.pos), 0, 0.0))
+        self.generated += st.generated
+        self.tried += st.tried
+        self.accepted += st.accepted
+        self.tolerated += st.tolerated
+        if log is not None:
+            log.extend((int(g), bool(a), float(t)) for g, a, t in zip(glog, alog, tlog))
+        return dict(generated=st.generated, accepted=st.accepted, windows=st.windows, evaluated=st.evaluated)
+
+    def _run_python(self, h, nsteps, k, log):
+        """Speculative windows with proposals from the Python generators."""
+        L = _lib.lib()
+        real = self.realCoordinates
+        step = 0
+        windows = evaluated = accepted = 0
+        C = len(self.constraints)
+        while step < nsteps:
+            kw = min(k, nsteps - step)
+            gids = np.empty(kw, np.int32)
+            boxes, reals, u, st_pre, st_post = [], [], [], [], []
+            for p in range(kw):
+                gi = self.selector.select_index(len(self.groups))
+                rel = self.groups[gi]
+                before = np.array(real[rel], dtype=FLOAT_TYPE)
+                mr = self.generators[gi].move(before)
+                mb = box_coordinates(self.reciprocalBasisVectors, mr)
+                st_pre.append((random.getstate(), np.random.get_state()))
+                u.append(random.random())
+                st_post.append((random.getstate(), np.random.get_state()))
+                gids[p] = gi
+                boxes.append(mb)
+                reals.append(mr)
+            nb = np.ascontiguousarray(np.concatenate(boxes), dtype=np.float32)
+            tse = np.empty(kw, np.float32)
+            _lib.check(L.frmc_engine_eval(h, kw, gids.ctypes.data_as(_lib.c_i32p), nb.ctypes.data_as(_lib.c_f32p),
+                                          tse.ctypes.data_as(_lib.c_f32p), None))
+            windows += 1
+            evaluated += kw
+            cur = np.float32(self.totalStandardError)
+            acc = -1
+            for p in range(kw):
+                self.generated += 1
+                self.tried += 1
+                ok, drew = True, False
+                if tse[p] > cur:
+                    drew = True
+                    if u[p] > self.tolerance:
+                        ok = False
+                    else:
+                        self.tolerated += 1
+                step += 1
+                if log is not None:
+                    log.append((int(gids[p]), ok, float(tse[p] if ok else cur)))
+                if ok:
+                    acc = p
+                    break
+            if acc >= 0:
+                pys, nps = (st_post if drew else st_pre)[acc]
+                random.setstate(pys)
+                np.random.set_state(nps)
+                _lib.check(L.frmc_engine_commit(h, acc))
+                self.accepted += 1
+                accepted += 1
+                self.totalStandardError = tse[acc]
+                real[self.groups[gids[acc]]] = reals[acc]
+        # the native mirror of real coordinates is refreshed from this loop
+        r = np.ascontiguousarray(real, dtype=np.float32)
+        _lib.check(L.frmc_engine_set_real(h, r.ctypes.data_as(_lib.c_f32p)))
+        return dict(generated=step, accepted=accepted, windows=windows, evaluated=evaluated)

@@ -1,0 +1,121 @@
+"""Move proposals with the reference's random-number semantics.
+
+Both of fullrmc's global MT19937 streams are used exactly as the reference
+uses them: Python's ``random`` (group selection, translation amplitude,
+rotation angle, acceptance) and NumPy's legacy ``numpy.random`` (directions).
+
+  RandomSelector        Selectors/RandomSelectors.py:87-94
+  TranslationGenerator  Generators/Translations.py:112-187 with
+                        generate_random_vector, Core/Collection.py:674-701
+  RotationGenerator     Generators/Rotations.py:113-178 with
+                        get_rotation_matrix, Core/Collection.py:437-456
+  MoveGeneratorCombinator  Core/MoveGenerator.py:792-811 (shuffle=False)
+"""
+import random
+
+import numpy as np
+
+from .Globals import FLOAT_TYPE, INT_TYPE, PI, PRECISION
+
+
+class RandomSelector:
+    """Uniform group index: ``random.randint(0, ngroups-1)``."""
+
+    def select_index(self, ngroups):
+        return INT_TYPE(random.randint(0, ngroups - 1))
+
+
+def generate_random_vector(minAmp, maxAmp):
+    """Random direction (NumPy stream) scaled by an amplitude drawn from the
+    Python stream in [minAmp, maxAmp); float32 throughout."""
+    vector = np.array(1 - 2 * np.random.random(3), dtype=FLOAT_TYPE)
+    norm = np.linalg.norm(vector)
+    while norm == 0:
+        vector = np.array(1 - 2 * np.random.random(3), dtype=FLOAT_TYPE)
+        norm = np.linalg.norm(vector)
+    vector /= FLOAT_TYPE(norm)
+    base = FLOAT_TYPE(vector * minAmp)
+    span = FLOAT_TYPE(maxAmp - minAmp)
+    vector *= FLOAT_TYPE(random.random() * span)
+    vector += base
+    return vector
+
+
+class TranslationGenerator:
+    """Translate the whole group by one random vector of length below
+    ``amplitude`` (or within ``(min, max)``)."""
+
+    native_kind = "translation"
+
+    def __init__(self, amplitude=0.2):
+        self.set_amplitude(amplitude)
+
+    def set_amplitude(self, amplitude):
+        if isinstance(amplitude, (list, tuple)):
+            mn, mx = FLOAT_TYPE(amplitude[0]), FLOAT_TYPE(amplitude[1])
+            if not (mn >= 0 and mx > mn):
+                raise ValueError("translation amplitude must satisfy 0 <= min < max")
+            self.amplitude = (mn, mx)
+        else:
+            a = float(amplitude)
+            if not a > 0:
+                raise ValueError("translation amplitude must be bigger than 0")
+            self.amplitude = (FLOAT_TYPE(0), FLOAT_TYPE(a))
+
+    def move(self, coordinates):
+        return coordinates + generate_random_vector(self.amplitude[0], self.amplitude[1])
+
+
+def get_rotation_matrix(rotationVector, angle):
+    angle = float(angle)
+    axis = rotationVector / np.sqrt(np.dot(rotationVector, rotationVector))
+    a = np.cos(angle / 2)
+    b, c, d = -axis * np.sin(angle / 2.)
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a + c * c - b * b - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a + d * d - b * b - c * c]], dtype=FLOAT_TYPE)
+
+
+class RotationGenerator:
+    """Rotate the group about its centre by a random axis and an angle below
+    ``amplitude`` degrees."""
+
+    native_kind = None
+
+    def __init__(self, amplitude=2):
+        a = float(amplitude)
+        if not 0 < a < 360:
+            raise ValueError("rotation amplitude must be in (0, 360) degrees")
+        self.amplitude = FLOAT_TYPE(PI * a / 180.)
+
+    def move(self, coordinates):
+        if coordinates.shape[0] <= 1:
+            raise ValueError("rotation needs at least two atoms in the group")
+        n = 0
+        while n < PRECISION:
+            axis = 1 - 2 * np.random.random(3)
+            n = np.linalg.norm(axis)
+        axis /= n
+        angle = (1 - 2 * random.random()) * self.amplitude
+        R = get_rotation_matrix(axis, angle)
+        center = np.sum(coordinates, 0) / coordinates.shape[0]
+        rotated = coordinates - center
+        for i in range(rotated.shape[0]):
+            rotated[i, :] = np.dot(R, rotated[i, :])
+        return np.array(rotated + center, dtype=FLOAT_TYPE)
+
+
+class MoveGeneratorCombinator:
+    """Apply several generators in list order (no shuffling)."""
+
+    native_kind = None
+
+    def __init__(self, combination, shuffle=False):
+        if shuffle:
+            raise NotImplementedError("shuffled combinations consume random.shuffle draws; not supported")
+        self.combination = list(combination)
+
+    def move(self, coordinates):
+        for g in self.combination:
+            coordinates = g.move(coordinates)
+        return coordinates

@@ -1,0 +1,85 @@
+// frmc_device.h -- device-side float32 arithmetic of the pair path.
+//
+// Every helper reproduces the IEEE binary32 expression tree of the
+// reference Cython loops (left-to-right, round-to-nearest-even, no fused
+// multiply-add: the library is compiled with -ffp-contract=off), so a GPU
+// distance or bin is bit-identical to the one the reference computes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace frmc {
+
+// pairs_distances.pyx:31-32: x>0 ? floor(x+0.5) : ceil(x-0.5).  floor/ceil of
+// a float are exact, so the float versions equal libm's double path.
+__device__ __forceinline__ float ref_round(float x) {
+    return x > 0.0f ? floorf(x + 0.5f) : ceilf(x - 0.5f);
+}
+
+struct Basis {
+    float b[9];  // row-major 3x3, b[3*r+c] = basis[r,c]
+};
+
+// Minimum-image real-space difference for a fractional difference
+// (pairs_distances.pyx:373-382): b = d - round(d); r_c = (bx*B0c + by*B1c) + bz*B2c.
+__device__ __forceinline__ void pbc_real(float dx, float dy, float dz, const Basis &B, float &rx, float &ry,
+                                         float &rz) {
+    float bx = dx - ref_round(dx);
+    float by = dy - ref_round(dy);
+    float bz = dz - ref_round(dz);
+    rx = bx * B.b[0] + by * B.b[3] + bz * B.b[6];
+    ry = bx * B.b[1] + by * B.b[4] + bz * B.b[7];
+    rz = bx * B.b[2] + by * B.b[5] + bz * B.b[8];
+}
+
+// Orthorhombic specialisation: with zero off-diagonal basis entries the two
+// extra terms of each component are +-0 and leave a non-zero product
+// unchanged (a zero product only changes sign, which squaring removes), so
+// the squared norm below is bit-identical to the general form.
+__device__ __forceinline__ void pbc_real_ortho(float dx, float dy, float dz, const Basis &B, float &rx,
+                                               float &ry, float &rz) {
+    float bx = dx - ref_round(dx);
+    float by = dy - ref_round(dy);
+    float bz = dz - ref_round(dz);
+    rx = bx * B.b[0];
+    ry = by * B.b[4];
+    rz = bz * B.b[8];
+}
+
+// ((rx*rx + ry*ry) + rz*rz), the float argument of the reference's sqrt
+__device__ __forceinline__ float norm2(float rx, float ry, float rz) {
+    return rx * rx + ry * ry + rz * rz;
+}
+
+// (float)sqrt((double)s) == correctly rounded float sqrt.  The library is
+// built with correctly rounded fp32 divide/sqrt (hipcc default), which
+// __builtin_sqrtf honours; a device known-answer test checks it.
+__device__ __forceinline__ float cr_sqrt(float s) { return __builtin_sqrtf(s); }
+__device__ __forceinline__ float cr_div(float a, float b) { return a / b; }
+
+// Histogram grid for one constraint.  s_lo/s_hi are the exact squared-norm
+// thresholds equivalent to the reference's distance tests
+// (d<minD rejected  <=>  s < s_lo;  d>=maxD rejected  <=>  s >= s_hi), both
+// computed on the host from the monotonicity of correctly rounded sqrt.
+struct Grid {
+    float minD, maxD, bin;
+    float s_lo, s_hi;
+    int32_t H;
+};
+
+// Bin index of a squared norm already known to be inside [s_lo, s_hi):
+// pairs_histograms.pyx:64  binIndex = (int)((distance-minDistance)/bin)
+__device__ __forceinline__ int32_t bin_of(float s, const Grid &g) {
+    float d = cr_sqrt(s);
+    return (int32_t)cr_div(d - g.minD, g.bin);
+}
+
+__device__ __forceinline__ bool in_range(float s, const Grid &g) { return s >= g.s_lo && s < g.s_hi; }
+
+struct HistArgs {
+    Grid g;
+    int32_t nel;
+    int64_t size;  // nel*nel*H
+};
+
+}  // namespace frmc

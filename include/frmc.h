@@ -1,0 +1,270 @@
+/*
+ * frmc.h -- C ABI of libfrmc_hip.so, the gfx950 implementation of fullrmc's
+ * pair-distance / pair-histogram hot path.
+ *
+ * Conventions
+ *  - Every array argument is a DEVICE pointer (HBM), C-ordered, float32 or
+ *    int32 as in the reference (fullrmc Globals.py:29-30).  The caller owns
+ *    all memory; nothing here keeps a pointer past the call except the
+ *    frmc_engine_* handle, which owns its state.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).  All
+ *    entry points are asynchronous on that stream unless stated otherwise.
+ *  - Return value: 0 on success, FRMC_E* on failure; frmc_last_error()
+ *    gives the message (thread-local).  Nothing throws across the ABI.
+ *  - Histogram outputs are (nel, nel, histSize) float32 arrays holding
+ *    integer counts, exactly as the reference returns them.
+ *
+ * Each entry point cites the reference interface it replaces
+ * (paths relative to the fullrmc source tree).
+ */
+#ifndef FRMC_H
+#define FRMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRMC_OK 0
+#define FRMC_EINVAL 1   /* bad argument (shape, size, index) */
+#define FRMC_EHIP 2     /* HIP runtime error */
+#define FRMC_ENOMEM 3   /* device allocation failed */
+#define FRMC_ESTATE 4   /* engine handle misuse */
+
+const char *frmc_last_error(void);
+int frmc_version(void);
+/* number of pair bins that fell outside the histogram array in the last
+   histogram call on this thread (the reference writes those out of bounds;
+   see DESIGN.md "bin index quirk"). */
+int64_t frmc_last_overflow(void);
+
+/* ------------------------------------------------------------------------
+ * pairs_distances  (Extensions/pairs_distances.pyx)
+ * ---------------------------------------------------------------------- */
+
+/* pairs_distances_to_indexcoords (pairs_distances.pyx:874-916).
+   out[i] = |coords[atomIndex]-coords[i]| (PBC: fractional minimum image,
+   then basis).  With allAtoms==0 only out[atomIndex:n] is written. */
+int frmc_pairs_distances_to_indexcoords(int32_t atomIndex, const float *coords, int32_t n,
+                                        const float *basis, int isPBC, int allAtoms, float *out,
+                                        void *stream);
+
+/* pairs_distances_to_multi_indexcoords (pairs_distances.pyx:975-1021).
+   out is (n, k): column c holds the distances of atom indexes[c]. */
+int frmc_pairs_distances_to_multi_indexcoords(const int32_t *indexes, int32_t k, const float *coords,
+                                              int32_t n, const float *basis, int isPBC, int allAtoms,
+                                              float *out, void *stream);
+
+/* pairs_distances_to_point (pairs_distances.pyx:827-871); point is (3,) */
+int frmc_pairs_distances_to_point(const float *point, const float *coords, int32_t n, const float *basis,
+                                  int isPBC, float *out, void *stream);
+
+/* pairs_distances_to_multi_points (pairs_distances.pyx:924-972).  points is
+   (3, k) -- the reference iterates points[:,i] -- and out is (n, k). */
+int frmc_pairs_distances_to_multi_points(const float *points, int32_t k, const float *coords, int32_t n,
+                                         const float *basis, int isPBC, float *out, void *stream);
+
+/* point_to_point_distance (pairs_distances.pyx:786-824); out is (1,) */
+int frmc_point_to_point_distance(const float *point1, const float *point2, const float *basis, int isPBC,
+                                 float *out, void *stream);
+
+/* pairs_differences_to_indexcoords (pairs_distances.pyx:626-672); out (n,3) */
+int frmc_pairs_differences_to_indexcoords(int32_t atomIndex, const float *coords, int32_t n,
+                                          const float *basis, int isPBC, int allAtoms, float *out,
+                                          void *stream);
+
+/* pairs_differences_to_multi_indexcoords (pairs_distances.pyx:727-783); out (n,3,k) */
+int frmc_pairs_differences_to_multi_indexcoords(const int32_t *indexes, int32_t k, const float *coords,
+                                                int32_t n, const float *basis, int isPBC, int allAtoms,
+                                                float *out, void *stream);
+
+/* pairs_differences_to_point (pairs_distances.pyx:580-623); out (n,3) */
+int frmc_pairs_differences_to_point(const float *point, const float *coords, int32_t n, const float *basis,
+                                    int isPBC, float *out, void *stream);
+
+/* pairs_differences_to_multi_points (pairs_distances.pyx:675-724); points (3,k), out (n,3,k) */
+int frmc_pairs_differences_to_multi_points(const float *points, int32_t k, const float *coords, int32_t n,
+                                           const float *basis, int isPBC, float *out, void *stream);
+
+/* from_to_points_differences (pairs_distances.pyx:483-527); out (n,3) = BC(to-from) */
+int frmc_from_to_points_differences(const float *pointsFrom, const float *pointsTo, int32_t n,
+                                    const float *basis, int isPBC, float *out, void *stream);
+
+/* pair_difference_to_point (pairs_distances.pyx:534-577); out (3,) = BC(point2-point1) */
+int frmc_pair_difference_to_point(const float *point1, const float *point2, const float *basis, int isPBC,
+                                  float *out, void *stream);
+
+/* ------------------------------------------------------------------------
+ * pairs_histograms  (Extensions/pairs_histograms.pyx)
+ * ---------------------------------------------------------------------- */
+
+/* single_pairs_histograms (pairs_histograms.pyx:77-141).  Adds the counts of
+   atom atomIndex vs distances[start:n] into hintra/hinter IN PLACE
+   (start = 0 if allAtoms else atomIndex). */
+int frmc_single_pairs_histograms(int32_t atomIndex, const float *distances, int32_t n,
+                                 const int32_t *moleculeIndex, const int32_t *elementIndex,
+                                 float *hintra, float *hinter, int32_t numberOfElements, int32_t histSize,
+                                 float minDistance, float maxDistance, float bin, int allAtoms, void *stream);
+
+/* multiple_pairs_histograms_coords (pairs_histograms.pyx:150-217).  Writes
+   (overwrites) hintra/hinter with the counts of every atom in indexes vs all
+   atoms (allAtoms) or vs the atoms after it (allAtoms==0). */
+int frmc_multiple_pairs_histograms_coords(const int32_t *indexes, int32_t nidx, const float *boxCoords,
+                                          int32_t n, const float *basis, int isPBC,
+                                          const int32_t *moleculeIndex, const int32_t *elementIndex,
+                                          int32_t numberOfElements, float minDistance, float maxDistance,
+                                          float bin, int32_t histSize, int allAtoms, float *hintra,
+                                          float *hinter, void *stream);
+
+/* multiple_pairs_histograms_dists (pairs_histograms.pyx:225-281).
+   distances is (n, k); column c belongs to atom indexes[c] (c < nidx <= k). */
+int frmc_multiple_pairs_histograms_dists(const int32_t *indexes, int32_t nidx, const float *distances,
+                                         int32_t n, int32_t k, const int32_t *moleculeIndex,
+                                         const int32_t *elementIndex, int32_t numberOfElements,
+                                         float minDistance, float maxDistance, float bin, int32_t histSize,
+                                         int allAtoms, float *hintra, float *hinter, void *stream);
+
+/* full_pairs_histograms_coords (pairs_histograms.pyx:289-335): every
+   unordered pair once, in slot [elem(earlier), elem(later)].  Tiled
+   all-pairs kernel with LDS-private bins. */
+int frmc_full_pairs_histograms_coords(const float *boxCoords, int32_t n, const float *basis, int isPBC,
+                                      const int32_t *moleculeIndex, const int32_t *elementIndex,
+                                      int32_t numberOfElements, float minDistance, float maxDistance,
+                                      float bin, int32_t histSize, float *hintra, float *hinter,
+                                      void *stream);
+
+/* full_pairs_histograms_dists (pairs_histograms.pyx:343-383): distances (n,k),
+   indexes = arange(k), allAtoms false. */
+int frmc_full_pairs_histograms_dists(const float *distances, int32_t n, int32_t k,
+                                     const int32_t *moleculeIndex, const int32_t *elementIndex,
+                                     int32_t numberOfElements, float minDistance, float maxDistance, float bin,
+                                     int32_t histSize, float *hintra, float *hinter, void *stream);
+
+/* Same as frmc_full_pairs_histograms_coords but leaves int32 counts in
+   counts[2][nel][nel][H] (intra then inter) without the float conversion;
+   used by the device engine and by the benchmark. */
+int frmc_full_pairs_counts(const float *boxCoords, int32_t n, const float *basis, int isPBC,
+                           const int32_t *moleculeIndex, const int32_t *elementIndex, int32_t numberOfElements,
+                           float minDistance, float maxDistance, float bin, int32_t histSize, int32_t *counts,
+                           void *stream);
+
+/* ------------------------------------------------------------------------
+ * boundary_conditions_collection (Extensions/boundary_conditions_collection.pyx)
+ * ---------------------------------------------------------------------- */
+
+/* get_reciprocal_basis (boundary_conditions_collection.pyx:45-80), including
+   the basis[0,1] cofactor typo of :59.  rbasis (3,3) float32 device; vol is
+   a host double (the reference returns a Python float).  Synchronous. */
+int frmc_get_reciprocal_basis(const float *basis, float *rbasis, double *vol, void *stream);
+
+/* transform_coordinates (boundary_conditions_collection.pyx:88-110) */
+int frmc_transform_coordinates(const float *transMatrix, const float *coords, int32_t n, float *out,
+                               void *stream);
+
+/* box_coordinates_real_distances (boundary_conditions_collection.pyx:119-165) */
+int frmc_box_coordinates_real_distances(int32_t atomIndex, const int32_t *indexes, int32_t k,
+                                        const float *boxCoords, int32_t n, const float *basis, float *out,
+                                        void *stream);
+
+/* ------------------------------------------------------------------------
+ * Device-resident engine: the Engine.run step (Engine.py:3166-3344) with the
+ * constraint chain of PairDistributionConstraint / StructureFactorConstraint
+ * (PairDistributionConstraints.py:847-895, 1044-1166;
+ * StructureFactorConstraints.py:772-834, 1011-1096) on the GPU, evaluating
+ * k proposals per window.  Descriptor arrays are HOST pointers, copied at
+ * creation.  No reference counterpart as a single call: the reference
+ * reaches the same results through per-move Python calls.
+ * ---------------------------------------------------------------------- */
+
+#define FRMC_PDC 0   /* model = G(r) on the histogram grid */
+#define FRMC_SFC 1   /* model = S(Q) = sum_r G(r) M[r,q] + 1 */
+
+typedef struct {
+    int32_t kind;               /* FRMC_PDC or FRMC_SFC */
+    int32_t histSize;           /* H */
+    float minDistance, maxDistance, bin;
+    int32_t nQ;                 /* SFC: number of Q points */
+    const float *pairWeights;   /* [T] w_ij, T = nel(nel+1)/2 pairs in sorted-element order */
+    const float *pairDensity;   /* [T] D_ij = N_ij / V as float32 */
+    const float *shellVolumes;  /* [H] */
+    const float *prefactor;     /* [H] (4*PI*shellCenters)*rho0 as float32 */
+    const float *experimental;  /* [H] (PDC) or [nQ] (SFC) */
+    const float *dataWeights;   /* same length or NULL */
+    const float *sqMatrix;      /* SFC: [H][nQ] dr*sin(r*Q)/Q as float32 */
+    float scaleFactor;          /* constant scale factor (1 = none) */
+} frmc_constraint_desc;
+
+typedef struct {
+    int32_t n, nel, isPBC;
+    const float *box;            /* (n,3) box (fractional) coordinates */
+    const float *real;           /* (n,3) real coordinates, or NULL (needed by the native generator) */
+    const int32_t *mol, *elem;   /* (n,) molecule and element index */
+    float basis[9], recip[9];    /* basis vectors (rows) and reciprocal basis */
+    int32_t ngroups;
+    const int32_t *group_offsets; /* [ngroups+1] CSR offsets into group_atoms */
+    const int32_t *group_atoms;   /* atom indexes of every group, in group order */
+    int32_t nconstraints;
+    frmc_constraint_desc constraints[4];
+    int32_t kmax;                /* largest window (proposals per evaluation) */
+    void *stream;                /* NULL: the engine creates its own stream */
+} frmc_engine_desc;
+
+typedef struct frmc_engine frmc_engine;
+
+/* MT19937 state in the layout CPython's random.getstate() and NumPy's
+   RandomState.get_state() expose: 624 words and the read position. */
+typedef struct {
+    uint32_t mt[624];
+    int32_t pos;
+} frmc_mt19937;
+
+typedef struct {
+    int64_t generated, tried, accepted, tolerated;
+    int64_t windows, evaluated;  /* windows launched, proposals evaluated */
+} frmc_run_stats;
+
+int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *desc);
+int frmc_engine_destroy(frmc_engine *e);
+/* full O(N^2) histograms of every constraint, then chi-squares and total
+   (Engine.initialize_used_constraints -> compute_data, Engine.py:3084-3119) */
+int frmc_engine_compute_data(frmc_engine *e, float *chi2_out, float *tse_out);
+/* chi-squares and total of the current histograms */
+int frmc_engine_refresh(frmc_engine *e, float *chi2_out, float *tse_out);
+/* evaluate k proposals: group_ids[k], new_box = moved box coordinates of
+   each proposal's group atoms, concatenated in proposal order.  Writes the
+   total chi-square after each move (tse_out[k]) and the per-constraint
+   chi-squares (chi2_out[k][C], may be NULL).  Synchronous. */
+int frmc_engine_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box, float *tse_out,
+                     float *chi2_out);
+/* accept proposal p of the last evaluation (accept_move, PDC:1131-1152) */
+int frmc_engine_commit(frmc_engine *e, int32_t p);
+int frmc_engine_sync(frmc_engine *e);
+/* read back coordinates, slot-exact counts [2][nel][nel][H]+1 and pair-type
+   sums [T][H] of one constraint, current chi-squares and total (host ptrs) */
+int frmc_engine_get_state(frmc_engine *e, float *box_out, int32_t constraint, int32_t *slot_out, int32_t *base_out,
+                          float *chi2_out, float *tse_out);
+/* Native run loop for single-group translation moves
+   (RandomSelector + TranslationGenerator, Selectors/RandomSelectors.py:87-94,
+   Generators/Translations.py:171-187, Core/Collection.py:674-701): proposals
+   come from the two MT19937 streams (py = random, np = numpy.random), k per
+   window, evaluated speculatively; the first accepted one is committed and
+   both streams are rewound to just after it, so the trajectory equals the
+   sequential one.  Optional logs (length nsteps): the group index of each
+   generated move, whether it was accepted, and the total after the step. */
+int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float minAmp, float maxAmp,
+                                float tolerance, frmc_mt19937 *py_state, frmc_mt19937 *np_state,
+                                frmc_run_stats *stats, int32_t *group_log, uint8_t *accept_log, float *tse_log);
+/* replace the histograms of one constraint with slot counts
+   [2][nel][nel][H]+1 (host), e.g. a constraint's current data */
+int frmc_engine_set_counts(frmc_engine *e, int32_t constraint, const int32_t *slot_in);
+/* replace / read the host mirror of real coordinates (n,3) the native
+   generator moves from */
+int frmc_engine_set_real(frmc_engine *e, const float *real_in);
+int frmc_engine_get_real(frmc_engine *e, float *real_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRMC_H */

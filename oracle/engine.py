@@ -1,0 +1,226 @@
+"""ORACLE / TEST INFRASTRUCTURE ONLY -- never imported by the product path.
+
+Sequential restatement of fullrmc's Engine.run step for the configurations
+of this path, on the CPU: group selection, move generation, constraint
+before/after histograms through the C restatement (refcpu), the NumPy
+constraint chain (chain.py), the accept rule, and accept/reject updates.
+
+  step loop           Engine.py:3458-3516, 3166-3228, 3278-3344
+  RandomSelector      Selectors/RandomSelectors.py:87-94
+  Translation move    Generators/Translations.py:171-187, Core/Collection.py:674-701
+  Rotation move       Generators/Rotations.py:141-178, Core/Collection.py:437-456
+  Combinator          Core/MoveGenerator.py:792-811 (no shuffle)
+  PDC / SFC update    PairDistributionConstraints.py:1044-1166,
+                      StructureFactorConstraints.py:975-1096
+"""
+import random
+
+import numpy as np
+
+from . import chain
+from . import refcpu
+
+F32 = np.float32
+PRECISION = F32(1e-5)
+
+
+def random_vector(minAmp, maxAmp):
+    vector = np.array(1 - 2 * np.random.random(3), dtype=F32)
+    norm = np.linalg.norm(vector)
+    while norm == 0:
+        vector = np.array(1 - 2 * np.random.random(3), dtype=F32)
+        norm = np.linalg.norm(vector)
+    vector /= F32(norm)
+    base = F32(vector * minAmp)
+    maxAmp = F32(maxAmp - minAmp)
+    vector *= F32(random.random() * maxAmp)
+    vector += base
+    return vector
+
+
+class Translation:
+    def __init__(self, amplitude):
+        self.amp = (F32(0), F32(float(amplitude)))
+
+    def move(self, coords):
+        return coords + random_vector(self.amp[0], self.amp[1])
+
+
+class Rotation:
+    def __init__(self, amplitude_deg):
+        self.amp = F32(chain.PI * float(amplitude_deg) / 180.)
+
+    def move(self, coords):
+        n = 0
+        while n < PRECISION:
+            axis = 1 - 2 * np.random.random(3)
+            n = np.linalg.norm(axis)
+        axis /= n
+        angle = (1 - 2 * random.random()) * self.amp
+        angle = float(angle)
+        ax = axis / np.sqrt(np.dot(axis, axis))
+        a = np.cos(angle / 2)
+        b, c, d = -ax * np.sin(angle / 2.)
+        R = np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                      [2 * (b * c + a * d), a * a + c * c - b * b - d * d, 2 * (c * d - a * b)],
+                      [2 * (b * d - a * c), 2 * (c * d + a * b), a * a + d * d - b * b - c * c]], dtype=F32)
+        center = np.sum(coords, 0) / coords.shape[0]
+        rot = coords - center
+        for i in range(rot.shape[0]):
+            rot[i, :] = np.dot(R, rot[i, :])
+        return np.array(rot + center, dtype=F32)
+
+
+class Combined:
+    def __init__(self, generators):
+        self.generators = generators
+
+    def move(self, coords):
+        for g in self.generators:
+            coords = g.move(coords)
+        return coords
+
+
+class System:
+    """Box, species and molecules: the engine attributes the constraints read."""
+
+    def __init__(self, real, elements_of_atoms, molecules, basis, recip):
+        self.real = np.array(real, dtype=F32)
+        self.basis = np.array(basis, dtype=F32)
+        self.recip = np.array(recip, dtype=F32)
+        self.box = refcpu.transform_coordinates(self.recip, self.real)
+        self.allElements = list(elements_of_atoms)
+        self.elements = sorted(set(self.allElements))
+        lut = {e: i for i, e in enumerate(self.elements)}
+        self.elem = np.array([lut[e] for e in self.allElements], dtype=np.int32)
+        self.mol = np.array(molecules, dtype=np.int32)
+        self.counts = {}
+        for e in self.allElements:
+            self.counts[e] = self.counts.get(e, 0) + 1
+        self.volume = F32(abs(np.linalg.det(self.basis.astype(np.float64))))
+        self.rho0 = F32(len(self.real)) / F32(self.volume)
+        self.nel = len(self.elements)
+
+
+class PDC:
+    """PairDistributionConstraint on the oracle path."""
+
+    def __init__(self, sysm, experimentalData, weights, scaleFactor=1.0, dataWeights=None):
+        self.s = sysm
+        self.exp = np.asarray(experimentalData, dtype=F32)
+        self.bin, self.min, self.max, self.centers, self.H, self.vols = chain.pdc_grid(self.exp)
+        self.y = self.exp[:, 1]
+        self.w = weights
+        self.sf = scaleFactor
+        self.dw = dataWeights
+
+    def model(self, intra, inter):
+        s = self.s
+        return chain.total_gr(intra, inter, s.elements, s.counts, self.w, s.volume, s.rho0, self.vols,
+                              self.centers, self.sf)
+
+    def hist_full(self, box, mol, elem):
+        return refcpu.full_pairs_histograms_coords(box, self.s.basis, True, mol, elem, self.s.nel, self.min,
+                                                   self.max, self.bin, self.H)
+
+    def hist_multi(self, idx, box):
+        return refcpu.multiple_pairs_histograms_coords(idx, box, self.s.basis, True, self.s.mol, self.s.elem,
+                                                       self.s.nel, self.min, self.max, self.bin, self.H,
+                                                       allAtoms=True)
+
+    def chi2(self, intra, inter):
+        return chain.chi_square(self.y, self.model(intra, inter), self.dw)
+
+    # reference state machine
+    def compute_data(self):
+        self.intra, self.inter = self.hist_full(self.s.box, self.s.mol, self.s.elem)
+        self.sd = self.chi2(self.intra, self.inter)
+
+    def _group(self, idx, box):
+        iM, eM = self.hist_multi(idx, box)
+        iF, eF = refcpu.full_pairs_histograms_coords(box[idx], self.s.basis, True, self.s.mol[idx],
+                                                     self.s.elem[idx], self.s.nel, self.min, self.max, self.bin,
+                                                     self.H)
+        return iM - iF, eM - eF
+
+    def before(self, idx):
+        self.bi, self.be = self._group(idx, self.s.box)
+
+    def after(self, idx, moved_box):
+        saved = np.array(self.s.box[idx], dtype=F32)
+        self.s.box[idx] = moved_box
+        self.ai, self.ae = self._group(idx, self.s.box)
+        self.s.box[idx] = saved
+        self.new_i = self.intra - self.bi + self.ai
+        self.new_e = self.inter - self.be + self.ae
+        self.after_sd = self.chi2(self.new_i, self.new_e)
+
+    def accept(self):
+        self.intra, self.inter, self.sd = self.new_i, self.new_e, self.after_sd
+
+
+class SFC(PDC):
+    """StructureFactorConstraint on the oracle path."""
+
+    def __init__(self, sysm, experimentalData, weights, rmin, rmax, dr, scaleFactor=1.0, dataWeights=None):
+        self.s = sysm
+        self.exp = np.asarray(experimentalData, dtype=F32)
+        self.q = self.exp[:, 0].astype(F32)
+        self.y = self.exp[:, 1].astype(F32)
+        self.min, self.max, self.bin, self.centers, self.H, self.vols = chain.sfc_grid(rmin, rmax, dr)
+        self.M = chain.sq_matrix(self.centers, self.q)
+        self.w = weights
+        self.sf = scaleFactor
+        self.dw = dataWeights
+
+    def model(self, intra, inter):
+        s = self.s
+        return chain.total_sq(intra, inter, s.elements, s.counts, self.w, s.volume, s.rho0, self.vols,
+                              self.centers, self.M, self.sf)
+
+
+class Engine:
+    def __init__(self, sysm, groups, generator, constraints, tolerance=0.0):
+        self.s = sysm
+        self.groups = [np.array(g, dtype=np.int32) for g in groups]
+        self.generator = generator
+        self.constraints = constraints
+        self.tolerance = F32(F32(tolerance) / 100.)
+        self.generated = self.tried = self.accepted = self.tolerated = 0
+
+    def initialize(self):
+        for c in self.constraints:
+            c.compute_data()
+        self.tse = chain.total_error([c.sd for c in self.constraints])
+
+    def run(self, nsteps, log=None):
+        s = self.s
+        for _ in range(nsteps):
+            self.generated += 1
+            gi = np.int32(random.randint(0, len(self.groups) - 1))
+            idx = self.groups[gi]
+            before = np.array(s.real[idx], dtype=F32)
+            moved_real = self.generator.move(before)
+            moved_box = refcpu.transform_coordinates(s.recip, moved_real)
+            self.tried += 1
+            for c in self.constraints:
+                c.before(idx)
+                c.after(idx, moved_box)
+            new = chain.total_error([c.after_sd for c in self.constraints])
+            reject = False
+            if new > self.tse:
+                if random.random() > self.tolerance:
+                    reject = True
+                else:
+                    self.tolerated += 1
+                    self.tse = new
+            else:
+                self.tse = new
+            if not reject:
+                self.accepted += 1
+                for c in self.constraints:
+                    c.accept()
+                s.real[idx] = moved_real
+                s.box[idx] = moved_box
+            if log is not None:
+                log.append((int(gi), not reject, float(self.tse)))

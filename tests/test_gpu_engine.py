@@ -1,0 +1,212 @@
+"""GPU parity of the run loop: the device-resident speculative engine and the
+per-move drop-in loop reproduce the sequential oracle engine's trajectory
+(accepted-move sequence, totals after every step), its slot-exact
+histograms, chi-squares, coordinates and both random streams."""
+import random
+
+import numpy as np
+import pytest
+
+from fullrmc_amd import systems
+from fullrmc_amd.Constraints import PairDistributionConstraint, StructureFactorConstraint
+from fullrmc_amd.Engine import Engine
+from fullrmc_amd.Generators import MoveGeneratorCombinator, RotationGenerator, TranslationGenerator
+from oracle import chain, refcpu
+from oracle import engine as oeng
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+WEIGHTS = {"O": 8, "Si": 14, "Fe": 26, "Al": 13, "C": 6, "H": 1, "Ar": 18}
+
+
+def element_index(elements):
+    lut = {e: i for i, e in enumerate(sorted(set(elements)))}
+    return np.array([lut[e] for e in elements], dtype=np.int32)
+
+
+def counts_of(elements):
+    c = {}
+    for e in elements:
+        c[e] = c.get(e, 0) + 1
+    return c
+
+
+def pdc_target(s, sigma, seed, r_first, dr, nbins):
+    """experimentalData of a jittered copy of s (G(r) by the oracle chain)."""
+    t = systems.jitter(s, sigma, seed)
+    x = (r_first + dr * np.arange(nbins)).astype(F32)
+    exp = np.stack([x, np.zeros_like(x)], 1).astype(F32)
+    osys = oeng.System(t["real"], s["elements"], s["molecules"], s["basis"], s["recip"])
+    w = chain.normalized_weighting(osys.counts, {e: WEIGHTS[e] for e in osys.elements})
+    p = oeng.PDC(osys, exp, w)
+    hi, he = p.hist_full(osys.box, osys.mol, osys.elem)
+    exp[:, 1] = p.model(hi, he)
+    return exp
+
+
+def sfc_target(s, sigma, seed, q, rmin, rmax, dr):
+    t = systems.jitter(s, sigma, seed)
+    osys = oeng.System(t["real"], s["elements"], s["molecules"], s["basis"], s["recip"])
+    w = chain.normalized_weighting(osys.counts, {e: WEIGHTS[e] for e in osys.elements})
+    exp = np.stack([q, np.zeros_like(q)], 1).astype(F32)
+    c = oeng.SFC(osys, exp, w, rmin, rmax, dr)
+    hi, he = c.hist_full(osys.box, osys.mol, osys.elem)
+    exp[:, 1] = c.model(hi, he)
+    return exp
+
+
+def oracle_run(s, groups, gen, cons_fn, nsteps, seed, tolerance=0.0):
+    random.seed(seed)
+    np.random.seed(seed)
+    osys = oeng.System(s["real"], s["elements"], s["molecules"], s["basis"], s["recip"])
+    w = chain.normalized_weighting(osys.counts, {e: WEIGHTS[e] for e in osys.elements})
+    oe = oeng.Engine(osys, groups, gen, cons_fn(osys, w), tolerance)
+    oe.initialize()
+    log = []
+    oe.run(nsteps, log)
+    return oe, log, random.getstate(), np.random.get_state()
+
+
+def device_engine(s, cons, groups=None, gen=None):
+    E = Engine()
+    E.set_system(s["real"], s["elements"], s["molecules"], s["basis"], s["recip"])
+    if groups is not None:
+        E.set_groups(groups, gen)
+    E.add_constraints(cons)
+    return E
+
+
+def assert_same_state(E, oe, dlog, olog, states):
+    assert len(dlog) == len(olog)
+    first = next((i for i, (a, b) in enumerate(zip(dlog, olog)) if a != b), None)
+    assert first is None, "trajectories diverge at step %d: %r vs %r" % (first, dlog[first], olog[first])
+    assert np.float32(E.totalStandardError).tobytes() == np.float32(oe.tse).tobytes()
+    for c, oc in zip(E.constraints, oe.constraints):
+        assert c.data["intra"].tobytes() == oc.intra.tobytes()
+        assert c.data["inter"].tobytes() == oc.inter.tobytes()
+        assert np.float32(c.standardError).tobytes() == np.float32(oc.sd).tobytes()
+    assert E.realCoordinates.tobytes() == oe.s.real.tobytes()
+    assert E.boxCoordinates.tobytes() == oe.s.box.tobytes()
+    pys, nps = states
+    assert random.getstate() == pys
+    d_np = np.random.get_state()
+    assert np.array_equal(d_np[1], nps[1]) and d_np[2] == nps[2]
+
+
+@pytest.fixture(scope="module")
+def single_box():
+    s = systems.random_box(500, 18.0, 2.0, seed=21)
+    s["exp"] = pdc_target(s, 0.1, 22, 0.025, 0.05, 180)
+    return s
+
+
+def _pdc_cons(s):
+    return lambda osys, w: [oeng.PDC(osys, s["exp"], w)]
+
+
+@pytest.mark.parametrize("k", [1, 7, 64])
+def test_native_translation_matches_oracle(single_box, k):
+    s = single_box
+    n = len(s["real"])
+    nsteps = 400
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], oeng.Translation(0.2), _pdc_cons(s), nsteps, 5)
+    random.seed(5)
+    np.random.seed(5)
+    E = device_engine(s, [PairDistributionConstraint(s["exp"])])
+    dlog = []
+    st = E.run_device(nsteps, k=k, log=dlog)
+    assert st["generated"] == nsteps
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    assert 0 < sum(a for _, a, _ in dlog) < nsteps
+
+
+def test_native_translation_with_tolerance(single_box):
+    s = single_box
+    n = len(s["real"])
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], oeng.Translation(0.3), _pdc_cons(s), 300, 9,
+                                    tolerance=30.0)
+    random.seed(9)
+    np.random.seed(9)
+    E = device_engine(s, [PairDistributionConstraint(s["exp"])])
+    E.set_move_generator(TranslationGenerator(0.3))
+    E.set_tolerance(30.0)
+    dlog = []
+    E.run_device(300, k=32, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    assert E.tolerated > 0
+
+
+def test_plain_dropin_loop_matches_oracle(single_box):
+    s = single_box
+    n = len(s["real"])
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], oeng.Translation(0.2), _pdc_cons(s), 60, 3)
+    random.seed(3)
+    np.random.seed(3)
+    E = device_engine(s, [PairDistributionConstraint(s["exp"])])
+    dlog = []
+    E.run(60, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+
+
+def test_pdc_plus_sfc_binary_alloy():
+    s = systems.b2_alloy(cells=(6, 6, 6), a=3.01, sigma=0.08, seed=2)
+    pdc_exp = pdc_target(s, 0.08, 3, 0.01, 0.02, 400)
+    q = np.linspace(0.5, 25, 120).astype(F32)
+    sfc_exp = sfc_target(s, 0.08, 3, q, 0.0, 8.0, 0.02)
+    n = len(s["real"])
+
+    def cons(osys, w):
+        return [oeng.PDC(osys, pdc_exp, w), oeng.SFC(osys, sfc_exp, w, 0.0, 8.0, 0.02)]
+
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], oeng.Translation(0.1), cons, 300, 4)
+    random.seed(4)
+    np.random.seed(4)
+    E = device_engine(s, [PairDistributionConstraint(pdc_exp),
+                          StructureFactorConstraint(sfc_exp, rmin=0.0, rmax=8.0, dr=0.02)])
+    E.set_move_generator(TranslationGenerator(0.1))
+    dlog = []
+    E.run_device(300, k=16, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+
+
+def test_molecular_triclinic_translation_plus_rotation(small_triclinic):
+    s = small_triclinic
+    exp = pdc_target(s, 0.05, 8, 0.025, 0.05, 140)
+    mols = {}
+    for i, m in enumerate(s["molecules"]):
+        mols.setdefault(int(m), []).append(i)
+    groups = list(mols.values())
+    ogen = oeng.Combined([oeng.Translation(0.2), oeng.Rotation(10)])
+    oe, olog, pys, nps = oracle_run(s, groups, ogen, lambda osys, w: [oeng.PDC(osys, exp, w)], 80, 6)
+    random.seed(6)
+    np.random.seed(6)
+    gen = MoveGeneratorCombinator([TranslationGenerator(0.2), RotationGenerator(10)])
+    E = device_engine(s, [PairDistributionConstraint(exp)], groups=groups, gen=gen)
+    dlog = []
+    E.run_device(80, k=8, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+
+
+def test_config1_size_properties():
+    """B2 alloy at full size (13,500 atoms, PDC + SFC): after a device run the
+    engine's histograms equal a fresh full recompute in the symmetrised sums
+    and its total equals the chain evaluated on that recompute."""
+    s = systems.b2_alloy(seed=7)
+    pdc_exp = np.stack([(0.01 + 0.02 * np.arange(1000)).astype(F32), np.zeros(1000, F32)], 1)
+    q = np.linspace(0.5, 25, 500).astype(F32)
+    sfc_exp = np.stack([q, np.ones_like(q)], 1).astype(F32)
+    random.seed(1)
+    np.random.seed(1)
+    E = device_engine(s, [PairDistributionConstraint(pdc_exp),
+                          StructureFactorConstraint(sfc_exp, rmin=0.0, rmax=20.0, dr=0.02)])
+    E.set_move_generator(TranslationGenerator(0.1))
+    st = E.run_device(2000, k=64)
+    assert st["generated"] == 2000
+    sym = lambda h: h + h.transpose(1, 0, 2)
+    chis = []
+    for c in E.constraints:
+        fi, fe = c._full()
+        assert np.array_equal(sym(c.data["intra"] + c.data["inter"]), sym(fi + fe))
+        chis.append(c.compute_standard_error(c._model({"intra": fi, "inter": fe})))
+        assert np.float32(chis[-1]).tobytes() == np.float32(c.standardError).tobytes()
+    assert np.float32(np.sum(chis)).tobytes() == np.float32(E.totalStandardError).tobytes()

@@ -1,0 +1,378 @@
+"""Pins of the CPU oracle (oracle/refcpu.c, oracle/chain.py, oracle/engine.py)
+against what fixes it independently of the restatement: float64 brute force,
+closed forms, invariants, NumPy / CPython themselves as semantics oracles,
+and known answers chosen so that a plausible restatement mistake (FMA
+contraction, rint instead of the reference's round, wrong operand order,
+dropped term, transposed basis, wrong slot) fails.
+
+The reference ships no tests, golden vectors or fixtures for this path
+(its tests/ directory is git-ignored); see tests/golden/ for the vectors
+committed here.
+"""
+import itertools
+import math
+import random
+
+import numpy as np
+import pytest
+
+from fullrmc_amd import systems
+from oracle import chain, refcpu
+
+F32 = np.float32
+
+
+# ------------------------------------------------------------------ MT19937
+
+@pytest.mark.parametrize("seed", [0, 1, 12345, 2 ** 32 - 1, 2 ** 40 + 7])
+def test_mt19937_matches_cpython_random(seed):
+    random.seed(seed)
+    m = refcpu.MT.from_python(random.getstate())
+    for n in (1, 2, 3, 7, 600, 2000, 13500, 200000):
+        assert m.randbelow(n) == random.randint(0, n - 1)
+        assert m.double() == random.random()
+    for _ in range(3000):  # crosses several twists
+        assert m.double() == random.random()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 12345, 2 ** 32 - 1])
+def test_mt19937_matches_numpy_legacy(seed):
+    np.random.seed(seed)
+    m = refcpu.MT.from_numpy(np.random.get_state())
+    for _ in range(2000):
+        v = np.random.random(3)
+        assert [m.double(), m.double(), m.double()] == list(v)
+
+
+def test_linalg_norm_rule():
+    """np.linalg.norm of a float32 3-vector == sqrt(float32(double sum of
+    float32 products)) -- the rule the native generator uses."""
+    rng = np.random.default_rng(0)
+    bad = 0
+    for _ in range(20000):
+        v = np.array(1 - 2 * rng.random(3), dtype=F32)
+        p = [float(F32(x * x)) for x in v]
+        acc = 0.0
+        for q in p:
+            acc += q
+        rule = np.sqrt(F32(acc))
+        bad += rule.tobytes() != np.linalg.norm(v).tobytes()
+    assert bad == 0
+
+
+# ------------------------------------------------------- NumPy semantics
+
+def _pairwise(a):
+    """NumPy's pairwise float32 summation, restated from its algorithm."""
+    n = len(a)
+    if n < 8:
+        r = F32(0)
+        for x in a:
+            r = F32(r + x)
+        return r
+    if n <= 128:
+        r = [F32(a[j]) for j in range(8)]
+        i = 8
+        while i < n - (n % 8):
+            for j in range(8):
+                r[j] = F32(r[j] + a[i + j])
+            i += 8
+        res = F32(F32(F32(r[0] + r[1]) + F32(r[2] + r[3])) + F32(F32(r[4] + r[5]) + F32(r[6] + r[7])))
+        while i < n:
+            res = F32(res + a[i])
+            i += 1
+        return res
+    n2 = n // 2
+    n2 -= n2 % 8
+    return F32(_pairwise(a[:n2]) + _pairwise(a[n2:]))
+
+
+@pytest.mark.parametrize("n", [1, 5, 8, 9, 127, 128, 129, 300, 500, 999, 1000, 1001, 2000, 2047])
+def test_pairwise_sum_structure(n):
+    rng = np.random.default_rng(n)
+    for _ in range(3):
+        a = (rng.standard_normal(n) ** 2 * rng.uniform(1e-3, 1e3)).astype(F32)
+        assert _pairwise(a).tobytes() == np.add.reduce(a).tobytes()
+
+
+@pytest.mark.parametrize("H,nQ", [(300, 7), (1000, 500), (2000, 1000)])
+def test_axis0_sum_is_sequential(H, nQ):
+    rng = np.random.default_rng(H)
+    P = rng.standard_normal((H, nQ)).astype(F32)
+    s = np.sum(P, axis=0)
+    seq = P[0].copy()
+    for r in range(1, H):
+        seq = (seq + P[r]).astype(F32)
+    assert s.tobytes() == seq.tobytes()
+
+
+def test_nep50_scalars():
+    x = F32(0.3)
+    assert type(0.123 * x) is np.float32 and (0.123 * x) == F32(F32(0.123) * x)
+    # random() > tolerance compares in float32
+    t = F32(0.1)
+    assert not (0.10000000149011613 > t)
+
+
+# ------------------------------------------------------------- distances
+
+def _min_image_f64(a, c, basis):
+    """float64 brute force over the 27 neighbouring images."""
+    B = basis.astype(np.float64)
+    d = (a.astype(np.float64) - c.astype(np.float64))
+    best = np.inf
+    for s in itertools.product((-1, 0, 1), repeat=3):
+        v = (d + np.array(s) - np.round(d)) @ B
+        best = min(best, float(np.sqrt(v @ v)))
+    return best
+
+
+def test_pbc_distance_vs_float64_brute_force_cubic(small_random):
+    s = small_random
+    d = refcpu.pairs_distances_to_indexcoords(17, s["box"], s["basis"], True)
+    for j in range(0, 600, 7):
+        ref = _min_image_f64(s["box"][17], s["box"][j], s["basis"])
+        assert abs(float(d[j]) - ref) <= 2e-6 * max(ref, 1.0)
+
+
+def test_pbc_distance_is_reference_fractional_rounding_in_triclinic(small_triclinic):
+    """In a skewed cell the reference wraps each fractional component
+    independently (not the true minimum image); the float64 evaluation of
+    that same rule agrees to float32 precision."""
+    s = small_triclinic
+    B = s["basis"].astype(np.float64)
+    d = refcpu.pairs_distances_to_indexcoords(3, s["box"], s["basis"], True)
+    a = s["box"][3].astype(np.float64)
+    for j in range(0, len(s["box"]), 5):
+        df = a - s["box"][j].astype(np.float64)
+        b = df - np.where(df > 0, np.floor(df + 0.5), np.ceil(df - 0.5))
+        v = b @ B
+        assert abs(float(d[j]) - math.sqrt(v @ v)) <= 3e-6 * max(math.sqrt(v @ v), 1.0)
+
+
+def test_distance_symmetry_and_self(small_triclinic):
+    s = small_triclinic
+    n = len(s["box"])
+    D = refcpu.pairs_distances_to_multi_indexcoords(np.arange(0, n, 13, dtype=np.int32), s["box"], s["basis"], True)
+    for c, i in enumerate(range(0, n, 13)):
+        assert D[i, c] == 0.0
+        for c2, j in enumerate(range(0, n, 13)):
+            assert D[j, c].tobytes() == D[i, c2].tobytes()  # bitwise symmetric
+
+
+def test_round_quirk_known_answer():
+    """round(0.49999997f) = floor(0.49999997f + 0.5f) = floor(1.0f) = 1, so a
+    fractional difference of 0.49999997 maps to -0.50000003: the distance is
+    0.50000003*L, not 0.49999997*L as rint/round-half-even would give."""
+    L = F32(10.0)
+    basis = np.diag([L, L, L]).astype(F32)
+    x = F32(0.49999997)
+    coords = np.array([[x, 0, 0], [0, 0, 0]], dtype=F32)
+    d = refcpu.pairs_distances_to_indexcoords(0, coords, basis, True)[1]
+    expect = F32(F32(x - F32(1.0)) * L)
+    assert d == abs(expect)
+    assert d != F32(x * L)
+
+
+def test_no_fma_known_answer():
+    """A squared norm whose FMA-contracted value differs from the separately
+    rounded one: the oracle must return the non-contracted result."""
+    rng = np.random.default_rng(7)
+    basis = np.diag([1.0, 1.0, 1.0]).astype(F32)
+    found = 0
+    for _ in range(2000):
+        p = rng.random((2, 3)).astype(F32) * F32(0.4)
+        d = p[0] - p[1]
+        sep = F32(F32(F32(d[0] * d[0]) + F32(d[1] * d[1])) + F32(d[2] * d[2]))
+        fma = F32(np.float64(F32(F32(d[0] * d[0]) + F32(d[1] * d[1]))) + np.float64(d[2]) * np.float64(d[2]))
+        got = refcpu.point_to_point_distance(p[0], p[1], basis, True)
+        assert got == F32(math.sqrt(float(sep)))
+        found += F32(math.sqrt(float(sep))) != F32(math.sqrt(float(fma)))
+    assert found > 0  # the test can tell the two apart
+
+
+def test_transform_coordinates_order_and_float64(small_triclinic):
+    s = small_triclinic
+    M = s["recip"]
+    out = refcpu.transform_coordinates(M, s["real"])
+    assert out.tobytes() == systems.box_coordinates(M, s["real"]).tobytes()
+    ref64 = s["real"].astype(np.float64) @ M.astype(np.float64)
+    assert np.max(np.abs(out - ref64)) < 1e-6
+
+
+def test_reciprocal_basis_closed_form():
+    b = np.diag([30.0, 40.0, 50.0]).astype(F32)
+    r, vol = refcpu.get_reciprocal_basis(b)
+    assert np.allclose(r, np.linalg.inv(b.astype(np.float64)), rtol=1e-6)
+    assert vol == 60000.0
+    # with basis[0,1] != basis[0,2] the :59 cofactor typo shows
+    t = systems.triclinic_basis(42, 44, 46, 80, 95, 105)
+    t[0, 1], t[0, 2] = F32(0.5), F32(1.5)
+    r, vol = refcpu.get_reciprocal_basis(t)
+    B = t.astype(np.float64)
+    cof01 = -(B[0, 1] * B[2, 2] - B[2, 1] * B[0, 1])
+    assert abs(r[0, 1] - cof01 / vol) < 1e-6 * abs(cof01 / vol) + 1e-12
+    assert not np.allclose(r, np.linalg.inv(B), rtol=1e-4)
+
+
+# ------------------------------------------------------------ histograms
+
+def _bins_f64(d, mn, mx, bn):
+    d = np.asarray(d, dtype=np.float64)
+    ok = (d >= mn) & (d < mx)
+    return np.floor((d[ok] - mn) / bn).astype(int), ok
+
+
+def test_full_histogram_vs_float64_pair_enumeration(small_random):
+    """Every in-range unordered pair counted once in slot [e(earlier), e(later)];
+    away from bin edges the float64 bin equals the float32 bin."""
+    s = small_random
+    mn, mx, bn, H = F32(0.0), F32(8.0), F32(0.05), 160
+    hi, he = refcpu.full_pairs_histograms_coords(s["box"], s["basis"], True, s["molecules"], s["elem"], 2, mn, mx,
+                                                 bn, H)
+    B = s["basis"].astype(np.float64)
+    f = s["box"].astype(np.float64)
+    n = len(f)
+    expect = np.zeros((2, 2, H))
+    edge = 0
+    for i in range(n - 1):
+        d = f[i] - f[i + 1:]
+        d = d - np.where(d > 0, np.floor(d + 0.5), np.ceil(d - 0.5))
+        r = np.sqrt(np.sum((d @ B) ** 2, axis=1))
+        ok = (r >= mn) & (r < mx)
+        x = (r[ok] - float(mn)) / float(bn)
+        near = np.abs(x - np.rint(x)) < 1e-4
+        edge += int(near.sum())
+        b = np.floor(x).astype(int)
+        js = np.arange(i + 1, n)[ok]
+        np.add.at(expect, (s["elem"][i], s["elem"][js][~near], b[~near]), 1)
+    assert hi.sum() == 0  # one atom per molecule: no intra pairs
+    assert expect.sum() > 1000
+    # every non-edge pair agrees; each edge pair is at most one extra count
+    assert expect.sum() <= he.sum() <= expect.sum() + edge
+    assert np.abs(he - expect).sum() <= edge
+
+
+def test_multiple_all_atoms_is_twice_full_symmetrised(small_triclinic):
+    s = small_triclinic
+    n = len(s["box"])
+    args = (s["box"], s["basis"], True, s["molecules"], s["elem"], 3, F32(0.0), F32(7.0), F32(0.05), 140)
+    fi, fe = refcpu.full_pairs_histograms_coords(*args)
+    mi, me = refcpu.multiple_pairs_histograms_coords(np.arange(n, dtype=np.int32), *args, allAtoms=True)
+    sym = lambda h: h + h.transpose(1, 0, 2)
+    assert np.array_equal(sym(mi), 2 * sym(fi)) and np.array_equal(sym(me), 2 * sym(fe))
+    assert fi.sum() > 0 and fe.sum() > 0  # rings give intra-molecular pairs
+
+
+def test_move_delta_equals_recompute(small_triclinic):
+    """data - (M-F)before + (M-F)after == full recompute after the move, in the
+    symmetrised sums G(r) consumes (slot-level equality does not hold)."""
+    s = small_triclinic
+    box = s["box"].copy()
+    args = (s["basis"], True, s["molecules"], s["elem"], 3, F32(0.0), F32(7.0), F32(0.05), 140)
+    fi, fe = refcpu.full_pairs_histograms_coords(box, *args)
+    idx = np.arange(13, 26, dtype=np.int32)  # molecule 1
+
+    def mf(b):
+        mi, me = refcpu.multiple_pairs_histograms_coords(idx, b, *args)
+        gi, ge = refcpu.full_pairs_histograms_coords(b[idx], args[0], True, s["molecules"][idx], s["elem"][idx],
+                                                     *args[4:])
+        return mi - gi, me - ge
+
+    bi, be = mf(box)
+    box2 = box.copy()
+    box2[idx] += F32(0.03)
+    ai, ae = mf(box2)
+    ri, re_ = refcpu.full_pairs_histograms_coords(box2, *args)
+    sym = lambda h: h + h.transpose(1, 0, 2)
+    assert np.array_equal(sym(fi - bi + ai), sym(ri)) and np.array_equal(sym(fe - be + ae), sym(re_))
+
+
+def test_bin_index_H_aliases_next_slot():
+    """A distance just below maxDistance whose (d-min)/bin rounds to H lands in
+    slot [a, b+1, 0] of the C-ordered array, as the reference's unchecked write."""
+    mn, bn, H = F32(0.0), F32(0.1), 10
+    mx = F32(1.0000001)
+    # find a float d < mx with int((d-mn)/bn) == H
+    d = F32(0.99999994)
+    while not (int(F32(F32(d - mn) / bn)) == H and d < mx):
+        d = np.nextafter(d, F32(2))
+    assert d < mx
+    dist = np.array([0.0, d], dtype=F32)
+    hi = np.zeros((2, 2, H), F32)
+    he = np.zeros((2, 2, H), F32)
+    ov = refcpu.single_pairs_histograms(0, dist, np.array([0, 1], np.int32), np.array([0, 0], np.int32), hi, he, mn,
+                                        mx, bn)
+    assert he[0, 1, 0] == 1 and he.sum() == 1 and ov == 0
+    # in the last slot the write would leave the array: counted, not stored
+    hi[:] = 0
+    he[:] = 0
+    ov = refcpu.single_pairs_histograms(0, dist, np.array([0, 1], np.int32), np.array([1, 1], np.int32), hi, he, mn,
+                                        mx, bn)
+    assert he.sum() == 0 and ov == 1
+
+
+# ----------------------------------------------------------------- chain
+
+def test_chain_ideal_counts_give_zero_gr():
+    """With pair counts equal to N_ij/V * shell volume, g(r)=1 and G(r)=0 up to
+    float32 rounding (closed form)."""
+    x = (0.025 + 0.05 * np.arange(300)).astype(F32)
+    exp = np.stack([x, np.zeros_like(x)], 1)
+    bn, mn, mx, centers, H, vols = chain.pdc_grid(exp)
+    counts = {"A": 500, "B": 1000}
+    V = F32(27000.0)
+    w = chain.normalized_weighting(counts, {"A": 8, "B": 14})
+    intra = np.zeros((2, 2, H), F32)
+    inter = np.zeros((2, 2, H), F32)
+    inter[0, 0] = 500 * 499 / 2 / 27000.0 * vols
+    inter[1, 1] = 1000 * 999 / 2 / 27000.0 * vols
+    inter[0, 1] = 500 * 1000 / 27000.0 * vols
+    G = chain.total_gr(intra, inter, ["A", "B"], counts, w, V, F32(1500) / V, vols, centers)
+    assert np.max(np.abs(G)) < 2e-4 * np.max(4 * np.pi * centers * 1500 / 27000.0)
+    assert sum(w.values()) == pytest.approx(1.0, rel=1e-6)
+
+
+def test_chain_float32_close_to_float64(small_random):
+    s = small_random
+    x = (0.025 + 0.05 * np.arange(160)).astype(F32)
+    exp = np.stack([x, np.zeros_like(x)], 1)
+    bn, mn, mx, centers, H, vols = chain.pdc_grid(exp)
+    hi, he = refcpu.full_pairs_histograms_coords(s["box"], s["basis"], True, s["molecules"], s["elem"], 2, mn, mx,
+                                                 bn, H)
+    counts = {"O": 400, "Si": 200}
+    w = chain.normalized_weighting(counts, {"O": 8, "Si": 14})
+    V = systems.volume(s["basis"])
+    G32 = chain.total_gr(hi, he, ["O", "Si"], counts, w, V, F32(600) / V, vols, centers)
+    # float64 evaluation of the same formula
+    g = np.zeros(H)
+    for a, b in [("O", "O"), ("O", "Si"), ("Si", "Si")]:
+        i, j = ["O", "Si"].index(a), ["O", "Si"].index(b)
+        if i == j:
+            nij = hi[i, i] + he[i, i]
+            D = counts[a] * (counts[a] - 1) / 2 / float(V)
+        else:
+            nij = hi[i, j] + hi[j, i] + he[i, j] + he[j, i]
+            D = counts[a] * counts[b] / float(V)
+        g += float(w[a + "-" + b]) * nij.astype(np.float64) / D
+    e = np.arange(161) * 0.05
+    vol64 = 4 / 3 * np.pi * (e[1:] ** 3 - e[:-1] ** 3)
+    G64 = 4 * np.pi * centers.astype(np.float64) * (600 / float(V)) * (g / vol64 - 1)
+    assert np.max(np.abs(G32 - G64)) < 1e-4 * (1 + np.max(np.abs(G64)))
+
+
+def test_sq_matrix_vs_float64():
+    centers = (0.01 + 0.02 * np.arange(1000)).astype(F32)
+    q = np.linspace(0.5, 25, 500).astype(F32)
+    M = chain.sq_matrix(centers, q)
+    ref = 0.02 * np.sin(np.outer(centers.astype(np.float64), q.astype(np.float64))) / q.astype(np.float64)
+    assert np.max(np.abs(M - ref)) < 5e-6
+
+
+def test_chi_square_is_pairwise_and_close_to_fsum():
+    rng = np.random.default_rng(1)
+    e = rng.random(1000).astype(F32)
+    m = rng.random(1000).astype(F32)
+    c = chain.chi_square(e, m)
+    assert c.tobytes() == _pairwise(((e - m) ** 2).astype(F32)).tobytes()
+    assert c == pytest.approx(math.fsum(((e.astype(np.float64) - m) ** 2)), rel=1e-5)
