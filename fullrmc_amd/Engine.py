@@ -184,6 +184,7 @@ class Engine:
         _device.require_gpu()
         if self._dev is not None and self._dev_kmax >= kmax and self._dev_synced:
             return self._dev
+        self._pull_device()
         self._drop_device()
         if not self.constraints:
             raise ValueError("no constraints")
@@ -276,14 +277,16 @@ class Engine:
         self.totalStandardError = np.float32(tse.value)
         self._dev_dirty = False
 
-    def run_device(self, numberOfSteps=100, k=64, log=None, native=True):
+    def run_device(self, numberOfSteps=100, k=64, log=None, native=True, pull=True):
         """Device-resident run with k speculative proposals per window.
 
         With translation moves on every group (the default generator) the
         whole loop, random streams included, runs in the native library;
         otherwise proposals come from the Python generators (the reference's
         own NumPy calls) and only the evaluation runs on the device.
-        Returns a dict of counters and timings."""
+        ``pull=False`` leaves the state on the device (the host mirror is
+        refreshed by the next :meth:`run`, :meth:`sync_from_device` or a
+        pulling call).  Returns a dict of counters and timings."""
         h = self._device_engine(max(k, 1))
         gen0 = self.generators[0]
         nat = (native and all(g is gen0 for g in self.generators)
@@ -295,8 +298,17 @@ class Engine:
             stats = self._run_python(h, int(numberOfSteps), int(k), log)
         stats["seconds"] = time.perf_counter() - t0
         self._dev_dirty = True
-        self._pull_device()
+        if pull:
+            self._pull_device()
         return stats
+
+    def sync_from_device(self):
+        """Refresh coordinates, histograms and chi-squares from the device."""
+        self._pull_device()
+
+    def device_handle(self):
+        """The frmc_engine handle of the device state (None before run_device)."""
+        return self._dev
 
     def _run_native(self, h, nsteps, k, gen, log):
         L = _lib.lib()

@@ -60,6 +60,8 @@ _PROTOS = {
     "frmc_last_error": [],
     "frmc_version": [],
     "frmc_last_overflow": [],
+    "frmc_set_timing": [c_int],
+    "frmc_last_kernel_ms": [],
     # pairs_distances
     "frmc_pairs_distances_to_indexcoords": [c_i32, c_vp, c_i32, c_vp, c_int, c_int, c_vp, c_vp],
     "frmc_pairs_distances_to_multi_indexcoords": [c_vp, c_i32, c_vp, c_i32, c_vp, c_int, c_int, c_vp, c_vp],
@@ -105,8 +107,13 @@ _PROTOS = {
     "frmc_engine_get_real": [c_vp, c_f32p],
     "frmc_engine_set_real": [c_vp, c_f32p],
     "frmc_engine_set_counts": [c_vp, c_i32, c_i32p],
+    "frmc_engine_profile": [c_vp, c_int],
+    "frmc_engine_kernel_times": [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64)],
+    "frmc_engine_set_adaptive": [c_vp, c_int],
 }
-_RESTYPES = {"frmc_last_error": ctypes.c_char_p, "frmc_last_overflow": c_i64}
+KERNEL_NAMES = ["k_delta", "k_eval", "k_commit"]
+_RESTYPES = {"frmc_last_error": ctypes.c_char_p, "frmc_last_overflow": c_i64, "frmc_set_timing": None,
+             "frmc_last_kernel_ms": c_f32}
 
 _lib = None
 

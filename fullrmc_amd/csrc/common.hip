@@ -10,6 +10,11 @@ namespace frmc {
 
 static thread_local char g_err[1024] = "";
 static thread_local int64_t g_overflow = 0;
+static thread_local int g_timing = 0;
+static thread_local float g_last_ms = 0.f;
+
+bool timing_enabled() { return g_timing != 0; }
+void set_last_kernel_ms(float ms) { g_last_ms = ms; }
 
 void set_error(const char *fmt, ...) {
     va_list ap;
@@ -118,6 +123,8 @@ extern "C" {
 const char *frmc_last_error(void) { return g_err; }
 int frmc_version(void) { return 10000; }
 int64_t frmc_last_overflow(void) { return g_overflow; }
+void frmc_set_timing(int enable) { g_timing = enable; }
+float frmc_last_kernel_ms(void) { return g_last_ms; }
 
 int frmc_get_reciprocal_basis(const float *basis, float *rbasis, double *vol, void *stream) {
     FRMC_REQUIRE(basis && rbasis && vol, "NULL pointer argument");

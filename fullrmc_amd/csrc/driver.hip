@@ -1,0 +1,207 @@
+// driver.hip -- native run loop for single-group translation moves.
+//
+// Reproduces fullrmc's proposal stream exactly from its two MT19937
+// generators (CPython's random: group index, amplitude, acceptance draw;
+// NumPy's legacy RandomState: direction), evaluates k proposals per window
+// on the device, walks them in order with the reference's accept rule, and
+// commits the first accepted one after rewinding both streams to just after
+// its draws.  Every proposal before the first acceptance was rejected with
+// exactly one acceptance draw (Engine.py:3308-3317 draws only when the total
+// got worse, and with the default tolerance a worse move is always
+// rejected), so the k proposals of a window are the ones the sequential loop
+// generates, and the trajectory is the sequential one.
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "engine.h"
+
+namespace frmc {
+
+// MT19937 (Matsumoto & Nishimura 1998); `used` counts 32-bit words drawn so a
+// stream can be rewound to any proposal boundary by replaying that many.
+struct Stream {
+    frmc_mt19937 *s;
+    int32_t used;
+    inline uint32_t next() {
+        if (s->pos >= 624) {
+            for (int i = 0; i < 624; i++) {
+                uint32_t y = (s->mt[i] & 0x80000000u) | (s->mt[(i + 1) % 624] & 0x7fffffffu);
+                s->mt[i] = s->mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+            }
+            s->pos = 0;
+        }
+        uint32_t y = s->mt[s->pos++];
+        y ^= (y >> 11);
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= (y >> 18);
+        used++;
+        return y;
+    }
+    // random.random() and RandomState.random_sample(): 53-bit double
+    inline double uniform() {
+        uint32_t a = next() >> 5, b = next() >> 6;
+        return (a * 67108864.0 + b) * (1.0 / 9007199254740992.0);
+    }
+    // random.randint(0, n-1) == randrange(0, n) -> _randbelow_with_getrandbits(n)
+    inline int32_t randbelow(int32_t n) {
+        int k = 0;
+        while (k < 32 && ((uint32_t)1 << k) <= (uint32_t)n) k++;  // n.bit_length()
+        uint32_t r;
+        do {
+            r = next() >> (32 - k);
+        } while (r >= (uint32_t)n);
+        return (int32_t)r;
+    }
+};
+
+// np.linalg.norm of a float32 3-vector: x.dot(x) goes through BLAS sdot,
+// which rounds each product to float32 and accumulates in double; NumPy
+// casts the sum to float32 and takes the float32 sqrt.
+static inline float np_norm3(const float v[3]) {
+    const float p0 = v[0] * v[0], p1 = v[1] * v[1], p2 = v[2] * v[2];
+    double acc = 0.0;
+    acc += (double)p0;
+    acc += (double)p1;
+    acc += (double)p2;
+    return sqrtf((float)acc);
+}
+
+// generate_random_vector(minAmp, maxAmp) (Core/Collection.py:674-701)
+static inline void random_vector(Stream &py, Stream &np_, float minAmp, float maxAmp, float v[3]) {
+    float norm;
+    do {
+        for (int d = 0; d < 3; d++) v[d] = (float)(1.0 - 2.0 * np_.uniform());
+        norm = np_norm3(v);
+    } while (norm == 0.0f);
+    for (int d = 0; d < 3; d++) v[d] = v[d] / norm;
+    float base[3];
+    for (int d = 0; d < 3; d++) base[d] = v[d] * minAmp;
+    const float amp = maxAmp - minAmp;
+    const float scale = (float)py.uniform() * amp;  // Python float * float32 -> float32
+    for (int d = 0; d < 3; d++) v[d] = v[d] * scale;
+    for (int d = 0; d < 3; d++) v[d] = v[d] + base[d];
+}
+
+// transform_coordinates (boundary_conditions_collection.pyx:105-108)
+static inline void to_box(const float M[9], const float r[3], float b[3]) {
+    b[0] = r[0] * M[0] + r[1] * M[3] + r[2] * M[6];
+    b[1] = r[0] * M[1] + r[1] * M[4] + r[2] * M[7];
+    b[2] = r[0] * M[2] + r[1] * M[5] + r[2] * M[8];
+}
+
+// Window size from the running acceptance rate: enough proposals that the
+// window holds an acceptance with ~95 % probability, within [kmin, kmax].
+static int32_t window_size(int64_t generated, int64_t accepted, int32_t kmax) {
+    const double p = (accepted + 1.0) / (generated + 2.0);
+    double k = p >= 1.0 ? 1.0 : log(0.05) / log(1.0 - p);
+    const int32_t kmin = std::min<int32_t>(kmax, 4);
+    int32_t kk = (int32_t)std::ceil(k);
+    return std::max(kmin, std::min(kmax, kk));
+}
+
+}  // namespace frmc
+
+using namespace frmc;
+
+extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float minAmp, float maxAmp,
+                                           float tolerance, frmc_mt19937 *py_state, frmc_mt19937 *np_state,
+                                           frmc_run_stats *stats, int32_t *group_log, uint8_t *accept_log,
+                                           float *tse_log) {
+    FRMC_REQUIRE(e && py_state && np_state && stats, "NULL argument");
+    FRMC_REQUIRE(e->have_real, "engine was created without real coordinates");
+    FRMC_REQUIRE(k >= 1 && k <= e->kmax, "k=%d out of range [1,%d]", k, e->kmax);
+    FRMC_REQUIRE(py_state->pos >= 0 && py_state->pos <= 624 && np_state->pos >= 0 && np_state->pos <= 624,
+                 "bad MT19937 position");
+    std::vector<int32_t> gids(k), aoff(k + 1);
+    std::vector<float> nbox((size_t)k * e->gmax * 3), nreal((size_t)k * e->gmax * 3);
+    std::vector<double> uacc(k);
+    std::vector<int32_t> py_pre(k), py_post(k), np_used(k);
+    std::vector<float> tse(k);
+    int64_t step = 0;
+    int64_t gen0 = 0, acc0 = 0;  // acceptance statistics of this call
+    while (step < nsteps) {
+        int32_t kw = e->adaptive ? window_size(gen0, acc0, k) : k;
+        kw = (int32_t)std::min<int64_t>(kw, nsteps - step);
+        const frmc_mt19937 py0 = *py_state, np0 = *np_state;
+        Stream py{py_state, 0}, np_{np_state, 0};
+        int32_t off = 0;
+        for (int32_t p = 0; p < kw; p++) {
+            // Engine.__on_runtime_step_select_group -> RandomSelector.select_index
+            const int32_t gi = py.randbelow(e->G);
+            gids[p] = gi;
+            // TranslationGenerator.transform_coordinates
+            float v[3];
+            random_vector(py, np_, minAmp, maxAmp, v);
+            py_pre[p] = py.used;
+            np_used[p] = np_.used;
+            uacc[p] = py.uniform();  // the acceptance draw, consumed only if the move is worse
+            py_post[p] = py.used;
+            aoff[p] = off;
+            const int32_t g0 = e->h_goff[gi], g = e->h_goff[gi + 1] - g0;
+            for (int32_t a = 0; a < g; a++) {
+                const size_t id = (size_t)e->h_gatoms[g0 + a];
+                float r[3], b[3];
+                for (int d = 0; d < 3; d++) r[d] = e->h_real[3 * id + d] + v[d];
+                to_box(e->recip, r, b);
+                for (int d = 0; d < 3; d++) {
+                    nreal[3 * (size_t)(off + a) + d] = r[d];
+                    nbox[3 * (size_t)(off + a) + d] = b[d];
+                }
+            }
+            off += g;
+        }
+        aoff[kw] = off;
+        int rc = frmc_engine_eval(e, kw, gids.data(), nbox.data(), tse.data(), nullptr);
+        if (rc) return rc;
+        stats->windows++;
+        stats->evaluated += kw;
+        // walk the window in order (Engine.py:3308-3338)
+        int32_t acc_p = -1;
+        bool drew = false;
+        for (int32_t p = 0; p < kw; p++) {
+            stats->generated++;
+            stats->tried++;
+            gen0++;
+            bool accept = true;
+            drew = false;
+            if (tse[p] > e->cur_tse) {
+                drew = true;
+                // random() > tolerance, the Python float compared as float32
+                if ((float)uacc[p] > tolerance) accept = false;
+                else stats->tolerated++;
+            }
+            if (group_log) group_log[step] = gids[p];
+            if (accept_log) accept_log[step] = accept ? 1 : 0;
+            if (tse_log) tse_log[step] = accept ? tse[p] : e->cur_tse;
+            step++;
+            if (accept) {
+                stats->accepted++;
+                acc0++;
+                acc_p = p;
+                break;
+            }
+        }
+        if (acc_p >= 0) {
+            // rewind both streams to just after the accepted proposal's draws
+            *py_state = py0;
+            *np_state = np0;
+            Stream rp{py_state, 0}, rn{np_state, 0};
+            const int32_t npy = drew ? py_post[acc_p] : py_pre[acc_p];
+            while (rp.used < npy) (void)rp.next();
+            while (rn.used < np_used[acc_p]) (void)rn.next();
+            rc = frmc_engine_commit(e, acc_p);
+            if (rc) return rc;
+            const int32_t gi = gids[acc_p], g0 = e->h_goff[gi], g = e->h_goff[gi + 1] - g0;
+            for (int32_t a = 0; a < g; a++) {
+                const size_t id = (size_t)e->h_gatoms[g0 + a];
+                for (int d = 0; d < 3; d++) e->h_real[3 * id + d] = nreal[3 * (size_t)(aoff[acc_p] + a) + d];
+            }
+        }
+        // without an acceptance both streams already stand after the window
+    }
+    FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
+    return FRMC_OK;
+}

@@ -1,0 +1,160 @@
+// engine.h -- internal layout of the device-resident engine (not part of the
+// C ABI).  State in HBM:
+//   xyz        (n,3) float32 box coordinates, molecule / element index,
+//   groups     CSR table (offsets, atom indexes),
+//   per constraint c:
+//     slot[c]  int32 counts [2][nel][nel][H] (+1 overflow counter): the
+//              reference's hintra/hinter data, slot-exact,
+//     base[c]  int32 [T][H] symmetrised pair-type sums of slot, T =
+//              nel(nel+1)/2 element pairs in sorted order -- all that G(r)
+//              consumes (PairDistributionConstraints.py:873-875),
+//   window buffers for k proposals: int32 pair-type deltas dn[k][sum_c T*H_c]
+//     (two buffers: a window accumulates into one while its evaluation
+//     zeroes the other), S(Q) squared deviations, chi-square partials and
+//     per-proposal completion counters.
+// Window inputs (group index, old/new box coordinates of the group atoms) and
+// outputs (totals, chi-squares, completion flags) live in pinned host memory
+// that the kernels read and write directly, so a window needs no DMA.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "frmc_device.h"
+#include "frmc_host.h"
+
+namespace frmc {
+
+constexpr int kMaxC = 4;   // constraints per engine
+constexpr int kMaxG = 64;  // atoms per group
+constexpr int kPG = 4;     // proposals per k_eval block (one wave each)
+constexpr int kQC = 64;    // Q points per k_eval block (one per lane)
+constexpr int kRT = 128;   // rows of the M tile staged per pass
+
+struct DevC {
+    int32_t kind;   // FRMC_PDC or FRMC_SFC
+    Grid g;
+    int32_t H, M;   // histogram bins, model points (H for PDC, nQ for SFC)
+    int32_t hasW, hasSF;
+    float sf;
+    const float *w, *D, *shellV, *pref, *expv, *dataW, *sqm;
+    const int32_t *leaves;  // (start,len) pairs of NumPy's pairwise-sum leaves of length M
+    const int32_t *prog;    // postfix combine program over the leaves
+    int32_t nleaves, nprog;
+    int32_t *base;          // [T][H]
+    int32_t *slot;          // [2][nel][nel][H] + overflow counter
+    int32_t dnOff;          // offset in a proposal's delta block
+};
+
+// Window staging in pinned host memory (device-accessible).
+struct Stage {
+    int32_t *pgroup;  // [kmax]
+    int32_t *poff;    // [kmax+1]
+    float *pold;      // [kmax*gmax*3]
+    float *pnew;      // [kmax*gmax*3]
+};
+
+struct View {
+    int32_t n, nel, T, C, isPBC, ortho;
+    Basis B;
+    float *xyz;                    // (n,3) current box coordinates
+    const int32_t *mol, *elem;
+    const int32_t *goff, *gatoms;  // groups
+    DevC c[kMaxC];
+    int32_t dnStride;              // ints per proposal delta block
+    int32_t Hmax, Mmax, maxLeaves, nqc;
+    // window
+    const int32_t *pgroup, *poff;  // staging (host-mapped)
+    const float *pold, *pnew;
+    int32_t *dn;                   // delta buffer of this window
+    int32_t *dn_zero;              // the other buffer, zeroed during this window's evaluation
+    int64_t zero_len;
+    int32_t *cnt;                  // [kmax] blocks finished per proposal (reset by the last)
+    float *sqbuf;                  // [kmax][C][Mmax]
+    float *chi2d;                  // [kmax][C] device partials
+    float *out_tse;                // [kmax] host-mapped
+    float *out_chi2;               // [kmax][C] host-mapped
+    volatile uint32_t *out_done;   // [kmax] host-mapped completion sequence numbers
+    uint32_t seq;
+};
+
+// Windows of single-atom moves travel to k_delta in the kernel arguments
+// (atom index and new box coordinates per proposal, old coordinates read from
+// HBM), which avoids reading pinned host memory at kernel start.
+constexpr int kInline = 64;
+struct Inline {
+    int32_t k;  // 0: use the staging buffers
+    int32_t atom[kInline];
+    float nw[kInline][3];
+};
+
+// launchers (engine_kernels.hip)
+int launch_delta(const View &V, const Inline &I, int32_t k, int32_t nchunk, hipStream_t st);
+int launch_eval(const View &V, int32_t k, hipStream_t st);
+int launch_commit(const View &V, int32_t p, hipStream_t st);
+int launch_symmetrise(int32_t *base, const int32_t *slot, int32_t nel, int32_t H, hipStream_t st);
+int launch_fill(int32_t *p, int64_t n, hipStream_t st);
+int launch_full(const float *coords, int32_t n, const float *basis, int isPBC, bool ortho, const int32_t *mol,
+                const int32_t *elem, const HistArgs &A, int32_t *counts, hipStream_t st);
+void pairwise_plan(int32_t n, std::vector<int32_t> &leaves, std::vector<int32_t> &prog);
+size_t eval_lds_bytes(const View &V);
+
+}  // namespace frmc
+
+struct frmc_engine {
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    int32_t n = 0, nel = 0, T = 0, C = 0, G = 0, isPBC = 0, ortho = 0, kmax = 0, gmax = 0, total_gatoms = 0;
+    int32_t ncu = 256;
+    frmc::Basis B;
+    float *xyz = nullptr, *dbasis = nullptr;
+    int32_t *mol = nullptr, *elem = nullptr, *goff = nullptr, *gatoms = nullptr;
+    std::vector<int32_t> h_goff, h_gatoms;
+    frmc::DevC c[frmc::kMaxC];
+    int32_t dnStride = 0, Hmax = 0, Mmax = 0, maxLeaves = 1;
+    // window buffers
+    int32_t *dnbuf[2] = {nullptr, nullptr};
+    int cur_dn = 0;
+    int32_t dn_used[2] = {0, 0};  // proposals whose deltas a buffer holds (0: all zero)
+    int32_t *cnt = nullptr;
+    float *sqbuf = nullptr, *chi2d = nullptr;
+    frmc::Stage stage[2];
+    int cur_stage = 0;     // staging buffer of the last evaluated window
+    float *h_tse = nullptr, *h_chi2 = nullptr;
+    uint32_t *h_done = nullptr;
+    uint32_t seq = 0;
+    int32_t last_k = 0;
+    // host mirror of the accepted state
+    float cur_tse = 0.f;
+    float cur_chi2[frmc::kMaxC] = {0};
+    std::vector<float> h_real, h_box;
+    float recip[9] = {0};
+    bool have_real = false;
+    bool adaptive = true;  // window size follows the acceptance rate
+    std::vector<void *> allocs, host_allocs;
+    // optional per-kernel timing with HIP events on the engine stream
+    bool profile = false;
+    struct Pending {
+        int kid;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    double kernel_ms[FRMC_NKERNELS] = {0};
+    int64_t kernel_launches[FRMC_NKERNELS] = {0};
+};
+
+namespace frmc {
+View make_view(frmc_engine *e, int stage_idx);
+void collect_times(frmc_engine *e);
+
+// bracket one kernel launch with events when profiling is on
+struct KTimer {
+    frmc_engine *e;
+    int kid;
+    hipEvent_t a = nullptr;
+    KTimer(frmc_engine *e_, int kid_);
+    ~KTimer();
+};
+}  // namespace frmc

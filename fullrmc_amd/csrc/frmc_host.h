@@ -12,6 +12,8 @@ namespace frmc {
 
 void set_error(const char *fmt, ...);
 void set_overflow(int64_t v);
+bool timing_enabled();
+void set_last_kernel_ms(float ms);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -398,7 +398,28 @@ static int launch_full_t(const float *coords, int32_t n, const float *basis, int
 int launch_full(const float *coords, int32_t n, const float *basis, int isPBC, bool ortho, const int32_t *mol,
                 const int32_t *elem, const HistArgs &A, int32_t *counts, hipStream_t st) {
     if (n < 2) return FRMC_OK;
-    return launch_full_t<256>(coords, n, basis, isPBC, ortho, mol, elem, A, counts, st);
+    hipEvent_t a = nullptr, b = nullptr;
+    const bool timing = timing_enabled();
+    if (timing) {
+        FRMC_CHECK_HIP(hipEventCreate(&a));
+        FRMC_CHECK_HIP(hipEventCreate(&b));
+        FRMC_CHECK_HIP(hipEventRecord(a, st));
+    }
+    int rc = launch_full_t<256>(coords, n, basis, isPBC, ortho, mol, elem, A, counts, st);
+    if (timing) {
+        float ms = 0.f;
+        hipError_t e = hipEventRecord(b, st);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (e != hipSuccess && rc == FRMC_OK) {
+            set_error("kernel timing failed: %s", hipGetErrorString(e));
+            rc = FRMC_EHIP;
+        }
+        set_last_kernel_ms(ms);
+    }
+    return rc;
 }
 
 }  // namespace frmc

@@ -93,12 +93,31 @@ def _rsa(n, basis, rmin, rng, max_tries=200):
     return (pts @ B).astype(F32)
 
 
+def _rsa_cubic_batched(n, L, rmin, rng, batch=40000):
+    """Random insertion in a periodic cube, vectorised over batches of
+    candidates (periodic k-d trees): a candidate is kept if no kept atom lies
+    within rmin and no earlier candidate of its batch does."""
+    from scipy.spatial import cKDTree
+    pts = np.empty((0, 3))
+    while len(pts) < n:
+        cand = rng.random((batch, 3)) * L
+        ok = np.ones(batch, dtype=bool)
+        if len(pts):
+            near = cKDTree(pts, boxsize=L).query_ball_point(cand, rmin, return_length=True)
+            ok &= near == 0
+        pairs = cKDTree(cand, boxsize=L).query_pairs(rmin, output_type="ndarray")
+        if len(pairs):
+            ok[pairs.max(axis=1)] = False
+        pts = np.concatenate([pts, cand[ok][: n - len(pts)]])
+    return pts.astype(F32)
+
+
 def random_box(n, L, rmin, seed, species=("Ar",), fractions=None):
     """Uniform random atoms in a cube of edge L (Angstrom) with no pair
     closer than rmin; species assigned by fractions (default all the first)."""
     rng = np.random.default_rng(seed)
     basis = cubic_basis(L)
-    real = _rsa(n, basis, rmin, rng)
+    real = _rsa_cubic_batched(n, L, rmin, rng) if n > 20000 else _rsa(n, basis, rmin, rng)
     if fractions is None:
         elements = [species[0]] * n
     else:
@@ -181,9 +200,9 @@ def ring_box(nmol=500, cell=(42.0, 44.0, 46.0, 80.0, 95.0, 105.0), seed=0, min_c
 
 
 def sio2_box(n=200000, L=145.0, rmin=1.5, seed=0):
-    """Amorphous SiO2-like random box: n//3 Si (66,667 at n=200,000), the
-    rest O, uniform with a minimum pair distance."""
-    nsi = n // 3
+    """Amorphous SiO2-like random box: round(n/3) Si (66,667 at n=200,000),
+    the rest O, uniform with a minimum pair distance."""
+    nsi = int(round(n / 3.0))
     sysm = random_box(n, L, rmin, seed, species=("O", "Si"))
     rng = np.random.default_rng(seed + 1)
     el = np.array(["O"] * n, dtype=object)

@@ -38,6 +38,11 @@ int frmc_version(void);
    histogram call on this thread (the reference writes those out of bounds;
    see DESIGN.md "bin index quirk"). */
 int64_t frmc_last_overflow(void);
+/* When enabled (per thread), the all-pairs kernel of the next
+   full_pairs call is bracketed by HIP events on its stream and the call
+   synchronises; frmc_last_kernel_ms() returns that kernel's duration. */
+void frmc_set_timing(int enable);
+float frmc_last_kernel_ms(void);
 
 /* ------------------------------------------------------------------------
  * pairs_distances  (Extensions/pairs_distances.pyx)
@@ -255,6 +260,19 @@ int frmc_engine_get_state(frmc_engine *e, float *box_out, int32_t constraint, in
 int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float minAmp, float maxAmp,
                                 float tolerance, frmc_mt19937 *py_state, frmc_mt19937 *np_state,
                                 frmc_run_stats *stats, int32_t *group_log, uint8_t *accept_log, float *tse_log);
+/* Per-kernel timing with HIP events recorded on the engine stream around
+   every launch (adds two event records per launch while enabled).
+   frmc_engine_profile(e,1) resets and enables, (e,0) disables;
+   frmc_engine_kernel_times fills FRMC_NKERNELS totals (ms) and counts. */
+#define FRMC_K_DELTA 0   /* k_delta: moved group vs all atoms, k proposals */
+#define FRMC_K_EVAL 1    /* k_eval: G(r), S(Q), chi-squares and total per proposal */
+#define FRMC_K_COMMIT 2  /* k_commit: histograms and coordinates of the accepted move */
+#define FRMC_NKERNELS 3
+int frmc_engine_profile(frmc_engine *e, int enable);
+int frmc_engine_kernel_times(frmc_engine *e, double *ms_out, int64_t *launches_out);
+/* window size of frmc_engine_run_translation: adaptive (default; k is the
+   maximum, the size follows the acceptance rate) or always k */
+int frmc_engine_set_adaptive(frmc_engine *e, int enable);
 /* replace the histograms of one constraint with slot counts
    [2][nel][nel][H]+1 (host), e.g. a constraint's current data */
 int frmc_engine_set_counts(frmc_engine *e, int32_t constraint, const int32_t *slot_in);
