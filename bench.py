@@ -175,8 +175,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the pair path runs on a single MI355X in this build "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py measures one GPU: the pair path runs on a single MI355X; "
+              "a sharded multi-GPU run is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import torch

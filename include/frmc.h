@@ -35,8 +35,11 @@ extern "C" {
 const char *frmc_last_error(void);
 int frmc_version(void);
 /* number of pair bins that fell outside the histogram array in the last
-   histogram call on this thread (the reference writes those out of bounds;
-   see DESIGN.md "bin index quirk"). */
+   histogram call on this thread.  The reference computes the bin as
+   (int)((d-min)/bin) with no bounds check, so a distance just below
+   maxDistance can give bin == histSize: inside the (nel,nel,H) array that
+   aliases the next row's bin 0 (reproduced), past the array's end it would
+   write out of bounds (counted here instead). */
 int64_t frmc_last_overflow(void);
 /* When enabled (per thread), the all-pairs kernel of the next
    full_pairs call is bracketed by HIP events on its stream and the call
