@@ -34,31 +34,34 @@ PEAK_HBM_GBS = 8000.0
 OPS_PER_PAIR = 35          # PBC pair: diff 3, round 6x3, sub 3, matvec 15, norm2 5, sqrt 1, bin 2 (minus 9 -> 35)
 
 
-def build_workload(seed=2026):
+def target_curves(start, target, dr, H, q):
+    """float64 G(r) on r = dr/2 + dr*k (k < H) and S(Q) on q of the target
+    box, from its pair counts (GPU full histogram), with the start box's
+    composition; returns (elem of start, pdc experimentalData, sfc
+    experimentalData) as float32."""
     from fullrmc_amd import systems
     from fullrmc_amd.Constraints.PairDistributionConstraints import ATOMIC_NUMBER, normalized_weighting
     from fullrmc_amd.Core.pairs_histograms import full_pairs_histograms_coords
 
-    start = systems.b2_alloy(cells=(15, 15, 30), a=3.01, sigma=0.08, seed=seed)
-    target = systems.b2_alloy(cells=(15, 15, 30), a=3.01, sigma=0.08, seed=seed + 1)
     els = sorted(set(start["elements"]))
     lut = {e: i for i, e in enumerate(els)}
     elem = np.array([lut[e] for e in start["elements"]], dtype=np.int32)
+    telem = np.array([lut[e] for e in target["elements"]], dtype=np.int32)
+    nel = len(els)
     counts = {e: start["elements"].count(e) for e in els}
     n = len(elem)
     V = float(systems.volume(start["basis"]))
     rho = n / V
     w = {k: float(v) for k, v in normalized_weighting(counts, {e: ATOMIC_NUMBER[e] for e in els}).items()}
-    # float64 target G(r) on r = 0.01 + 0.02 k from the target box's pair counts
     tbox = systems.box_coordinates(target["recip"], target["real"])
-    hi, he = full_pairs_histograms_coords(tbox, target["basis"], True, target["molecules"], elem, 2,
-                                          np.float32(0.0), np.float32(20.0), np.float32(0.02), 1000)
-    r = 0.01 + 0.02 * np.arange(1000)
-    edges = 0.02 * np.arange(1001)
+    hi, he = full_pairs_histograms_coords(tbox, target["basis"], True, target["molecules"], telem, nel,
+                                          np.float32(0.0), np.float32(dr * H), np.float32(dr), H)
+    r = dr / 2 + dr * np.arange(H)
+    edges = dr * np.arange(H + 1)
     shell = 4.0 / 3.0 * np.pi * (edges[1:] ** 3 - edges[:-1] ** 3)
-    g = np.zeros(1000)
-    for a in range(2):
-        for b in range(a, 2):
+    g = np.zeros(H)
+    for a in range(nel):
+        for b in range(a, nel):
             ea, eb = els[a], els[b]
             if a == b:
                 nij = hi[a, a].astype(np.float64) + he[a, a]
@@ -68,14 +71,36 @@ def build_workload(seed=2026):
                 D = counts[ea] * counts[eb] / V
             g += w[ea + "-" + eb] * nij / D / shell
     G = 4 * np.pi * r * rho * (g - 1)
-    q = np.linspace(0.5, 25.0, 500)
-    S = 1 + np.sum(G[:, None] * 0.02 * np.sin(np.outer(r, q)) / q, axis=0)
+    S = 1 + np.sum(G[:, None] * dr * np.sin(np.outer(r, q)) / q, axis=0)
     pdc = np.stack([r, G], 1).astype(np.float32)
     sfc = np.stack([q, S], 1).astype(np.float32)
+    return elem, pdc, sfc
+
+
+def build_workload(seed=2026):
+    """B2 alloy, 13,500 atoms (15x15x30 cells), PDC r in [0,20) @0.02, SFC on
+    its own r grid with Q = linspace(0.5, 25, 500); the target is an
+    independently jittered copy."""
+    from fullrmc_amd import systems
+    start = systems.b2_alloy(cells=(15, 15, 30), a=3.01, sigma=0.08, seed=seed)
+    target = systems.b2_alloy(cells=(15, 15, 30), a=3.01, sigma=0.08, seed=seed + 1)
+    elem, pdc, sfc = target_curves(start, target, 0.02, 1000, np.linspace(0.5, 25.0, 500))
     return start, elem, pdc, sfc
 
 
-def make_engine(start, pdc, sfc, seed):
+def build_sio2_workload(seed=2030):
+    """SiO2-like box, 200,000 atoms (66,667 Si) in a 145 A cube with a 1.5 A
+    minimum distance; PDC r in [0,20) @0.01 (2000 bins), SFC on its own r grid
+    with Q = linspace(0.3, 30, 1000); the target is an independent box."""
+    from fullrmc_amd import systems
+    start = systems.sio2_box(200000, 145.0, 1.5, seed=seed)
+    target = systems.sio2_box(200000, 145.0, 1.5, seed=seed + 1)
+    target["elements"] = start["elements"]  # same composition and species order
+    elem, pdc, sfc = target_curves(start, target, 0.01, 2000, np.linspace(0.3, 30.0, 1000))
+    return start, elem, pdc, sfc
+
+
+def make_engine(start, pdc, sfc, seed, dr=0.02):
     from fullrmc_amd.Constraints import PairDistributionConstraint, StructureFactorConstraint
     from fullrmc_amd.Engine import Engine
     from fullrmc_amd.Generators import TranslationGenerator
@@ -85,47 +110,70 @@ def make_engine(start, pdc, sfc, seed):
     E.set_system(start["real"], start["elements"], start["molecules"], start["basis"], start["recip"])
     E.set_move_generator(TranslationGenerator(0.1))
     E.add_constraints([PairDistributionConstraint(pdc),
-                       StructureFactorConstraint(sfc, rmin=0.0, rmax=20.0, dr=0.02)])
+                       StructureFactorConstraint(sfc, rmin=0.0, rmax=20.0, dr=dr)])
     return E
 
 
-def gpu_full_histogram(start, elem, reps=3):
-    """One O(N^2) recompute on the PDC grid: kernel time by HIP events."""
+def gpu_full_histogram(start, elem, dr, H, reps=3, float64=False):
+    """One O(N^2) recompute on the PDC grid (r in [0, dr*H)): median kernel
+    time by HIP events; float64=True times the binary64-recompute mode and
+    also returns its number of double re-checks."""
     from fullrmc_amd import _device as D
     from fullrmc_amd import _lib, systems
     L = _lib.lib()
     box = systems.box_coordinates(start["recip"], start["real"])
     n = len(box)
+    nel = int(elem.max()) + 1
     d_c, d_b = D.upload(box), D.upload(start["basis"])
     d_m, d_e = D.upload(start["molecules"]), D.upload(elem)
-    counts = D.empty((2 * 2 * 2 * 1000 + 1,), np.int32)
+    counts = D.empty((2 * nel * nel * H + 4,), np.int32)
+    rechecked = ctypes.c_int64(0)
     L.frmc_set_timing(1)
     ms = []
     for _ in range(reps + 1):
-        _lib.check(L.frmc_full_pairs_counts(d_c.data_ptr(), n, d_b.data_ptr(), 1, d_m.data_ptr(), d_e.data_ptr(),
-                                            2, 0.0, 20.0, 0.02, 1000, counts.data_ptr(), D.stream()))
+        if float64:
+            _lib.check(L.frmc_full_pairs_counts_f64(d_c.data_ptr(), n, d_b.data_ptr(), 1, d_m.data_ptr(),
+                                                    d_e.data_ptr(), nel, 0.0, dr * H, dr, H, counts.data_ptr(),
+                                                    ctypes.byref(rechecked), D.stream()))
+        else:
+            _lib.check(L.frmc_full_pairs_counts(d_c.data_ptr(), n, d_b.data_ptr(), 1, d_m.data_ptr(),
+                                                d_e.data_ptr(), nel, 0.0, dr * H, dr, H, counts.data_ptr(),
+                                                D.stream()))
         ms.append(L.frmc_last_kernel_ms())
     L.frmc_set_timing(0)
     ms = sorted(ms[1:])[len(ms[1:]) // 2]
     pairs = n * (n - 1) / 2
-    return pairs / (ms * 1e-3), ms
+    return pairs / (ms * 1e-3), ms, int(rechecked.value)
 
 
-def cpu_baseline(start, pdc, sfc, seconds):
-    """The CPU oracle (C restatement of the reference loops + the NumPy chain),
-    one thread, on a bounded sample of the same workload."""
+def cpu_engine(start, pdc, sfc, dr, seed=7, init_from=None):
+    """The oracle engine on the workload; init_from (a device Engine, synced to
+    the host) supplies the coordinates and slot-exact histograms instead of an
+    O(N^2) oracle recompute."""
     from oracle import chain
     from oracle import engine as oeng
     from fullrmc_amd.Constraints.PairDistributionConstraints import ATOMIC_NUMBER
-    random.seed(7)
-    np.random.seed(7)
-    osys = oeng.System(start["real"], start["elements"], start["molecules"], start["basis"], start["recip"])
+    random.seed(seed)
+    np.random.seed(seed)
+    real = start["real"] if init_from is None else init_from.realCoordinates
+    osys = oeng.System(real, start["elements"], start["molecules"], start["basis"], start["recip"])
     w = chain.normalized_weighting(osys.counts, {e: ATOMIC_NUMBER[e] for e in osys.elements})
     oe = oeng.Engine(osys, [[i] for i in range(len(start["real"]))], oeng.Translation(0.1),
-                     [oeng.PDC(osys, pdc, w), oeng.SFC(osys, sfc, w, 0.0, 20.0, 0.02)])
-    t0 = time.perf_counter()
-    oe.initialize()
-    t_full = time.perf_counter() - t0
+                     [oeng.PDC(osys, pdc, w), oeng.SFC(osys, sfc, w, 0.0, 20.0, dr)])
+    if init_from is not None:
+        assert osys.box.tobytes() == np.asarray(init_from.boxCoordinates, np.float32).tobytes()
+    if init_from is None:
+        oe.initialize()
+    else:
+        for c, dc in zip(oe.constraints, init_from.constraints):
+            c.intra = np.array(dc.data["intra"], dtype=np.float32)
+            c.inter = np.array(dc.data["inter"], dtype=np.float32)
+            c.sd = c.chi2(c.intra, c.inter)
+        oe.tse = chain.total_error([c.sd for c in oe.constraints])
+    return oe
+
+
+def cpu_moves_per_s(oe, seconds):
     n = 0
     t0 = time.perf_counter()
     while True:
@@ -133,9 +181,23 @@ def cpu_baseline(start, pdc, sfc, seconds):
         n += 5
         el = time.perf_counter() - t0
         if el > seconds:
-            break
-    npairs = len(start["real"]) * (len(start["real"]) - 1) / 2
-    return n / el, n, el, 2 * npairs / t_full
+            return n / el, n, el
+
+
+def cpu_full_histogram_sample(start, elem, dr, H, rows, threads):
+    """The oracle's all-pairs loop (C, OpenMP, private bins) over a sample of
+    `rows` evenly spaced row atoms (each against all later atoms): pairs/s."""
+    from oracle import refcpu
+    from fullrmc_amd import systems
+    box = systems.box_coordinates(start["recip"], start["real"])
+    n = len(box)
+    idx = np.linspace(0, n - 2, rows).astype(np.int32)
+    pairs = float(np.sum(n - 1 - idx.astype(np.int64)))
+    t0 = time.perf_counter()
+    refcpu.multiple_pairs_histograms_coords(idx, box, start["basis"], True, start["molecules"], elem,
+                                            int(elem.max()) + 1, np.float32(0.0), np.float32(dr * H),
+                                            np.float32(dr), H, allAtoms=False, ncores=threads)
+    return pairs / (time.perf_counter() - t0), pairs
 
 
 def eval_flops_per_proposal(H_pdc=1000, H_sfc=1000, nQ=500, T=3):
@@ -147,20 +209,76 @@ def eval_flops_per_proposal(H_pdc=1000, H_sfc=1000, nQ=500, T=3):
 
 
 def kernel_roofline(name, avg_s, n, props_per_launch):
-    """Algorithmic work of one launch of the named engine kernel."""
+    """Algorithmic work of one launch of the named engine kernel, against the
+    FP32 vector peak (the pair path issues no matrix instructions)."""
     if name == "k_delta":
         flops = OPS_PER_PAIR * 2.0 * n * props_per_launch      # old + new position vs all atoms
-    elif name == "k_eval":
+    elif name == "k_eval_a":
         flops = eval_flops_per_proposal() * props_per_launch
+    elif name == "k_eval_b":
+        flops = 2.0 * 500 * props_per_launch                   # SFC squared deviations summed, totals
     elif name == "k_commit":
         flops = OPS_PER_PAIR * 2.0 * n
     else:
         return None
+    if not avg_s > 0:
+        return None
     tf = flops / avg_s / 1e12
     return {"bound": "mfma", "compute": "fp32 VALU (no matrix instructions; MI355X fp32 vector and dense "
-            "matrix peaks are both 157.3 TFLOP/s)", "achieved": round(tf, 3), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "matrix peaks are both 157.3 TFLOP/s)", "achieved": round(tf, 4), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32_TFLOPS, 6), "traffic": None,
             "kernel": name, "avg_launch_us": round(avg_s * 1e6, 2), "flops_per_launch": flops}
+
+
+def sio2_section(args):
+    """200,000-atom SiO2 box: full histogram (float32 reference binning and
+    the binary64-recompute mode) and device moves with k <= 128, beside the
+    CPU oracle on bounded samples."""
+    import torch
+    t0 = time.perf_counter()
+    start, elem, pdc, sfc = build_sio2_workload()
+    n = len(elem)
+    print("bench: SiO2 workload built (%d atoms, %.1f s)" % (n, time.perf_counter() - t0), file=sys.stderr,
+          flush=True)
+    out = {"workload": "SiO2-like 200000 atoms (66667 Si), cube 145 A, min distance 1.5 A; PDC r[0,20]@0.01 "
+                       "(2000 bins) + SFC Q[0.3,30]x1000, single-atom translation 0.1 A, k<=128"}
+    gps, fms, _ = gpu_full_histogram(start, elem, 0.01, 2000)
+    gps64, fms64, re64 = gpu_full_histogram(start, elem, 0.01, 2000, float64=True)
+    tf = gps * OPS_PER_PAIR / 1e12
+    out["full_histogram"] = {"gpairs_per_s": round(gps / 1e9, 2), "kernel_ms": round(fms, 3),
+                             "pairs": n * (n - 1) // 2,
+                             "roofline": {"bound": "mfma", "compute": "fp32 VALU", "achieved": round(tf, 3),
+                                          "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                          "frac": round(tf / PEAK_FP32_TFLOPS, 4)},
+                             "float64_recompute_mode": {"gpairs_per_s": round(gps64 / 1e9, 2),
+                                                        "kernel_ms": round(fms64, 3), "rechecked_pairs": re64}}
+    E = make_engine(start, pdc, sfc, seed=3, dr=0.01)
+    E.run_device(500, k=128, pull=False)
+    torch.cuda.synchronize()
+    acc0 = E.accepted
+    t1 = time.perf_counter()
+    st = E.run_device(args.sio2_moves, k=128, pull=False)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    out["moves_per_s"] = round(args.sio2_moves / el, 1)
+    out["moves"] = args.sio2_moves
+    out["accepted_fraction"] = round((E.accepted - acc0) / args.sio2_moves, 4)
+    out["windows"] = st["windows"]
+    out["proposals_evaluated"] = st["evaluated"]
+    print("bench: SiO2 moves done (%.1f moves/s)" % out["moves_per_s"], file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        E.sync_from_device()
+        oe = cpu_engine(start, pdc, sfc, 0.01, init_from=E)
+        cps, cn, cel = cpu_moves_per_s(oe, args.cpu_seconds / 2)
+        threads = len(os.sched_getaffinity(0))
+        cgp, cpairs = cpu_full_histogram_sample(start, elem, 0.01, 2000, 200, threads)
+        out["cpu_baseline"] = {"value": round(cps, 2), "unit": "moves/s", "cores": 1, "kind": "port",
+                               "sample": "%d moves (oracle, 1 thread) in %.1f s" % (cn, cel),
+                               "full_histogram_gpairs_per_s": round(cgp / 1e9, 4), "full_histogram_cores": threads,
+                               "full_histogram_sample": "200 row atoms vs all later atoms (%.3g pairs)" % cpairs}
+        out["speedup_vs_cpu"] = round(out["moves_per_s"] / cps, 1)
+        out["full_histogram_speedup_vs_cpu_all_cores"] = round(gps / cgp, 1)
+    return out
 
 
 def main():
@@ -173,6 +291,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--profile-every", type=int, default=8,
+                    help="time the kernels of every n-th window with HIP events (0: off)")
+    ap.add_argument("--sio2", action=argparse.BooleanOptionalAction, default=True,
+                    help="also run the 200,000-atom SiO2 box (full histograms and k=128 moves)")
+    ap.add_argument("--sio2-moves", type=int, default=3000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the pair path runs on a single MI355X; "
@@ -193,7 +316,7 @@ def main():
         E.run_device(args.moves, k=args.k, pull=False)
     h = E.device_handle()
     acc0 = E.accepted
-    _lib.check(L.frmc_engine_profile(h, 1))
+    _lib.check(L.frmc_engine_profile(h, args.profile_every))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     evaluated = windows = 0
@@ -234,7 +357,7 @@ def main():
 
     if not args.no_extras:
         # full-histogram recompute (one O(N^2) pass on the PDC grid)
-        gps, fms = gpu_full_histogram(start, elem)
+        gps, fms, _ = gpu_full_histogram(start, elem, 0.02, 1000)
         tf = gps * OPS_PER_PAIR / 1e12
         result["full_histogram"] = {"gpairs_per_s": round(gps / 1e9, 3), "kernel_ms": round(fms, 3),
                                     "pairs": n * (n - 1) // 2,
@@ -257,12 +380,21 @@ def main():
         result["dropin_per_move_loop_moves_per_s"] = round(30 / (time.perf_counter() - t2), 1)
         print("bench: extras done", file=sys.stderr, flush=True)
 
+    if args.sio2:
+        result["sio2_200k"] = sio2_section(args)
+
     if not args.no_cpu:
-        cps, cn, cel, cgp = cpu_baseline(start, pdc, sfc, args.cpu_seconds)
+        oe = cpu_engine(start, pdc, sfc, 0.02)
+        cps, cn, cel = cpu_moves_per_s(oe, args.cpu_seconds)
+        threads = len(os.sched_getaffinity(0))
+        cgp, cpairs = cpu_full_histogram_sample(start, elem, 0.02, 1000, 400, threads)
         result["cpu_baseline"] = {"value": round(cps, 2), "unit": "moves/s", "cores": 1, "kind": "port",
                                   "sample": "%d moves of the same workload (oracle: C restatement of the reference "
                                             "loops + NumPy chain, 1 thread) in %.1f s" % (cn, cel),
-                                  "full_histogram_gpairs_per_s": round(cgp / 1e9, 4)}
+                                  "full_histogram_gpairs_per_s": round(cgp / 1e9, 4),
+                                  "full_histogram_cores": threads,
+                                  "full_histogram_sample": "%d row atoms vs all later atoms (%.3g pairs), oracle C "
+                                                           "loop, OpenMP with private bins" % (400, cpairs)}
         result["speedup_vs_cpu"] = round(value / cps, 1)
     print(json.dumps(result))
 

@@ -139,6 +139,31 @@ def full_pairs_histograms_coords(boxCoords, basis, isPBC, moleculeIndex, element
     return D.download(d_hi), D.download(d_he)
 
 
+def full_pairs_histograms_coords_float64(boxCoords, basis, isPBC, moleculeIndex, elementIndex, numberOfElements,
+                                         minDistance, maxDistance, bin, histSize):
+    """All-pairs histograms of the binary64 recompute: the same pairs and slots
+    as :func:`full_pairs_histograms_coords`, with every distance, range test
+    and bin index as if evaluated in double from the float32 coordinates
+    (float32 kernel with a double re-check of every pair near a bin edge).
+    Not part of the reference's API.  Returns ``(hintra, hinter, rechecked)``
+    with float32 histograms and the number of pairs re-checked in double."""
+    import ctypes
+    (boxCoords, basis, isPBC, mol, el, nel, H, mn, mx, bn) = _coords_common(
+        boxCoords, basis, isPBC, moleculeIndex, elementIndex, numberOfElements, minDistance, maxDistance,
+        bin, histSize, 1)
+    n = boxCoords.shape[0]
+    size = nel * nel * H
+    d_c, d_b, d_mol, d_el = D.upload(boxCoords), D.upload(basis), D.upload(mol), D.upload(el)
+    d_cnt = D.empty((2 * size + 4,), np.int32)
+    rechecked = ctypes.c_int64(0)
+    check(lib().frmc_full_pairs_counts_f64(
+        d_c.data_ptr(), n, d_b.data_ptr(), int(isPBC), d_mol.data_ptr(), d_el.data_ptr(), nel, mn, mx, bn, H,
+        d_cnt.data_ptr(), ctypes.byref(rechecked), D.stream()))
+    c = D.download(d_cnt)
+    return (c[:size].astype(np.float32).reshape(nel, nel, H), c[size:2 * size].astype(np.float32).reshape(nel, nel, H),
+            int(rechecked.value))
+
+
 def full_pairs_histograms_dists(distances, moleculeIndex, elementIndex, numberOfElements, minDistance,
                                 maxDistance, bin, histSize, ncores=1):
     """All-pairs histograms from an (n, k) distance array with

@@ -87,6 +87,8 @@ _PROTOS = {
                                          c_f32, c_f32, c_f32, c_i32, c_vp, c_vp, c_vp],
     "frmc_full_pairs_counts": [c_vp, c_i32, c_vp, c_int, c_vp, c_vp, c_i32,
                                c_f32, c_f32, c_f32, c_i32, c_vp, c_vp],
+    "frmc_full_pairs_counts_f64": [c_vp, c_i32, c_vp, c_int, c_vp, c_vp, c_i32,
+                                   c_f32, c_f32, c_f32, c_i32, c_vp, ctypes.POINTER(ctypes.c_int64), c_vp],
     # boundary_conditions_collection
     "frmc_get_reciprocal_basis": [c_vp, c_vp, ctypes.POINTER(ctypes.c_double), c_vp],
     "frmc_transform_coordinates": [c_vp, c_vp, c_i32, c_vp, c_vp],
@@ -111,7 +113,7 @@ _PROTOS = {
     "frmc_engine_kernel_times": [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64)],
     "frmc_engine_set_adaptive": [c_vp, c_int],
 }
-KERNEL_NAMES = ["k_delta", "k_eval", "k_commit"]
+KERNEL_NAMES = ["k_delta", "k_eval_a", "k_commit", "k_eval_b"]
 _RESTYPES = {"frmc_last_error": ctypes.c_char_p, "frmc_last_overflow": c_i64, "frmc_set_timing": None,
              "frmc_last_kernel_ms": c_f32}
 

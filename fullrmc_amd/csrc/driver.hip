@@ -118,6 +118,7 @@ extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32
     std::vector<int32_t> gids(k), aoff(k + 1);
     std::vector<float> nbox((size_t)k * e->gmax * 3), nreal((size_t)k * e->gmax * 3);
     std::vector<double> uacc(k);
+    std::vector<float> uf(k);
     std::vector<int32_t> py_pre(k), py_post(k), np_used(k);
     std::vector<float> tse(k);
     int64_t step = 0;
@@ -154,7 +155,8 @@ extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32
             off += g;
         }
         aoff[kw] = off;
-        int rc = frmc_engine_eval(e, kw, gids.data(), nbox.data(), tse.data(), nullptr);
+        for (int32_t p = 0; p < kw; p++) uf[p] = (float)uacc[p];
+        int rc = eval_device_decided(e, kw, gids.data(), nbox.data(), uf.data(), tolerance, tse.data());
         if (rc) return rc;
         stats->windows++;
         stats->evaluated += kw;
@@ -192,7 +194,7 @@ extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32
             const int32_t npy = drew ? py_post[acc_p] : py_pre[acc_p];
             while (rp.used < npy) (void)rp.next();
             while (rn.used < np_used[acc_p]) (void)rn.next();
-            rc = frmc_engine_commit(e, acc_p);
+            rc = settle_window(e, acc_p);
             if (rc) return rc;
             const int32_t gi = gids[acc_p], g0 = e->h_goff[gi], g = e->h_goff[gi + 1] - g0;
             for (int32_t a = 0; a < g; a++) {
@@ -201,7 +203,7 @@ extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32
             }
         }
         // without an acceptance both streams already stand after the window
+        else if ((rc = settle_window(e, -1))) return rc;
     }
-    FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
-    return FRMC_OK;
+    return finish_windows(e);
 }

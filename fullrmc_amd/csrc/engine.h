@@ -28,9 +28,7 @@ namespace frmc {
 
 constexpr int kMaxC = 4;   // constraints per engine
 constexpr int kMaxG = 64;  // atoms per group
-constexpr int kPG = 4;     // proposals per k_eval block (one wave each)
-constexpr int kQC = 64;    // Q points per k_eval block (one per lane)
-constexpr int kRT = 128;   // rows of the M tile staged per pass
+constexpr int kQC = 64;    // Q points per k_eval_a work unit (one per lane of its wave)
 
 struct DevC {
     int32_t kind;   // FRMC_PDC or FRMC_SFC
@@ -38,10 +36,14 @@ struct DevC {
     int32_t H, M;   // histogram bins, model points (H for PDC, nQ for SFC)
     int32_t hasW, hasSF;
     float sf;
-    const float *w, *D, *shellV, *pref, *expv, *dataW, *sqm;
+    const float *w, *D, *shellV, *pref, *expv, *dataW;
+    const float *sqm;       // S(Q) matrix transposed: [M][hpb], column q of M[r][q] zero-padded
+    int32_t hpb;            // H rounded up to a multiple of 32
     const int32_t *leaves;  // (start,len) pairs of NumPy's pairwise-sum leaves of length M
     const int32_t *prog;    // postfix combine program over the leaves
     int32_t nleaves, nprog;
+    int32_t sameGrid;       // 1: same histogram grid as the previous constraint (bins reused)
+    int32_t uOff;           // first k_eval_a work unit (PDC: 1 unit, SFC: ceil(M/kQC) units)
     int32_t *base;          // [T][H]
     int32_t *slot;          // [2][nel][nel][H] + overflow counter
     int32_t dnOff;          // offset in a proposal's delta block
@@ -63,7 +65,7 @@ struct View {
     const int32_t *goff, *gatoms;  // groups
     DevC c[kMaxC];
     int32_t dnStride;              // ints per proposal delta block
-    int32_t Hmax, Mmax, maxLeaves, nqc;
+    int32_t Hmax, Mmax, maxLeaves, nunits;
     // window
     const int32_t *pgroup, *poff;  // staging (host-mapped)
     const float *pold, *pnew;
@@ -76,26 +78,46 @@ struct View {
     float *out_tse;                // [kmax] host-mapped
     float *out_chi2;               // [kmax][C] host-mapped
     volatile uint32_t *out_done;   // [kmax] host-mapped completion sequence numbers
+    float *tse_dev;                // [kmax] totals in device memory (read by k_commit)
+    volatile int32_t *out_commit;  // [2][2] host-mapped: proposal committed by k_commit, its window's seq
     uint32_t seq;
 };
 
 // Windows of single-atom moves travel to k_delta in the kernel arguments
-// (atom index and new box coordinates per proposal, old coordinates read from
-// HBM), which avoids reading pinned host memory at kernel start.
+// (atom index, element, old and new box coordinates per proposal, from the
+// host mirror), so the kernel reads neither pinned host memory nor the
+// proposal atoms' rows before its main loop.
 constexpr int kInline = 64;
 struct Inline {
     int32_t k;  // 0: use the staging buffers
     int32_t atom[kInline];
+    int32_t el[kInline];
+    float old[kInline][3];
     float nw[kInline][3];
 };
 
+// Which proposal of a window k_commit commits: p >= 0 a given one (the
+// caller decided), p < 0 the first proposal that the reference's rule accepts
+// (Engine.py:3308-3317) given the current total cur, the tolerance and the
+// window's acceptance draws u (each rounded to float32, as the comparison
+// random() > tolerance is made); none accepted: nothing is committed.
+struct CommitCtl {
+    int32_t k, p;
+    float cur, tol;
+    const int32_t *dn;  // the window's delta buffer (pair-type deltas per proposal)
+    float u[kInline];
+};
+
+constexpr int kDP = 4;  // proposals per k_delta block
+
 // launchers (engine_kernels.hip)
-int launch_delta(const View &V, const Inline &I, int32_t k, int32_t nchunk, hipStream_t st);
+int launch_delta(const View &V, const Inline &I, int32_t k, hipStream_t st);
 int launch_eval(const View &V, int32_t k, hipStream_t st);
-int launch_commit(const View &V, int32_t p, hipStream_t st);
+int launch_eval_b(const View &V, int32_t k, hipStream_t st);
+int launch_commit(const View &V, const Inline &I, const CommitCtl &C, hipStream_t st);
 int launch_symmetrise(int32_t *base, const int32_t *slot, int32_t nel, int32_t H, hipStream_t st);
 int launch_fill(int32_t *p, int64_t n, hipStream_t st);
-int launch_full(const float *coords, int32_t n, const float *basis, int isPBC, bool ortho, const int32_t *mol,
+int launch_full(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho, const int32_t *mol,
                 const int32_t *elem, const HistArgs &A, int32_t *counts, hipStream_t st);
 void pairwise_plan(int32_t n, std::vector<int32_t> &leaves, std::vector<int32_t> &prog);
 size_t eval_lds_bytes(const View &V);
@@ -110,9 +132,9 @@ struct frmc_engine {
     frmc::Basis B;
     float *xyz = nullptr, *dbasis = nullptr;
     int32_t *mol = nullptr, *elem = nullptr, *goff = nullptr, *gatoms = nullptr;
-    std::vector<int32_t> h_goff, h_gatoms;
+    std::vector<int32_t> h_goff, h_gatoms, h_elem;
     frmc::DevC c[frmc::kMaxC];
-    int32_t dnStride = 0, Hmax = 0, Mmax = 0, maxLeaves = 1;
+    int32_t dnStride = 0, Hmax = 0, Mmax = 0, maxLeaves = 1, nunits = 0;
     // window buffers
     int32_t *dnbuf[2] = {nullptr, nullptr};
     int cur_dn = 0;
@@ -123,6 +145,12 @@ struct frmc_engine {
     int cur_stage = 0;     // staging buffer of the last evaluated window
     float *h_tse = nullptr, *h_chi2 = nullptr;
     uint32_t *h_done = nullptr;
+    float *tse_dev = nullptr;
+    int32_t *h_commit = nullptr;     // [2][2] written by k_commit (slot = window parity)
+    bool device_decided = false;     // the last window's commit was queued on the device
+    int32_t pending_check = -2;      // decision of the last device-decided window (-2: none)
+    uint32_t pending_seq = 0;
+    frmc::Inline last_inline;        // proposals of the last evaluated window (k = 0: staged)
     uint32_t seq = 0;
     int32_t last_k = 0;
     // host mirror of the accepted state
@@ -135,6 +163,7 @@ struct frmc_engine {
     std::vector<void *> allocs, host_allocs;
     // optional per-kernel timing with HIP events on the engine stream
     bool profile = false;
+    int profile_every = 1;  // time the launches of every n-th window
     struct Pending {
         int kid;
         hipEvent_t a, b;
@@ -147,6 +176,10 @@ struct frmc_engine {
 
 namespace frmc {
 View make_view(frmc_engine *e, int stage_idx);
+int eval_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
+                        const float *u, float tol, float *tse_out);
+int settle_window(frmc_engine *e, int32_t p);
+int finish_windows(frmc_engine *e);
 void collect_times(frmc_engine *e);
 
 // bracket one kernel launch with events when profiling is on

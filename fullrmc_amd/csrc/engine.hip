@@ -38,10 +38,7 @@ View make_view(frmc_engine *e, int s) {
     V.Hmax = e->Hmax;
     V.Mmax = e->Mmax;
     V.maxLeaves = e->maxLeaves;
-    int32_t nqc = 1;
-    for (int i = 0; i < e->C; i++)
-        if (e->c[i].kind == FRMC_SFC) nqc = std::max(nqc, (e->c[i].M + kQC - 1) / kQC);
-    V.nqc = nqc;
+    V.nunits = e->nunits;
     V.pgroup = e->stage[s].pgroup;
     V.poff = e->stage[s].poff;
     V.pold = e->stage[s].pold;
@@ -55,6 +52,8 @@ View make_view(frmc_engine *e, int s) {
     V.out_tse = e->h_tse;
     V.out_chi2 = e->h_chi2;
     V.out_done = e->h_done;
+    V.tse_dev = e->tse_dev;
+    V.out_commit = e->h_commit;
     V.seq = e->seq;
     return V;
 }
@@ -71,7 +70,7 @@ static hipEvent_t take_event(frmc_engine *e) {
 }
 
 KTimer::KTimer(frmc_engine *e_, int kid_) : e(e_), kid(kid_) {
-    if (e->profile) {
+    if (e->profile && e->seq % (uint32_t)e->profile_every == 0) {
         a = take_event(e);
         (void)hipEventRecord(a, e->st);
     }
@@ -145,14 +144,6 @@ static void free_engine(frmc_engine *e) {
     delete e;
 }
 
-// blocks of the delta kernel per proposal: enough for ~2 blocks per CU, at
-// least 1024 atoms per block
-static int32_t delta_chunks(frmc_engine *e, int32_t k) {
-    int64_t want = (2LL * e->ncu + k - 1) / k;
-    int64_t maxc = std::max<int64_t>(1, e->n / 1024);
-    return (int32_t)std::max<int64_t>(1, std::min(want, maxc));
-}
-
 static double now_s() {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -185,15 +176,31 @@ static int wait_done(frmc_engine *e, int32_t k) {
     return FRMC_OK;
 }
 
-// launch delta + eval for the k proposals in staging buffer s and wait
-static int run_window(frmc_engine *e, int s, int32_t k, const Inline *I) {
+// The decision of the last device-decided window must match the host's: k_commit
+// has completed once a later window's flags are set (same stream).
+static int check_device_decision(frmc_engine *e) {
+    if (e->pending_check == -2) return FRMC_OK;
+    volatile int32_t *hc = e->h_commit + 2 * (e->pending_seq & 1);
+    if ((uint32_t)hc[1] != e->pending_seq || hc[0] != e->pending_check) {
+        set_error("device acceptance decision (proposal %d, window %u) differs from the host's (%d, %u)", hc[0],
+                  (unsigned)hc[1], e->pending_check, (unsigned)e->pending_seq);
+        e->pending_check = -2;
+        return FRMC_ESTATE;
+    }
+    e->pending_check = -2;
+    return FRMC_OK;
+}
+
+// launch delta + eval for the k proposals in staging buffer s (and, with C,
+// the commit of the proposal the device accepts) and wait for the totals
+static int run_window(frmc_engine *e, int s, int32_t k, const Inline *I, const CommitCtl *C = nullptr) {
     const bool with_delta = I != nullptr;
     e->seq++;
     View V = make_view(e, s);
     int rc;
     if (with_delta) {
         KTimer t(e, FRMC_K_DELTA);
-        rc = launch_delta(V, *I, k, delta_chunks(e, k), e->st);
+        rc = launch_delta(V, *I, k, e->st);
         if (rc) return rc;
     }
     {
@@ -201,19 +208,34 @@ static int run_window(frmc_engine *e, int s, int32_t k, const Inline *I) {
         rc = launch_eval(V, k, e->st);
     }
     if (rc) return rc;
+    {
+        KTimer t(e, FRMC_K_EVAL_B);
+        rc = launch_eval_b(V, k, e->st);
+    }
+    if (rc) return rc;
+    if (C) {
+        KTimer t(e, FRMC_K_COMMIT);
+        rc = launch_commit(V, *I, *C, e->st);
+        if (rc) return rc;
+    }
     // k_eval_b zeroed the other buffer for the next window
     e->dn_used[1 - e->cur_dn] = 0;
     e->dn_used[e->cur_dn] = with_delta ? k : 0;
     e->cur_dn = 1 - e->cur_dn;
     rc = wait_done(e, k);
     if (rc) return rc;
+    rc = check_device_decision(e);
+    if (rc) return rc;
     if (e->profile) collect_times(e);
     return FRMC_OK;
 }
 
 // Stage k proposals (group ids + new box coordinates; old ones from the host
-// mirror) into the staging buffer not read by an in-flight commit.
-static int stage_and_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box) {
+// mirror) into the staging buffer not read by an in-flight commit, and
+// evaluate them.  With u (acceptance draws) the device also commits the
+// proposal the reference's rule accepts, queued behind the evaluation.
+static int stage_and_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
+                          const float *u = nullptr, float tol = 0.0f) {
     const int s = 1 - e->cur_stage;
     Stage &S = e->stage[s];
     int32_t off = 0;
@@ -233,19 +255,86 @@ static int stage_and_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, c
     memcpy(S.pnew, new_box, sizeof(float) * 3 * (size_t)off);
     __sync_synchronize();
     e->cur_stage = s;
-    Inline I;
+    Inline &I = e->last_inline;
     I.k = 0;
     if (e->gmax == 1 && k <= kInline) {
         I.k = k;
         for (int32_t p = 0; p < k; p++) {
-            I.atom[p] = e->h_gatoms[e->h_goff[group_ids[p]]];
-            for (int d = 0; d < 3; d++) I.nw[p][d] = new_box[3 * (size_t)p + d];
+            const int32_t id = e->h_gatoms[e->h_goff[group_ids[p]]];
+            I.atom[p] = id;
+            I.el[p] = e->h_elem[id];
+            for (int d = 0; d < 3; d++) {
+                I.old[p][d] = e->h_box[3 * (size_t)id + d];
+                I.nw[p][d] = new_box[3 * (size_t)p + d];
+            }
         }
     }
-    int rc = run_window(e, s, k, &I);
+    int rc;
+    if (u && k <= kInline) {
+        CommitCtl C;
+        C.k = k;
+        C.p = -1;
+        C.cur = e->cur_tse;
+        C.tol = tol;
+        C.dn = e->dnbuf[e->cur_dn];  // the buffer this window's deltas go to
+        for (int32_t p = 0; p < k; p++) C.u[p] = u[p];
+        rc = run_window(e, s, k, &I, &C);
+        e->device_decided = true;
+    } else {
+        rc = run_window(e, s, k, &I);
+        e->device_decided = false;
+    }
     if (rc) return rc;
     e->last_k = k;
     return FRMC_OK;
+}
+
+// host mirror of a committed proposal p of the last window (the group's new
+// box coordinates, the totals)
+static void note_commit(frmc_engine *e, int32_t p) {
+    const Stage &S = e->stage[e->cur_stage];
+    const int32_t gi = S.pgroup[p], a0 = S.poff[p];
+    const int32_t g0 = e->h_goff[gi], g = e->h_goff[gi + 1] - g0;
+    for (int32_t a = 0; a < g; a++) {
+        const size_t id = (size_t)e->h_gatoms[g0 + a];
+        for (int d = 0; d < 3; d++) e->h_box[3 * id + d] = S.pnew[3 * (size_t)(a0 + a) + d];
+    }
+    e->cur_tse = e->h_tse[p];
+    for (int ci = 0; ci < e->C; ci++) e->cur_chi2[ci] = e->h_chi2[(int64_t)p * e->C + ci];
+    e->last_k = 0;  // the window now describes a stale state
+}
+
+// Evaluate a window whose commit the device decides; the host applies the
+// same rule to the returned totals, then calls settle_window.
+int eval_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
+                        const float *u, float tol, float *tse_out) {
+    int rc = stage_and_eval(e, k, group_ids, new_box, u, tol);
+    if (rc) return rc;
+    memcpy(tse_out, e->h_tse, sizeof(float) * k);
+    return FRMC_OK;
+}
+
+// wait for the stream and verify the last device-made decision
+int finish_windows(frmc_engine *e) {
+    FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
+    return check_device_decision(e);
+}
+
+// The host's decision for the last window: p >= 0 committed (by the device
+// when it decided, else by a commit launched here), -1 none.
+int settle_window(frmc_engine *e, int32_t p) {
+    if (e->device_decided) {
+        e->pending_check = p;
+        e->pending_seq = e->seq;
+        if (p >= 0) note_commit(e, p);
+        else e->last_k = 0;
+        return FRMC_OK;
+    }
+    if (p < 0) {
+        e->last_k = 0;
+        return FRMC_OK;
+    }
+    return frmc_engine_commit(e, p);
 }
 
 }  // namespace frmc
@@ -316,6 +405,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         ECHK(dupload(e, &e->dbasis, d->basis, 9));
         ECHK(dupload(e, &e->mol, d->mol, (size_t)d->n));
         ECHK(dupload(e, &e->elem, d->elem, (size_t)d->n));
+        e->h_elem.assign(d->elem, d->elem + d->n);
         ECHK(dupload(e, &e->goff, d->group_offsets, (size_t)d->ngroups + 1));
         ECHK(dupload(e, &e->gatoms, d->group_atoms, (size_t)e->total_gatoms));
         e->h_box.assign(d->box, d->box + 3 * (size_t)d->n);
@@ -335,7 +425,14 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             dc.H = cd.histSize;
             dc.M = cd.kind == FRMC_PDC ? cd.histSize : cd.nQ;
             if (dc.M < 1) EFAIL(FRMC_EINVAL, "constraint %d: no model points", ci);
+            if ((int64_t)e->nel * e->nel * cd.histSize * 2 + 1 > INT32_MAX)
+                EFAIL(FRMC_EINVAL, "constraint %d: nel^2*histSize too large", ci);
             dc.g = make_grid(cd.minDistance, cd.maxDistance, cd.bin, cd.histSize);
+            if (ci > 0) {
+                const Grid &pg = e->c[ci - 1].g;
+                dc.sameGrid = pg.minD == dc.g.minD && pg.maxD == dc.g.maxD && pg.bin == dc.g.bin &&
+                              pg.s_lo == dc.g.s_lo && pg.s_hi == dc.g.s_hi && pg.H == dc.g.H;
+            }
             dc.hasW = cd.dataWeights != nullptr;
             dc.hasSF = cd.scaleFactor != 1.0f;
             dc.sf = cd.scaleFactor;
@@ -349,7 +446,15 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             ECHK(dupload(e, &pf, cd.prefactor, (size_t)dc.H));
             ECHK(dupload(e, &ex, cd.experimental, (size_t)dc.M));
             if (cd.dataWeights) ECHK(dupload(e, &dw, cd.dataWeights, (size_t)dc.M));
-            if (cd.kind == FRMC_SFC) ECHK(dupload(e, &sm, cd.sqMatrix, (size_t)dc.H * dc.M));
+            if (cd.kind == FRMC_SFC) {
+                // k_eval_a streams one Q column per lane: the matrix is kept transposed,
+                // each column zero-padded to a multiple of the kernel's 32-row batches
+                dc.hpb = (dc.H + 31) / 32 * 32;
+                std::vector<float> mt((size_t)dc.M * dc.hpb, 0.0f);
+                for (int32_t r = 0; r < dc.H; r++)
+                    for (int32_t q = 0; q < dc.M; q++) mt[(size_t)q * dc.hpb + r] = cd.sqMatrix[(size_t)r * dc.M + q];
+                ECHK(dupload(e, &sm, mt.data(), mt.size()));
+            }
             dc.w = wp;
             dc.D = Dp;
             dc.shellV = sv;
@@ -375,9 +480,12 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             e->maxLeaves = std::max(e->maxLeaves, dc.nleaves);
         }
         e->dnStride = off;
-        if ((size_t)e->dnStride * 4 > 150 * 1024)
-            EFAIL(FRMC_EINVAL, "pair-type histograms of all constraints need %d KB of LDS (max 150)",
-                  e->dnStride * 4 / 1024);
+        int32_t units = 0;
+        for (int ci = 0; ci < e->C; ci++) {
+            e->c[ci].uOff = units;
+            units += e->c[ci].kind == FRMC_PDC ? 1 : (e->c[ci].M + kQC - 1) / kQC;
+        }
+        e->nunits = units;
         {
             View V = make_view(e, 0);
             if (eval_lds_bytes(V) > 150 * 1024) EFAIL(FRMC_EINVAL, "model arrays need more than 150 KB of LDS");
@@ -396,6 +504,9 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         ECHK(halloc(e, &e->h_tse, K));
         ECHK(halloc(e, &e->h_chi2, K * e->C));
         ECHK(halloc(e, &e->h_done, K));
+        ECHK(dalloc(e, &e->tse_dev, K));
+        ECHK(halloc(e, &e->h_commit, 4));
+        for (int q = 0; q < 4; q++) e->h_commit[q] = q % 2 ? 0 : -1;
         ECHK(hipDeviceSynchronize());
     }
 #undef ECHK
@@ -422,7 +533,7 @@ int frmc_engine_compute_data(frmc_engine *e, float *chi2_out, float *tse_out) {
         A.nel = e->nel;
         A.size = (int64_t)e->nel * e->nel * dc.H;
         int rc = launch_fill(dc.slot, 2 * A.size + 1, st);
-        if (!rc) rc = launch_full(e->xyz, e->n, e->dbasis, e->isPBC, e->ortho != 0, e->mol, e->elem, A, dc.slot, st);
+        if (!rc) rc = launch_full(e->xyz, e->n, e->B, e->isPBC, e->ortho != 0, e->mol, e->elem, A, dc.slot, st);
         if (!rc) rc = launch_symmetrise(dc.base, dc.slot, e->nel, dc.H, st);
         if (rc) return rc;
     }
@@ -461,29 +572,27 @@ int frmc_engine_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const 
 int frmc_engine_commit(frmc_engine *e, int32_t p) {
     FRMC_REQUIRE(e, "NULL engine");
     FRMC_REQUIRE(p >= 0 && p < e->last_k, "proposal %d not in the last evaluated window (k=%d)", p, e->last_k);
+    FRMC_REQUIRE(!e->device_decided, "the device already committed the last window");
     View V = make_view(e, e->cur_stage);
+    CommitCtl C;
+    C.k = e->last_k;
+    C.p = p;
+    C.cur = e->cur_tse;
+    C.tol = 0.0f;
+    C.dn = e->dnbuf[1 - e->cur_dn];  // the window's deltas (the other buffer is current)
     int rc;
     {
         KTimer t(e, FRMC_K_COMMIT);
-        rc = launch_commit(V, p, e->st);
+        rc = launch_commit(V, e->last_inline, C, e->st);
     }
     if (rc) return rc;
-    e->cur_tse = e->h_tse[p];
-    for (int ci = 0; ci < e->C; ci++) e->cur_chi2[ci] = e->h_chi2[(int64_t)p * e->C + ci];
-    const Stage &S = e->stage[e->cur_stage];
-    const int32_t gi = S.pgroup[p], a0 = S.poff[p];
-    for (int32_t a = 0; a < e->h_goff[gi + 1] - e->h_goff[gi]; a++) {
-        const size_t id = (size_t)e->h_gatoms[e->h_goff[gi] + a];
-        for (int d = 0; d < 3; d++) e->h_box[3 * id + d] = S.pnew[3 * (size_t)(a0 + a) + d];
-    }
-    e->last_k = 0;  // the window now describes a stale state
+    note_commit(e, p);
     return FRMC_OK;
 }
 
 int frmc_engine_sync(frmc_engine *e) {
     FRMC_REQUIRE(e, "NULL engine");
-    FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
-    return FRMC_OK;
+    return finish_windows(e);
 }
 
 int frmc_engine_get_state(frmc_engine *e, float *box_out, int32_t constraint, int32_t *slot_out, int32_t *base_out,
@@ -542,11 +651,13 @@ int frmc_engine_set_adaptive(frmc_engine *e, int enable) {
     return FRMC_OK;
 }
 
-int frmc_engine_profile(frmc_engine *e, int enable) {
+int frmc_engine_profile(frmc_engine *e, int every) {
     FRMC_REQUIRE(e, "NULL engine");
+    FRMC_REQUIRE(every >= 0, "every must be >= 0");
     FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
     collect_times(e);
-    e->profile = enable != 0;
+    e->profile = every != 0;
+    e->profile_every = every > 0 ? every : 1;
     for (int q = 0; q < FRMC_NKERNELS; q++) {
         e->kernel_ms[q] = 0.0;
         e->kernel_launches[q] = 0;

@@ -4,64 +4,65 @@
 //   k_delta   "moved group vs all N" before/after binning of k proposals for
 //             every constraint grid in one pass (the distance is computed once
 //             and binned per grid; the reference recomputes it per
-//             constraint).  Grid (chunks of the N atoms, proposals); each
-//             block accumulates int32 pair-type deltas in LDS and merges them
-//             once (plain stores for one chunk per proposal, atomics into the
-//             zeroed delta buffer otherwise).
-//   k_eval_a  grid (Q chunks of 64, groups of 4 proposals), one wave per
-//             proposal: base + delta -> G(r) in the reference's float32
-//             operation order (PairDistributionConstraints.py:854-895), the
-//             PDC chi-square by NumPy's pairwise sum (:810-838) in the first
-//             Q chunk, and S(Q) as the sequential-in-r float32 sum of
-//             G[r]*M[r,q] (StructureFactorConstraints.py:772-773): one Q per
-//             lane, M staged through LDS in tiles of 32 rows shared by the 4
-//             proposals of the block.
+//             constraint).  Grid (chunks of 256 atoms, groups of kDP
+//             proposals), one other atom per thread; int32 pair-type deltas
+//             go by atomics into the proposal's zeroed block of the window's
+//             delta buffer (a move touches a few thousand bins).
+//   k_eval_a  grid (work units, proposals), one wave each: base + delta ->
+//             G(r) in the reference's float32 operation order
+//             (PairDistributionConstraints.py:854-895); per PDC constraint the
+//             chi-square by NumPy's pairwise sum (:810-838), per 64 Q points of
+//             an SFC constraint S(Q) as the sequential-in-r float32 sum of
+//             G[r]*M[r,q] (StructureFactorConstraints.py:772-773), one Q per
+//             lane, M rows read coalesced from HBM/L2.
 //   k_eval_b  grid k: SFC chi-squares (pairwise), the total np.sum([chi2_c])
 //             (Engine.py:3026-3029), results and a completion flag written to
 //             pinned host memory; also zeroes the other delta buffer.
 //   k_commit  data - before + after for the accepted proposal: slot-exact
 //             counts (M - F slot convention, PairDistributionConstraints.py:
-//             1053-1078) and pair-type sums by atomics, then the new
-//             coordinates of the group (written only by block 0; no other
-//             block reads the group's coordinates).
+//             1053-1078) by atomics, pair-type sums += the proposal's deltas,
+//             then the new coordinates of the group (written only by block 0;
+//             the group's old and new coordinates come in the arguments).
 #include <algorithm>
 
 #include "engine.h"
 
 namespace frmc {
 
+// Loads of data that an earlier kernel of the window (or the host) wrote go
+// through coherent vector loads: a uniform address would otherwise be read
+// through the scalar data cache, which is not guaranteed to be refreshed
+// between back-to-back kernels of a stream.
+template <typename T>
+__device__ __forceinline__ T load_dev(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T load_host(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __device__ __forceinline__ int32_t pair_type(int32_t a, int32_t b, int32_t nel) {
     int32_t lo = a < b ? a : b, hi = a < b ? b : a;
     return lo * (2 * nel - lo + 1) / 2 + (hi - lo);
 }
 
-template <bool PBC, bool ORTHO>
-__device__ __forceinline__ float eng_pair_s(float ax, float ay, float az, float cx, float cy, float cz,
-                                            const Basis &B) {
-    float dx = ax - cx, dy = ay - cy, dz = az - cz;
-    float rx, ry, rz;
-    if (!PBC) {
-        rx = dx; ry = dy; rz = dz;
-    } else if (ORTHO) {
-        pbc_real_ortho(dx, dy, dz, B, rx, ry, rz);
-    } else {
-        pbc_real(dx, dy, dz, B, rx, ry, rz);
-    }
-    return norm2(rx, ry, rz);
+// Flat index [ea][eb][b] of the C-ordered (nel,nel,H) slot array, -1 past its
+// end: a bin b >= H (a distance just below maxDistance) aliases into a later
+// slot exactly as the reference's unchecked write does.
+__device__ __forceinline__ int32_t flat_slot(int32_t b, int32_t ea, int32_t eb, int32_t nel, int32_t H) {
+    const int32_t f = (ea * nel + eb) * H + b;
+    return f < nel * nel * H ? f : -1;
 }
 
-// Flat slot index [ea][eb][bin] of an in-range pair (bin == H aliases the
-// next slot as in the reference); -1 past the (nel,nel,H) array.
-__device__ __forceinline__ int64_t slot_index(float s, const DevC &c, int32_t ea, int32_t eb, int32_t nel) {
-    int64_t flat = ((int64_t)ea * nel + eb) * c.H + bin_of(s, c.g);
-    return (flat < 0 || flat >= (int64_t)nel * nel * c.H) ? -1 : flat;
-}
-
-// pair-type index (t*H + bin) of a flat slot index
-__device__ __forceinline__ int32_t type_of_slot(int64_t flat, const DevC &c, int32_t nel) {
-    int32_t a = (int32_t)(flat / ((int64_t)nel * c.H));
-    int32_t b = (int32_t)((flat / c.H) % nel);
-    return pair_type(a, b, nel) * c.H + (int32_t)(flat % c.H);
+// Pair-type index t*H + r of an in-range pair (the symmetrised sums G(r)
+// reads), consistent with flat_slot: direct for b < H, decoded from the flat
+// slot in the rare aliased case; -1 past the array.
+__device__ __forceinline__ int32_t type_index(int32_t b, int32_t ea, int32_t eb, int32_t nel, int32_t H) {
+    if (b < H) return pair_type(ea, eb, nel) * H + b;
+    const int32_t f = flat_slot(b, ea, eb, nel, H);
+    if (f < 0) return -1;
+    return pair_type(f / (nel * H), (f / H) % nel, nel) * H + f % H;
 }
 
 // group-internal pair q of a group of g atoms -> positions a > b
@@ -77,108 +78,109 @@ __device__ __forceinline__ void decode_pair(int32_t q, int32_t &a, int32_t &b) {
 struct GroupLds {
     float o[kMaxG][3], w[kMaxG][3];
     int32_t id[kMaxG], el[kMaxG], mol[kMaxG];
+    int32_t g;
 };
 
-__device__ __forceinline__ int32_t load_group(const View &V, int32_t p, GroupLds &G) {
-    const int32_t gi = V.pgroup[p];
-    const int32_t a0 = V.poff[p];
-    const int32_t g = V.poff[p + 1] - a0;
+// proposal p of the staged window into G (all threads of the block call it)
+__device__ __forceinline__ void load_group(const View &V, int32_t p, GroupLds &G) {
+    const int32_t gi = load_host(V.pgroup + p);
+    const int32_t a0 = load_host(V.poff + p);
+    const int32_t g = load_host(V.poff + p + 1) - a0;
     const int32_t *atoms = V.gatoms + V.goff[gi];
+    if (threadIdx.x == 0) G.g = g;
     for (int32_t a = threadIdx.x; a < g; a += blockDim.x) {
         const int32_t id = atoms[a];
         G.id[a] = id;
         G.el[a] = V.elem[id];
         G.mol[a] = V.mol[id];
         for (int d = 0; d < 3; d++) {
-            G.o[a][d] = V.pold[3 * (int64_t)(a0 + a) + d];
-            G.w[a][d] = V.pnew[3 * (int64_t)(a0 + a) + d];
+            G.o[a][d] = load_host(V.pold + 3 * (int64_t)(a0 + a) + d);
+            G.w[a][d] = load_host(V.pnew + 3 * (int64_t)(a0 + a) + d);
         }
     }
-    return g;
 }
 
 // ----------------------------------------------------------------- k_delta
 
+// -1 at the old and +1 at the new bin of one (group atom, other atom) pair,
+// per constraint grid (bins are reused when a grid repeats the previous one)
 template <bool PBC, bool ORTHO>
-__global__ void __launch_bounds__(256) k_delta(View V, Inline I, int32_t chunkLen, int32_t merge_atomic) {
-    extern __shared__ int32_t lds[];  // V.dnStride ints
-    __shared__ GroupLds G;
-    const int32_t p = blockIdx.y;
-    const int tid = threadIdx.x;
-    for (int32_t s = tid; s < V.dnStride; s += blockDim.x) lds[s] = 0;
-    int32_t g;
-    if (I.k > 0) {
-        g = 1;
-        if (tid == 0) {
-            const int32_t id = I.atom[p];
-            G.id[0] = id;
-            G.el[0] = V.elem[id];
-            G.mol[0] = V.mol[id];
-            for (int d = 0; d < 3; d++) {
-                G.o[0][d] = V.xyz[3 * (int64_t)id + d];
-                G.w[0][d] = I.nw[p][d];
-            }
+__device__ __forceinline__ void delta_pair(const View &V, int32_t *out, float so, float sn, int32_t ea, int32_t eb) {
+    int32_t bo = -1, bn = -1;
+    for (int32_t ci = 0; ci < V.C; ci++) {
+        const DevC &c = V.c[ci];
+        if (!c.sameGrid) {
+            bo = in_range(so, c.g) ? bin_of(so, c.g) : -1;
+            bn = in_range(sn, c.g) ? bin_of(sn, c.g) : -1;
         }
-    } else {
-        g = load_group(V, p, G);
+        if (bo >= 0) {
+            const int32_t t = type_index(bo, ea, eb, V.nel, c.H);
+            if (t >= 0) atomicSub(&out[c.dnOff + t], 1);
+        }
+        if (bn >= 0) {
+            const int32_t t = type_index(bn, ea, eb, V.nel, c.H);
+            if (t >= 0) atomicAdd(&out[c.dnOff + t], 1);
+        }
     }
-    __syncthreads();
+}
 
-    const int32_t j0 = blockIdx.x * chunkLen;
-    const int32_t j1 = min(V.n, j0 + chunkLen);
-    for (int32_t j = j0 + tid; j < j1; j += blockDim.x) {
+// grid (ceil(n/256), ceil(k/kDP)): one other atom per thread, kDP proposals
+// per block; deltas go straight into the proposal's (zeroed) block of dn.
+template <bool PBC, bool ORTHO>
+__global__ void __launch_bounds__(256) k_delta(View V, Inline I, int32_t k) {
+    __shared__ GroupLds G[kDP];
+    const int tid = threadIdx.x;
+    const int32_t p0 = blockIdx.y * kDP;
+    const int32_t np = min(kDP, k - p0);
+    const bool inl = I.k > 0;
+    if (!inl) {
+        for (int32_t q = 0; q < np; q++) load_group(V, p0 + q, G[q]);
+        __syncthreads();
+    }
+    const int32_t j = blockIdx.x * blockDim.x + tid;
+    const bool jok = j < V.n;
+    const int32_t jj = min(j, V.n - 1);  // clamped: loads stay in bounds
+    const float cx = load_dev(V.xyz + 3 * (int64_t)jj), cy = load_dev(V.xyz + 3 * (int64_t)jj + 1),
+                cz = load_dev(V.xyz + 3 * (int64_t)jj + 2);
+    const int32_t ej = V.elem[jj];
+    for (int32_t q = 0; q < np; q++) {
+        const int32_t p = p0 + q;
+        int32_t *out = V.dn + (int64_t)p * V.dnStride;
+        if (inl) {
+            // single-atom proposal: everything about the moved atom comes in the
+            // kernel arguments
+            const int32_t id = I.atom[p];
+            const int32_t ea = I.el[p];
+            // M counts (moved atom a, other j) with diff = c[a] - c[j], before and after
+            const float so = pair_s<PBC, ORTHO>(I.old[p][0], I.old[p][1], I.old[p][2], cx, cy, cz, V.B);
+            const float sn = pair_s<PBC, ORTHO>(I.nw[p][0], I.nw[p][1], I.nw[p][2], cx, cy, cz, V.B);
+            if (jok && j != id) delta_pair<PBC, ORTHO>(V, out, so, sn, ea, ej);
+            continue;
+        }
+        const GroupLds &Gq = G[q];
+        const int32_t g = Gq.g;
         bool member = false;
-        for (int32_t a = 0; a < g; a++) member |= (G.id[a] == j);
-        if (member) continue;
-        const float cx = V.xyz[3 * (int64_t)j], cy = V.xyz[3 * (int64_t)j + 1], cz = V.xyz[3 * (int64_t)j + 2];
-        const int32_t ej = V.elem[j];
-        for (int32_t a = 0; a < g; a++) {
-            // M counts (atom a, other j) with diff = c[a] - c[j], before and after the move
-            const float so = eng_pair_s<PBC, ORTHO>(G.o[a][0], G.o[a][1], G.o[a][2], cx, cy, cz, V.B);
-            const float sn = eng_pair_s<PBC, ORTHO>(G.w[a][0], G.w[a][1], G.w[a][2], cx, cy, cz, V.B);
-            for (int32_t ci = 0; ci < V.C; ci++) {
-                const DevC &c = V.c[ci];
-                if (in_range(so, c.g)) {
-                    int64_t f = slot_index(so, c, G.el[a], ej, V.nel);
-                    if (f >= 0) atomicSub(&lds[c.dnOff + type_of_slot(f, c, V.nel)], 1);
-                }
-                if (in_range(sn, c.g)) {
-                    int64_t f = slot_index(sn, c, G.el[a], ej, V.nel);
-                    if (f >= 0) atomicAdd(&lds[c.dnOff + type_of_slot(f, c, V.nel)], 1);
-                }
+        for (int32_t a = 0; a < g; a++) member |= (Gq.id[a] == j);
+        if (jok && !member) {
+            for (int32_t a = 0; a < g; a++) {
+                const float so = pair_s<PBC, ORTHO>(Gq.o[a][0], Gq.o[a][1], Gq.o[a][2], cx, cy, cz, V.B);
+                const float sn = pair_s<PBC, ORTHO>(Gq.w[a][0], Gq.w[a][1], Gq.w[a][2], cx, cy, cz, V.B);
+                delta_pair<PBC, ORTHO>(V, out, so, sn, Gq.el[a], ej);
             }
         }
-    }
-    // group-internal pairs: M - F leaves each unordered pair once, in slot
-    // [e(later position), e(earlier position)]
-    if (blockIdx.x == 0 && g > 1) {
-        for (int32_t q = tid; q < g * (g - 1) / 2; q += blockDim.x) {
-            int32_t a, b;
-            decode_pair(q, a, b);
-            const float so = eng_pair_s<PBC, ORTHO>(G.o[a][0], G.o[a][1], G.o[a][2], G.o[b][0], G.o[b][1], G.o[b][2], V.B);
-            const float sn = eng_pair_s<PBC, ORTHO>(G.w[a][0], G.w[a][1], G.w[a][2], G.w[b][0], G.w[b][1], G.w[b][2], V.B);
-            for (int32_t ci = 0; ci < V.C; ci++) {
-                const DevC &c = V.c[ci];
-                if (in_range(so, c.g)) {
-                    int64_t f = slot_index(so, c, G.el[a], G.el[b], V.nel);
-                    if (f >= 0) atomicSub(&lds[c.dnOff + type_of_slot(f, c, V.nel)], 1);
-                }
-                if (in_range(sn, c.g)) {
-                    int64_t f = slot_index(sn, c, G.el[a], G.el[b], V.nel);
-                    if (f >= 0) atomicAdd(&lds[c.dnOff + type_of_slot(f, c, V.nel)], 1);
-                }
+        // group-internal pairs: M - F leaves each unordered pair once, in slot
+        // [e(later position), e(earlier position)]
+        if (blockIdx.x == 0) {
+            for (int32_t pq = tid; pq < g * (g - 1) / 2; pq += blockDim.x) {
+                int32_t a, b;
+                decode_pair(pq, a, b);
+                const float so = pair_s<PBC, ORTHO>(Gq.o[a][0], Gq.o[a][1], Gq.o[a][2], Gq.o[b][0], Gq.o[b][1],
+                                                    Gq.o[b][2], V.B);
+                const float sn = pair_s<PBC, ORTHO>(Gq.w[a][0], Gq.w[a][1], Gq.w[a][2], Gq.w[b][0], Gq.w[b][1],
+                                                    Gq.w[b][2], V.B);
+                delta_pair<PBC, ORTHO>(V, out, so, sn, Gq.el[a], Gq.el[b]);
             }
         }
-    }
-    __syncthreads();
-    int32_t *out = V.dn + (int64_t)p * V.dnStride;
-    if (merge_atomic) {
-        for (int32_t s = tid; s < V.dnStride; s += blockDim.x) {
-            const int32_t v = lds[s];
-            if (v) atomicAdd(&out[s], v);
-        }
-    } else {
-        for (int32_t s = tid; s < V.dnStride; s += blockDim.x) out[s] = lds[s];
     }
 }
 
@@ -228,173 +230,177 @@ __device__ float run_prog(const int32_t *prog, int32_t nprog, const float *leafv
 
 constexpr int kStack = 40;  // postfix stack depth (tree depth of n <= 2^30)
 
-// Pairwise sums of kPG arrays a[w*stride + 0..M) for the waves whose
-// `active` flag is set; all threads of the block call it; wave w's lane 0
-// receives its sum.  scratch: kPG*(nleaves + kStack) floats; lprog: nprog ints
-// (a postfix program over L leaves has 2L-1 entries).
-__device__ float block_pairwise_pg(const float *a, int32_t stride, const DevC &c, float *scratch, int32_t *lprog,
-                                   bool active) {
-    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-    for (int32_t k = threadIdx.x; k < c.nprog; k += blockDim.x) lprog[k] = c.prog[k];
-    float *leafv = scratch + w * (c.nleaves + kStack);
-    if (active)
-        for (int32_t l = lane; l < c.nleaves; l += 64)
-            leafv[l] = leaf_sum(a + w * stride + c.leaves[2 * l], c.leaves[2 * l + 1]);
+// Pairwise sum of a[0..M) by one wave: lanes sum the leaves, lane 0 runs the
+// combine program; every lane returns the sum.  scratch: nleaves + kStack
+// floats; lprog: nprog ints (a postfix program over L leaves has 2L-1 entries).
+__device__ float wave_pairwise(const float *a, const DevC &c, float *scratch, int32_t *lprog) {
+    const int lane = threadIdx.x % 64;
+    for (int32_t k = lane; k < c.nprog; k += 64) lprog[k] = c.prog[k];
+    for (int32_t l = lane; l < c.nleaves; l += 64) scratch[l] = leaf_sum(a + c.leaves[2 * l], c.leaves[2 * l + 1]);
     __syncthreads();
     float res = 0.0f;
-    if (active && lane == 0) res = run_prog(lprog, c.nprog, leafv, leafv + c.nleaves);
-    __syncthreads();
-    return res;
+    if (lane == 0) res = run_prog(lprog, c.nprog, scratch, scratch + c.nleaves);
+    return __shfl(res, 0);
 }
 
 // --------------------------------------------------------------- k_eval_a
 
-__global__ void __launch_bounds__(256) k_eval_a(View V, int32_t k) {
+// One wave per (work unit, proposal): grid (nunits, k), 64 threads.  A work
+// unit is one PDC constraint, or 64 Q points of one SFC constraint.  The wave
+// rebuilds the proposal's G(r) from base + delta in the reference's float32
+// operation order (PairDistributionConstraints.py:854-895) into LDS, then
+//   PDC: squared deviations and their NumPy pairwise sum (:810-838) -> chi2d;
+//   SFC: S(Q) of its 64 Q points, one per lane, as the sequential float32 sum
+//        over r of G[r]*M[r,q] (StructureFactorConstraints.py:772-773), M
+//        read straight from HBM/L2 (coalesced rows, a batch of rows ahead in
+//        registers) and G broadcast from LDS -> squared deviations in sqbuf.
+// The S(Q) sum is a chain of H dependent adds per Q: the kernel is bound by
+// that latency, so every (Q chunk, proposal) runs on its own wave.
+__global__ void __launch_bounds__(64) k_eval_a(View V, int32_t k) {
     extern __shared__ float sm[];
-    float *Gs = sm;                              // [kPG][Hmax]
-    float *sq = Gs + kPG * V.Hmax;               // [kPG][Hmax] PDC squared deviations
-    float *Mt = sq + kPG * V.Hmax;               // [2][kRT][kQC]
-    float *scratch = Mt + 2 * kRT * kQC;         // [kPG][maxLeaves + kStack]
-    int32_t *lprog = (int32_t *)(scratch + kPG * (V.maxLeaves + kStack));  // [2*maxLeaves]
-    const int tid = threadIdx.x, w = tid / 64, lane = tid % 64;
-    const int32_t p = blockIdx.y * kPG + w;
-    const bool valid = p < k;
-    const int32_t qc = blockIdx.x;
-    const int32_t *dnp = V.dn + (int64_t)(valid ? p : 0) * V.dnStride;
-    const int32_t T = V.T, Hmax = V.Hmax;
-    for (int32_t ci = 0; ci < V.C; ci++) {
-        // constraint fields in registers: keeps every load below unconditional
-        // and out of per-element branches
-        const DevC c = V.c[ci];
-        const int32_t H = c.H, M = c.M;
-        if (c.kind == FRMC_PDC && qc != 0) continue;  // uniform over the block
-        const int32_t *dn = dnp + c.dnOff;
-        const int32_t *base = c.base;
-        if (valid) {
-            // U bins per lane per pass, their loads issued together per pair type
-            constexpr int U = 16;
-            for (int32_t r0 = 0; r0 < H; r0 += 64 * U) {
-                float Gr[U];
+    const int32_t Hp = (V.Hmax + 3) & ~3;        // keeps float4 reads aligned
+    float *Gs = sm;                              // [Hp] G(r) or PDC squared deviations
+    float *scratch = Gs + Hp;                    // [maxLeaves + kStack]
+    int32_t *lprog = (int32_t *)(scratch + V.maxLeaves + kStack);  // [2*maxLeaves]
+    const int lane = threadIdx.x;
+    const int32_t p = blockIdx.y;
+    const int32_t u = blockIdx.x;
+    int32_t ci = 0;
+    while (ci + 1 < V.C && u >= V.c[ci + 1].uOff) ci++;
+    // constraint fields in registers: keeps every load below unconditional
+    const DevC c = V.c[ci];
+    const int32_t H = c.H, M = c.M, T = V.T;
+    const bool pdc = c.kind == FRMC_PDC;
+    const int32_t *dn = V.dn + (int64_t)p * V.dnStride + c.dnOff;
+    const int32_t *base = c.base;
+    {
+        // U bins per lane per pass; every load is unconditional (clamped index,
+        // valid pointer): a load under a divergent branch is followed by its
+        // own wait, which serialises the batch
+        constexpr int U = 16;
+        const float *expv = pdc ? c.expv : c.shellV;                 // SFC: unused (expv has M entries)
+        const float *dataW = (pdc && c.hasW) ? c.dataW : c.shellV;   // unused unless hasW
+        for (int32_t r0 = 0; r0 < H; r0 += 64 * U) {
+            int32_t rr[U];
 #pragma unroll
-                for (int i = 0; i < U; i++) Gr[i] = 0.0f;
-                for (int32_t t = 0; t < T; t++) {
-                    int32_t bv[U], dv[U];
+            for (int i = 0; i < U; i++) rr[i] = min(r0 + lane + 64 * i, H - 1);
+            float Gr[U];
+            int32_t bv[U], dv[U];
 #pragma unroll
-                    for (int i = 0; i < U; i++) {
-                        const int32_t r = min(r0 + lane + 64 * i, H - 1);  // clamped: always in bounds
-                        bv[i] = base[t * H + r];
-                        dv[i] = dn[t * H + r];
-                    }
-                    const float wt = c.w[t], Dt = c.D[t];
-#pragma unroll
-                    for (int i = 0; i < U; i++) {
-                        // Gr += wij*nij/Dij with nij integer-valued (exact in float32)
-                        const float nij = (float)(bv[i] + dv[i]);
-                        float term = wt * nij;
-                        term = term / Dt;
-                        Gr[i] = Gr[i] + term;
-                    }
-                }
+            for (int i = 0; i < U; i++) {
+                Gr[i] = 0.0f;
+                bv[i] = load_dev(base + rr[i]);
+                dv[i] = load_dev(dn + rr[i]);
+            }
+            // pair types in order; the next type's counts load while this one is summed
+            for (int32_t t = 0; t < T; t++) {
+                const int32_t tn = min(t + 1, T - 1);
+                int32_t nb[U], nd[U];
 #pragma unroll
                 for (int i = 0; i < U; i++) {
-                    const int32_t r = r0 + lane + 64 * i;
-                    if (r >= H) continue;
-                    float g = Gr[i] / c.shellV[r];           // Gr /= shellVolumes
-                    float G = c.pref[r] * (g - 1.0f);         // (4*PI*r*rho0)*(Gr-1)
-                    if (c.kind == FRMC_PDC) {
-                        if (c.hasSF) G = G * c.sf;
-                        const float diff = c.expv[r] - G;
-                        float v = diff * diff;
-                        if (c.hasW) v = c.dataW[r] * v;
-                        sq[w * Hmax + r] = v;
-                    } else {
-                        Gs[w * Hmax + r] = G;
-                    }
+                    nb[i] = load_dev(base + tn * H + rr[i]);
+                    nd[i] = load_dev(dn + tn * H + rr[i]);
+                }
+                const float wt = c.w[t], Dt = c.D[t];
+#pragma unroll
+                for (int i = 0; i < U; i++) {
+                    // Gr += wij*nij/Dij with nij integer-valued (exact in float32)
+                    const float nij = (float)(bv[i] + dv[i]);
+                    float term = wt * nij;
+                    term = term / Dt;
+                    Gr[i] = Gr[i] + term;
+                    bv[i] = nb[i];
+                    dv[i] = nd[i];
                 }
             }
-        }
-        __syncthreads();
-        if (c.kind == FRMC_PDC) {
-            const float chi = block_pairwise_pg(sq, Hmax, c, scratch, lprog, valid);
-            if (valid && lane == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
-            continue;
-        }
-        // S(Q) for q = qc*kQC + lane: sequential float32 sum over r of G[r]*M[r,q].
-        // M tiles of kRT rows x kQC columns are double-buffered in LDS; the next
-        // tile is loaded into registers while the current one is consumed.
-        const int32_t q = qc * kQC + lane;
-        const bool qok = valid && q < M;
-        const float *sqm = c.sqm;
-        constexpr int PER = (kRT * kQC) / 256;
-        float pre[PER];
-        // element e of a tile: row e / kQC, column e % kQC; out-of-range rows
-        // and columns read a clamped in-bounds address and store zero
-        const int32_t col = min(qc * kQC + tid % kQC, M - 1);
-        const bool colok = qc * kQC + tid % kQC < M;
-        float acc = 0.0f;
+            float sv[U], pf[U], ex[U], dw[U];
 #pragma unroll
-        for (int i = 0; i < PER; i++) {
-            const int32_t row = min(tid / kQC + (256 / kQC) * i, H - 1);
-            const float v = sqm[(int64_t)row * M + col];
-            pre[i] = (colok && tid / kQC + (256 / kQC) * i < H) ? v : 0.0f;
-        }
-#pragma unroll
-        for (int i = 0; i < PER; i++) Mt[tid + 256 * i] = pre[i];
-        __syncthreads();
-        int buf = 0;
-        for (int32_t r0 = 0; r0 < H; r0 += kRT) {
-            const int32_t rows = min(kRT, H - r0);
-            const bool more = r0 + kRT < H;
-            if (more) {
-#pragma unroll
-                for (int i = 0; i < PER; i++) {
-                    const int32_t rr = r0 + kRT + tid / kQC + (256 / kQC) * i;
-                    const float v = sqm[(int64_t)min(rr, H - 1) * M + col];
-                    pre[i] = (colok && rr < H) ? v : 0.0f;
-                }
+            for (int i = 0; i < U; i++) {
+                sv[i] = c.shellV[rr[i]];
+                pf[i] = c.pref[rr[i]];
+                ex[i] = expv[rr[i]];
+                dw[i] = dataW[rr[i]];
             }
-            const float *Tl = Mt + buf * (kRT * kQC);
-            if (qok) {
-                const float *g = Gs + w * Hmax + r0;
-                int32_t rr = 0;
-                if (r0 == 0) {
-                    acc = g[0] * Tl[lane];
-                    rr = 1;
-                }
-                // groups of 16 rows: all LDS reads of a group are issued before
-                // its (order-preserving) chain of adds
-                for (; rr + 16 <= rows; rr += 16) {
-                    float pr[16];
 #pragma unroll
-                    for (int i = 0; i < 16; i++) pr[i] = g[rr + i] * Tl[(rr + i) * kQC + lane];
-#pragma unroll
-                    for (int i = 0; i < 16; i++) acc = acc + pr[i];
+            for (int i = 0; i < U; i++) {
+                float g = Gr[i] / sv[i];           // Gr /= shellVolumes
+                float G = pf[i] * (g - 1.0f);      // (4*PI*r*rho0)*(Gr-1)
+                float out = G;
+                if (pdc) {
+                    if (c.hasSF) G = G * c.sf;
+                    const float diff = ex[i] - G;
+                    float v = diff * diff;
+                    if (c.hasW) v = dw[i] * v;
+                    out = v;
                 }
-                for (; rr < rows; rr++) {
-                    const float prod = g[rr] * Tl[rr * kQC + lane];
-                    acc = acc + prod;
-                }
+                if (r0 + lane + 64 * i < H) Gs[r0 + lane + 64 * i] = out;
             }
-            if (more) {
-                float *dst = Mt + (1 - buf) * (kRT * kQC);
-#pragma unroll
-                for (int i = 0; i < PER; i++) dst[tid + 256 * i] = pre[i];
-            }
-            __syncthreads();
-            buf = 1 - buf;
         }
-        if (qok) {
-            float S = acc + 1.0f;
-            if (c.hasSF) {
-                float t = S - 1.0f;
-                t = c.sf * t;
-                S = t + 1.0f;
-            }
-            const float diff = c.expv[q] - S;
-            float v = diff * diff;
-            if (c.hasW) v = c.dataW[q] * v;
-            V.sqbuf[((int64_t)p * V.C + ci) * V.Mmax + q] = v;
+    }
+    __syncthreads();
+    if (pdc) {
+        const float chi = wave_pairwise(Gs, c, scratch, lprog);
+        if (lane == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
+        return;
+    }
+    // S(Q) for q = qc*kQC + lane.  The sum starts from +0 rather than the first
+    // product: the two differ only in the sign of a zero partial sum, which no
+    // later addend and not the final +1 can see.
+    const int32_t q = (u - c.uOff) * kQC + lane;
+    const bool qok = q < M;
+    const float *mcol = c.sqm + (int64_t)min(q, M - 1) * c.hpb;  // M[r][q] at mcol[r], 16-byte aligned
+    constexpr int B = 32;                        // rows per register batch (hpb is a multiple of B)
+    float acc = 0.0f;
+    float ma[B], mb[B];
+    auto load_batch = [&](float *m, int32_t r0) {
+        // a batch past the column (a prefetch that is never used) reads the last one
+        const float *src = mcol + min(r0, c.hpb - B);
+#pragma unroll
+        for (int i4 = 0; i4 < B / 4; i4++) {
+            const float4 v = *reinterpret_cast<const float4 *>(src + 4 * i4);
+            m[4 * i4] = v.x; m[4 * i4 + 1] = v.y; m[4 * i4 + 2] = v.z; m[4 * i4 + 3] = v.w;
         }
+    };
+    auto chain = [&](const float *m, int32_t r0) {
+        const float *g = Gs + r0;  // 16-byte aligned (r0 and Hp multiples of 4)
+        if (r0 + B <= H) {
+#pragma unroll
+            for (int i4 = 0; i4 < B / 4; i4++) {
+                const float4 gv = *reinterpret_cast<const float4 *>(g + 4 * i4);
+                const float p0 = gv.x * m[4 * i4], p1 = gv.y * m[4 * i4 + 1];
+                const float p2 = gv.z * m[4 * i4 + 2], p3 = gv.w * m[4 * i4 + 3];
+                acc = acc + p0;
+                acc = acc + p1;
+                acc = acc + p2;
+                acc = acc + p3;
+            }
+        } else {
+            for (int i = 0; i < H - r0; i++) {
+                const float prod = g[i] * m[i];
+                acc = acc + prod;
+            }
+        }
+    };
+    // two register batches in turn: the rows of one load while the other's chain runs
+    load_batch(ma, 0);
+    for (int32_t r0 = 0; r0 < H; r0 += 2 * B) {
+        load_batch(mb, r0 + B);
+        chain(ma, r0);
+        if (r0 + B >= H) break;
+        load_batch(ma, r0 + 2 * B);
+        chain(mb, r0 + B);
+    }
+    if (qok) {
+        float S = acc + 1.0f;
+        if (c.hasSF) {
+            float t = S - 1.0f;
+            t = c.sf * t;
+            S = t + 1.0f;
+        }
+        const float diff = c.expv[q] - S;
+        float v = diff * diff;
+        if (c.hasW) v = c.dataW[q] * v;
+        V.sqbuf[((int64_t)p * V.C + ci) * V.Mmax + q] = v;
     }
 }
 
@@ -415,7 +421,7 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
         float chi;
         if (c.kind == FRMC_SFC) {
             const float *src = V.sqbuf + ((int64_t)p * V.C + ci) * V.Mmax;
-            for (int32_t q = threadIdx.x; q < c.M; q += blockDim.x) sm[q] = src[q];
+            for (int32_t q = threadIdx.x; q < c.M; q += blockDim.x) sm[q] = load_dev(src + q);
             for (int32_t q = threadIdx.x; q < c.nprog; q += blockDim.x) lprog[q] = c.prog[q];
             __syncthreads();
             for (int32_t l = threadIdx.x; l < c.nleaves; l += blockDim.x)
@@ -424,7 +430,7 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
             chi = threadIdx.x == 0 ? run_prog(lprog, c.nprog, scratch, scratch + c.nleaves) : 0.0f;
             __syncthreads();
         } else {
-            chi = V.chi2d[(int64_t)p * V.C + ci];
+            chi = load_dev(V.chi2d + (int64_t)p * V.C + ci);
         }
         // np.sum([SD_c / varianceSquared]) with varianceSquared == 1: fewer
         // than 8 terms are summed sequentially from 0
@@ -434,6 +440,7 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
         }
     }
     if (threadIdx.x == 0) {
+        V.tse_dev[p] = tse;
         V.out_tse[p] = tse;
         __threadfence_system();
         V.out_done[p] = V.seq;
@@ -444,36 +451,85 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
 
 template <bool PBC, bool ORTHO>
 __device__ __forceinline__ void commit_pair(const View &V, float s, int32_t ea, int32_t eb, bool intra, int32_t sign) {
+    int32_t b = -1;
     for (int32_t ci = 0; ci < V.C; ci++) {
         const DevC &c = V.c[ci];
-        if (!in_range(s, c.g)) continue;
-        const int64_t size = (int64_t)V.nel * V.nel * c.H;
-        const int64_t f = slot_index(s, c, ea, eb, V.nel);
+        if (!c.sameGrid) b = in_range(s, c.g) ? bin_of(s, c.g) : -1;
+        if (b < 0) continue;
+        const int32_t size = V.nel * V.nel * c.H;
+        const int32_t f = flat_slot(b, ea, eb, V.nel, c.H);
         if (f < 0) {
-            if (sign > 0) atomicAdd(&c.slot[2 * size], 1);
+            if (sign > 0) atomicAdd(&c.slot[2 * size], 1);  // past the array: counted, not stored
             continue;
         }
         atomicAdd(&c.slot[(intra ? 0 : size) + f], sign);
-        atomicAdd(&c.base[type_of_slot(f, c, V.nel)], sign);
     }
 }
 
+// grid ceil(n/256): picks the proposal to commit (every block makes the same
+// decision from the window's totals), then slot counts of the accepted group
+// vs every other atom (one atom per thread), pair-type sums += the proposal's
+// deltas, and the group's new coordinates (block 0; no block reads the
+// group's coordinates).  The group comes from the kernel arguments (single-atom
+// windows: old coordinates from the host mirror) or the staging buffers.
 template <bool PBC, bool ORTHO>
-__global__ void __launch_bounds__(256) k_commit(View V, int32_t p) {
+__global__ void __launch_bounds__(256) k_commit(View V, Inline I, CommitCtl C) {
     __shared__ GroupLds G;
     const int tid = threadIdx.x;
-    const int32_t g = load_group(V, p, G);
+    int32_t p = C.p;
+    if (p < 0) {
+        // lane q tests proposal q (k <= 64): reject iff newTSE > TSE and
+        // random() > tolerance (Engine.py:3308-3317); the first accepted wins
+        const int lane = tid % 64;
+        const int32_t q = min(lane, C.k - 1);
+        const float t = load_dev(V.tse_dev + q);
+        const bool acc = lane < C.k && (!(t > C.cur) || !(C.u[q] > C.tol));
+        const unsigned long long m = __ballot(acc);
+        p = m ? __ffsll((long long)m) - 1 : -1;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        // two slots by window parity: the host checks this window's decision
+        // while the next window's commit may already have run
+        volatile int32_t *slot = V.out_commit + 2 * (V.seq & 1);
+        slot[0] = p;
+        __threadfence_system();
+        slot[1] = (int32_t)V.seq;
+    }
+    if (p < 0) return;
+    if (I.k > 0) {
+        if (tid == 0) {
+            const int32_t id = I.atom[p];
+            G.g = 1;
+            G.id[0] = id;
+            G.el[0] = I.el[p];
+            G.mol[0] = V.mol[id];
+            for (int d = 0; d < 3; d++) {
+                G.o[0][d] = I.old[p][d];
+                G.w[0][d] = I.nw[p][d];
+            }
+        }
+    } else {
+        load_group(V, p, G);
+    }
     __syncthreads();
-    const int32_t j = blockIdx.x * blockDim.x + tid;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + tid, gs = (int64_t)gridDim.x * blockDim.x;
+    const int32_t *dn = C.dn + (int64_t)p * V.dnStride;
+    for (int32_t ci = 0; ci < V.C; ci++) {
+        const DevC &c = V.c[ci];
+        for (int64_t s = gt; s < (int64_t)V.T * c.H; s += gs) c.base[s] = load_dev(c.base + s) + load_dev(dn + c.dnOff + s);
+    }
+    const int32_t g = G.g;
+    const int32_t j = (int32_t)gt;
     if (j < V.n) {
         bool member = false;
         for (int32_t a = 0; a < g; a++) member |= (G.id[a] == j);
         if (!member) {
-            const float cx = V.xyz[3 * (int64_t)j], cy = V.xyz[3 * (int64_t)j + 1], cz = V.xyz[3 * (int64_t)j + 2];
+            const float cx = load_dev(V.xyz + 3 * (int64_t)j), cy = load_dev(V.xyz + 3 * (int64_t)j + 1),
+                        cz = load_dev(V.xyz + 3 * (int64_t)j + 2);
             const int32_t ej = V.elem[j], mj = V.mol[j];
             for (int32_t a = 0; a < g; a++) {
-                const float so = eng_pair_s<PBC, ORTHO>(G.o[a][0], G.o[a][1], G.o[a][2], cx, cy, cz, V.B);
-                const float sn = eng_pair_s<PBC, ORTHO>(G.w[a][0], G.w[a][1], G.w[a][2], cx, cy, cz, V.B);
+                const float so = pair_s<PBC, ORTHO>(G.o[a][0], G.o[a][1], G.o[a][2], cx, cy, cz, V.B);
+                const float sn = pair_s<PBC, ORTHO>(G.w[a][0], G.w[a][1], G.w[a][2], cx, cy, cz, V.B);
                 const bool intra = G.mol[a] == mj;
                 commit_pair<PBC, ORTHO>(V, so, G.el[a], ej, intra, -1);
                 commit_pair<PBC, ORTHO>(V, sn, G.el[a], ej, intra, +1);
@@ -484,8 +540,8 @@ __global__ void __launch_bounds__(256) k_commit(View V, int32_t p) {
         for (int32_t q = tid; q < g * (g - 1) / 2; q += blockDim.x) {
             int32_t a, b;
             decode_pair(q, a, b);
-            const float so = eng_pair_s<PBC, ORTHO>(G.o[a][0], G.o[a][1], G.o[a][2], G.o[b][0], G.o[b][1], G.o[b][2], V.B);
-            const float sn = eng_pair_s<PBC, ORTHO>(G.w[a][0], G.w[a][1], G.w[a][2], G.w[b][0], G.w[b][1], G.w[b][2], V.B);
+            const float so = pair_s<PBC, ORTHO>(G.o[a][0], G.o[a][1], G.o[a][2], G.o[b][0], G.o[b][1], G.o[b][2], V.B);
+            const float sn = pair_s<PBC, ORTHO>(G.w[a][0], G.w[a][1], G.w[a][2], G.w[b][0], G.w[b][1], G.w[b][2], V.B);
             const bool intra = G.mol[a] == G.mol[b];
             commit_pair<PBC, ORTHO>(V, so, G.el[a], G.el[b], intra, -1);
             commit_pair<PBC, ORTHO>(V, sn, G.el[a], G.el[b], intra, +1);
@@ -520,42 +576,32 @@ __global__ void k_fill(int32_t *__restrict__ p, int64_t n) {
 
 // ------------------------------------------------------------------ launch
 
-int launch_delta(const View &V, const Inline &I, int32_t k, int32_t nchunk, hipStream_t st) {
-    if (nchunk < 1) nchunk = 1;
-    int32_t chunkLen = (V.n + nchunk - 1) / nchunk;
-    chunkLen = ((chunkLen + 255) / 256) * 256;
-    nchunk = (V.n + chunkLen - 1) / chunkLen;
-    const int merge_atomic = nchunk > 1;
-    const size_t shm = sizeof(int32_t) * V.dnStride;
-    const dim3 grid((unsigned)nchunk, (unsigned)k);
-#define FRMC_DELTA(P, O)                                                                                     \
-    do {                                                                                                     \
-        if (shm > 48 * 1024)                                                                                 \
-            (void)hipFuncSetAttribute((const void *)k_delta<P, O>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                      (int)shm);                                                             \
-        k_delta<P, O><<<grid, 256, shm, st>>>(V, I, chunkLen, merge_atomic);                                  \
-    } while (0)
+int launch_delta(const View &V, const Inline &I, int32_t k, hipStream_t st) {
+    const dim3 grid(blocks_for(V.n, 256), (unsigned)((k + kDP - 1) / kDP));
     if (V.isPBC) {
-        if (V.ortho) FRMC_DELTA(true, true); else FRMC_DELTA(true, false);
+        if (V.ortho) k_delta<true, true><<<grid, 256, 0, st>>>(V, I, k);
+        else k_delta<true, false><<<grid, 256, 0, st>>>(V, I, k);
     } else {
-        FRMC_DELTA(false, false);
+        k_delta<false, false><<<grid, 256, 0, st>>>(V, I, k);
     }
-#undef FRMC_DELTA
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;
 }
 
 size_t eval_lds_bytes(const View &V) {
-    return sizeof(float) * ((size_t)2 * kPG * V.Hmax + 2 * kRT * kQC + (size_t)kPG * (V.maxLeaves + kStack) +
-                            (size_t)2 * V.maxLeaves);
+    return sizeof(float) * ((size_t)((V.Hmax + 3) & ~3) + V.maxLeaves + kStack + (size_t)2 * V.maxLeaves);
 }
 
 int launch_eval(const View &V, int32_t k, hipStream_t st) {
     const size_t shm_a = eval_lds_bytes(V);
     if (shm_a > 48 * 1024)
         (void)hipFuncSetAttribute((const void *)k_eval_a, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_a);
-    k_eval_a<<<dim3((unsigned)V.nqc, (unsigned)((k + kPG - 1) / kPG)), 256, shm_a, st>>>(V, k);
+    k_eval_a<<<dim3((unsigned)V.nunits, (unsigned)k), 64, shm_a, st>>>(V, k);
     FRMC_CHECK_LAUNCH();
+    return FRMC_OK;
+}
+
+int launch_eval_b(const View &V, int32_t k, hipStream_t st) {
     const size_t shm_b = sizeof(float) * (V.Mmax + V.maxLeaves + kStack + (size_t)2 * V.maxLeaves);
     if (shm_b > 48 * 1024)
         (void)hipFuncSetAttribute((const void *)k_eval_b, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_b);
@@ -564,13 +610,13 @@ int launch_eval(const View &V, int32_t k, hipStream_t st) {
     return FRMC_OK;
 }
 
-int launch_commit(const View &V, int32_t p, hipStream_t st) {
+int launch_commit(const View &V, const Inline &I, const CommitCtl &C, hipStream_t st) {
     const unsigned g = blocks_for(V.n, 256);
     if (V.isPBC) {
-        if (V.ortho) k_commit<true, true><<<g, 256, 0, st>>>(V, p);
-        else k_commit<true, false><<<g, 256, 0, st>>>(V, p);
+        if (V.ortho) k_commit<true, true><<<g, 256, 0, st>>>(V, I, C);
+        else k_commit<true, false><<<g, 256, 0, st>>>(V, I, C);
     } else {
-        k_commit<false, false><<<g, 256, 0, st>>>(V, p);
+        k_commit<false, false><<<g, 256, 0, st>>>(V, I, C);
     }
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;

@@ -16,6 +16,20 @@ __device__ __forceinline__ float ref_round(float x) {
     return x > 0.0f ? floorf(x + 0.5f) : ceilf(x - 0.5f);
 }
 
+// b = x - ref_round(x) in four operations: x - copysign(floor(|x| + 0.5f), x).
+// For x > 0 it is the reference's expression.  For x < 0, x - 0.5f rounds to
+// exactly -(|x| + 0.5f) (round-to-nearest-even is symmetric under negation)
+// and ceil(-y) = -floor(y), so the rounded value is the same, signed zero
+// included; for x = +-0 both forms give b = +0.
+__device__ __forceinline__ float min_image(float x) { return x - copysignf(floorf(fabsf(x) + 0.5f), x); }
+
+// |x| - floor(|x| + 0.5f): equal to +-min_image(x) (for x < 0,
+// x + f = -(|x| - f) exactly), for callers that only square it.
+__device__ __forceinline__ float min_image_pm(float x) {
+    const float a = fabsf(x);
+    return a - floorf(a + 0.5f);
+}
+
 struct Basis {
     float b[9];  // row-major 3x3, b[3*r+c] = basis[r,c]
 };
@@ -24,23 +38,24 @@ struct Basis {
 // (pairs_distances.pyx:373-382): b = d - round(d); r_c = (bx*B0c + by*B1c) + bz*B2c.
 __device__ __forceinline__ void pbc_real(float dx, float dy, float dz, const Basis &B, float &rx, float &ry,
                                          float &rz) {
-    float bx = dx - ref_round(dx);
-    float by = dy - ref_round(dy);
-    float bz = dz - ref_round(dz);
+    float bx = min_image(dx);
+    float by = min_image(dy);
+    float bz = min_image(dz);
     rx = bx * B.b[0] + by * B.b[3] + bz * B.b[6];
     ry = bx * B.b[1] + by * B.b[4] + bz * B.b[7];
     rz = bx * B.b[2] + by * B.b[5] + bz * B.b[8];
 }
 
-// Orthorhombic specialisation: with zero off-diagonal basis entries the two
-// extra terms of each component are +-0 and leave a non-zero product
-// unchanged (a zero product only changes sign, which squaring removes), so
-// the squared norm below is bit-identical to the general form.
+// Orthorhombic specialisation for squared norms only: with zero off-diagonal
+// basis entries the two extra terms of each component are +-0 and leave a
+// non-zero product unchanged (a zero product only changes sign), and the
+// components may carry the opposite sign (min_image_pm), so only the squared
+// norm of the result is bit-identical to the general form.
 __device__ __forceinline__ void pbc_real_ortho(float dx, float dy, float dz, const Basis &B, float &rx,
                                                float &ry, float &rz) {
-    float bx = dx - ref_round(dx);
-    float by = dy - ref_round(dy);
-    float bz = dz - ref_round(dz);
+    float bx = min_image_pm(dx);
+    float by = min_image_pm(dy);
+    float bz = min_image_pm(dz);
     rx = bx * B.b[0];
     ry = by * B.b[4];
     rz = bz * B.b[8];
@@ -49,6 +64,23 @@ __device__ __forceinline__ void pbc_real_ortho(float dx, float dy, float dz, con
 // ((rx*rx + ry*ry) + rz*rz), the float argument of the reference's sqrt
 __device__ __forceinline__ float norm2(float rx, float ry, float rz) {
     return rx * rx + ry * ry + rz * rz;
+}
+
+// Squared norm of the difference a - c (box coordinates for PBC, real
+// coordinates otherwise) as the reference computes it before its sqrt.
+template <bool PBC, bool ORTHO>
+__device__ __forceinline__ float pair_s(float ax, float ay, float az, float cx, float cy, float cz,
+                                        const Basis &B) {
+    float dx = ax - cx, dy = ay - cy, dz = az - cz;
+    float rx, ry, rz;
+    if (!PBC) {
+        rx = dx; ry = dy; rz = dz;
+    } else if (ORTHO) {
+        pbc_real_ortho(dx, dy, dz, B, rx, ry, rz);
+    } else {
+        pbc_real(dx, dy, dz, B, rx, ry, rz);
+    }
+    return norm2(rx, ry, rz);
 }
 
 // (float)sqrt((double)s) == correctly rounded float sqrt.  The library is

@@ -18,6 +18,9 @@
 // (int)((d-min)/bin)) of hintra if same molecule else hinter; flat C-order
 // index, so an index equal to H aliases the next slot exactly like the
 // reference; flat indices past the array are dropped and counted.
+#include <math.h>
+#include <string.h>
+
 #include "frmc_device.h"
 #include "frmc_host.h"
 
@@ -32,21 +35,6 @@ __device__ __forceinline__ void load_basis_dev(const float *__restrict__ basis, 
     for (int k = 0; k < 9; k++) B.b[k] = basis[k];
     ortho = (B.b[1] == 0.0f && B.b[2] == 0.0f && B.b[3] == 0.0f && B.b[5] == 0.0f && B.b[6] == 0.0f &&
              B.b[7] == 0.0f);
-}
-
-template <bool PBC, bool ORTHO>
-__device__ __forceinline__ float pair_s(float ax, float ay, float az, float cx, float cy, float cz,
-                                        const Basis &B) {
-    float dx = ax - cx, dy = ay - cy, dz = az - cz;
-    float rx, ry, rz;
-    if (!PBC) {
-        rx = dx; ry = dy; rz = dz;
-    } else if (ORTHO) {
-        pbc_real_ortho(dx, dy, dz, B, rx, ry, rz);
-    } else {
-        pbc_real(dx, dy, dz, B, rx, ry, rz);
-    }
-    return norm2(rx, ry, rz);
 }
 
 // --------------------------------------------------------------- zero/convert
@@ -124,101 +112,205 @@ __global__ void __launch_bounds__(256) k_multi_coords(const int32_t *__restrict_
 
 // ------------------------------------------------------------- full pairs
 
-// Work item = (row block bi, chunk of column blocks).  Row blocks are TILE
-// atoms; column blocks are TILE atoms; only column blocks bj >= bi are used.
+// Work item = (row block bi of R*kFT atoms, column tile c of kFT atoms) with
+// c >= R*bi: the upper triangle of tiles, diagonal tiles included (there only
+// j > i is counted).  Items are numbered row block by row block, so row block
+// bi starts at item bi*nct - R*bi*(bi-1)/2; a workgroup takes a contiguous
+// range of items and keeps its row atoms in registers while bi is unchanged.
+constexpr int kFT = 256;
+
 struct FullSched {
-    int32_t nb;      // number of TILE blocks
-    int32_t chunk;   // column blocks per work item
-    int64_t items;   // total work items
+    int32_t R;          // row atoms per thread
+    int32_t nrb, nct;   // row blocks, column tiles
+    int64_t items;      // work items
+    int64_t per_block;  // items per workgroup
 };
 
-template <int TILE, bool PBC, bool ORTHO, bool LDS_HIST>
-__global__ void __launch_bounds__(TILE) k_full_pairs(const float *__restrict__ coords, int32_t n,
-                                                     const float *__restrict__ basis,
-                                                     const int32_t *__restrict__ mol,
-                                                     const int32_t *__restrict__ elem, HistArgs A, FullSched S,
-                                                     const int64_t *__restrict__ item_row_start,
-                                                     int32_t *__restrict__ counts) {
+__host__ __device__ __forceinline__ int64_t row_block_start(int64_t bi, int64_t nct, int64_t R) {
+    return bi * nct - R * bi * (bi - 1) / 2;
+}
+
+// Float64 edge re-check (EDGE mode): the histogram of the binary64 recompute
+// (every pair's distance evaluated in double from the float32 coordinates,
+// range and bin in double).  The float32 distance d32 of a pair is within
+// eps = 8u*(sum_m (|dx_m|+1)*rowsum_m + d32) of the double one (u = 2^-24;
+// error analysis of the float32 difference, the exact minimum-image
+// subtraction, the 3-term basis product, the squared norm and the rounded
+// sqrt, with a factor > 2 to spare); a pair whose d32 lies farther than eps
+// from every bin edge and range end, and whose fractional components are not
+// within a rounding tie of +-1/2, takes its bin from d32; every other pair is
+// recomputed in double.
+struct Edge64 {
+    double mn, mx, bn;      // the grid in double
+    float s_lo, s_hi;       // candidate window of float32 squared norms
+    float rowsum[3];        // sum_k |B[m][k]| per basis row (PBC), 1 otherwise
+};
+
+__device__ __forceinline__ double round64(double x) { return x > 0.0 ? floor(x + 0.5) : ceil(x - 0.5); }
+
+template <bool PBC>
+__device__ __forceinline__ double pair_d64(float ax, float ay, float az, float cx, float cy, float cz,
+                                           const Basis &B) {
+    const double dx = (double)ax - (double)cx, dy = (double)ay - (double)cy, dz = (double)az - (double)cz;
+    double rx = dx, ry = dy, rz = dz;
+    if (PBC) {
+        const double bx = dx - round64(dx), by = dy - round64(dy), bz = dz - round64(dz);
+        rx = (bx * (double)B.b[0] + by * (double)B.b[3]) + bz * (double)B.b[6];
+        ry = (bx * (double)B.b[1] + by * (double)B.b[4]) + bz * (double)B.b[7];
+        rz = (bx * (double)B.b[2] + by * (double)B.b[5]) + bz * (double)B.b[8];
+    }
+    return __builtin_sqrt((rx * rx + ry * ry) + rz * rz);
+}
+
+// Bin of a candidate pair in the binary64 recompute, or -1 out of range;
+// *rechecked is set when the pair needed the double evaluation.
+template <bool PBC>
+__device__ __forceinline__ int64_t edge_bin(float s, float ax, float ay, float az, float cx, float cy, float cz,
+                                            const Basis &B, const Edge64 &E, bool &rechecked) {
+    const float dx = ax - cx, dy = ay - cy, dz = az - cz;
+    const float d32 = cr_sqrt(s);
+    const float bound = (fabsf(dx) + 1.0f) * E.rowsum[0] + (fabsf(dy) + 1.0f) * E.rowsum[1] +
+                        (fabsf(dz) + 1.0f) * E.rowsum[2] + d32;
+    const double eps = 8.0 * 5.9604644775390625e-08 * 1.001 * (double)bound;
+    bool tie = false;
+    if (PBC) {
+        // fractional differences near +-1/2: float and double may pick different images
+        tie = fabsf(0.5f - fabsf(min_image(dx))) <= 4.8e-7f * (fabsf(dx) + 1.0f) ||
+              fabsf(0.5f - fabsf(min_image(dy))) <= 4.8e-7f * (fabsf(dy) + 1.0f) ||
+              fabsf(0.5f - fabsf(min_image(dz))) <= 4.8e-7f * (fabsf(dz) + 1.0f);
+    }
+    const double t = ((double)d32 - E.mn) / E.bn;
+    const double kf = floor(t);
+    const double margin = fmin(t - kf, kf + 1.0 - t) * E.bn;
+    rechecked = tie || margin <= eps || fabs((double)d32 - E.mn) <= eps || fabs((double)d32 - E.mx) <= eps;
+    if (rechecked) {
+        const double d = pair_d64<PBC>(ax, ay, az, cx, cy, cz, B);
+        if (d < E.mn || d >= E.mx) return -1;
+        return (int64_t)((d - E.mn) / E.bn);
+    }
+    if ((double)d32 < E.mn || (double)d32 >= E.mx) return -1;
+    return (int64_t)kf;
+}
+
+template <int R, bool PBC, bool ORTHO, bool LDS_HIST, bool EDGE>
+__global__ void __launch_bounds__(kFT) k_full_pairs(const float *__restrict__ coords, int32_t n, Basis B,
+                                                    const int32_t *__restrict__ mol,
+                                                    const int32_t *__restrict__ elem, HistArgs A, FullSched S,
+                                                    Edge64 E, int32_t *__restrict__ partial,
+                                                    int32_t *__restrict__ counts) {
     extern __shared__ int32_t lds_hist[];  // 2*size ints when LDS_HIST
-    __shared__ float4 tile_xyzm[TILE];
-    __shared__ int32_t tile_e[TILE];
+    __shared__ float4 tile_xyzm[kFT];
+    __shared__ int32_t tile_e[kFT];
     __shared__ int32_t lds_overflow;
+    __shared__ unsigned long long lds_rechecked;
 
-    Basis B;
-    bool ortho = false;
-    if (PBC) load_basis_dev(basis, B, ortho);
     const int tid = threadIdx.x;
-    if (LDS_HIST) {
-        for (int64_t s = tid; s < 2 * A.size; s += TILE) lds_hist[s] = 0;
+    const int32_t size = (int32_t)A.size;  // LDS_HIST: 2*size ints fit in LDS
+    if (LDS_HIST)
+        for (int32_t s = tid; s < 2 * size; s += kFT) lds_hist[s] = 0;
+    if (tid == 0) {
+        lds_overflow = 0;
+        lds_rechecked = 0;
     }
-    if (tid == 0) lds_overflow = 0;
-    __syncthreads();
 
-    for (int64_t item = blockIdx.x; item < S.items; item += gridDim.x) {
-        // find the row block of this item: item_row_start[bi] <= item < item_row_start[bi+1]
-        int32_t lo = 0, hi = S.nb;  // invariant: start[lo] <= item < start[hi]
-        while (hi - lo > 1) {
-            int32_t mid = (lo + hi) >> 1;
-            if (item_row_start[mid] <= item) lo = mid; else hi = mid;
-        }
-        const int32_t bi = lo;
-        const int32_t bj0 = bi + (int32_t)(item - item_row_start[bi]) * S.chunk;
-        const int32_t bj1 = min(S.nb, bj0 + S.chunk);
-
-        const int32_t i = bi * TILE + tid;
-        const bool row_ok = i < n;
-        float ix = 0.f, iy = 0.f, iz = 0.f;
-        int32_t im = 0, ie = 0;
-        if (row_ok) {
-            ix = coords[3 * (int64_t)i];
-            iy = coords[3 * (int64_t)i + 1];
-            iz = coords[3 * (int64_t)i + 2];
-            im = mol[i];
-            ie = elem[i];
-        }
-        for (int32_t bj = bj0; bj < bj1; bj++) {
-            __syncthreads();  // previous tile consumed
-            {
-                const int32_t j = bj * TILE + tid;
-                if (j < n) {
-                    tile_xyzm[tid] = make_float4(coords[3 * (int64_t)j], coords[3 * (int64_t)j + 1],
-                                                 coords[3 * (int64_t)j + 2], __int_as_float(mol[j]));
-                    tile_e[tid] = elem[j];
-                }
-            }
-            __syncthreads();
-            if (!row_ok) continue;
-            const int32_t jbase = bj * TILE;
-            const int32_t jn = min(TILE, n - jbase);
-            // diagonal tile: only j > i, i.e. t > tid
-            int32_t t0 = (bj == bi) ? tid + 1 : 0;
-#pragma unroll 4
-            for (int32_t t = t0; t < jn; t++) {
-                const float4 q = tile_xyzm[t];
-                // reference: atomIndex is the earlier atom i, diff = c_i - c_j
-                float s = pair_s<PBC, ORTHO>(ix, iy, iz, q.x, q.y, q.z, B);
-                if (!in_range(s, A.g)) continue;
-                int32_t b = bin_of(s, A.g);
-                int64_t flat = ((int64_t)ie * A.nel + tile_e[t]) * A.g.H + b;
-                bool intra = __float_as_int(q.w) == im;
-                if (flat < 0 || flat >= A.size) {
-                    atomicAdd(&lds_overflow, 1);
-                } else if (LDS_HIST) {
-                    atomicAdd(&lds_hist[(intra ? 0 : A.size) + flat], 1);
-                } else {
-                    atomicAdd(&counts[(intra ? 0 : A.size) + flat], 1);
-                }
+    const int64_t it0 = (int64_t)blockIdx.x * S.per_block;
+    const int64_t it1 = min(S.items, it0 + S.per_block);
+    // row block of the first item: largest bi with start(bi) <= it0
+    int32_t lo = 0, hi = S.nrb;
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (row_block_start(mid, S.nct, R) <= it0) lo = mid; else hi = mid;
+    }
+    int32_t bi = lo, cur_bi = -1;
+    float rx[R], ry[R], rz[R];
+    int32_t rm[R], re[R], ri[R];
+    int32_t lost = 0;
+    uint32_t nre = 0;
+    for (int64_t item = it0; item < it1; item++) {
+        while (bi + 1 < S.nrb && item >= row_block_start(bi + 1, S.nct, R)) bi++;
+        const int32_t c = R * bi + (int32_t)(item - row_block_start(bi, S.nct, R));
+        if (bi != cur_bi) {
+            cur_bi = bi;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                ri[r] = (bi * R + r) * kFT + tid;
+                const int32_t i = min(ri[r], n - 1);  // clamped; rows >= n never count
+                rx[r] = coords[3 * (int64_t)i];
+                ry[r] = coords[3 * (int64_t)i + 1];
+                rz[r] = coords[3 * (int64_t)i + 2];
+                rm[r] = mol[i];
+                re[r] = elem[i];
             }
         }
+        __syncthreads();  // previous tile consumed (and LDS bins cleared)
+        {
+            const int32_t j = min(c * kFT + tid, n - 1);
+            tile_xyzm[tid] = make_float4(coords[3 * (int64_t)j], coords[3 * (int64_t)j + 1],
+                                         coords[3 * (int64_t)j + 2], __int_as_float(mol[j]));
+            tile_e[tid] = elem[j];
+        }
+        __syncthreads();
+        const int32_t jbase = c * kFT;
+        const int32_t jn = min(kFT, n - jbase);
+        const bool diag = c < R * (bi + 1);  // tile overlaps the row block
+#pragma unroll 2
+        for (int32_t t = 0; t < jn; t++) {
+            const float4 q = tile_xyzm[t];
+            const int32_t ej = tile_e[t];
+            const int32_t j = jbase + t;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                // the earlier atom i is the reference's atomIndex: diff = c_i - c_j
+                const float s = pair_s<PBC, ORTHO>(rx[r], ry[r], rz[r], q.x, q.y, q.z, B);
+                const bool cand = EDGE ? (s >= E.s_lo && s < E.s_hi) : in_range(s, A.g);
+                const bool ok = cand && ri[r] < n && (!diag || j > ri[r]);
+                if (ok) {
+                    int64_t b;
+                    if (EDGE) {
+                        bool re64 = false;
+                        b = edge_bin<PBC>(s, rx[r], ry[r], rz[r], q.x, q.y, q.z, B, E, re64);
+                        nre += re64;
+                        if (b < 0) continue;
+                    } else {
+                        b = bin_of(s, A.g);
+                    }
+                    const int64_t flat = ((int64_t)re[r] * A.nel + ej) * A.g.H + b;
+                    const bool intra = __float_as_int(q.w) == rm[r];
+                    if (flat >= A.size) {
+                        lost++;
+                    } else if (LDS_HIST) {
+                        atomicAdd(&lds_hist[(intra ? 0 : size) + (int32_t)flat], 1);
+                    } else {
+                        atomicAdd(&counts[(intra ? 0 : A.size) + flat], 1);
+                    }
+                }
+            }
+        }
     }
+    if (lost) atomicAdd(&lds_overflow, lost);
+    if (EDGE && nre) atomicAdd(&lds_rechecked, (unsigned long long)nre);
     __syncthreads();
     if (LDS_HIST) {
-        for (int64_t s = tid; s < 2 * A.size; s += TILE) {
-            int32_t v = lds_hist[s];
-            if (v) atomicAdd(&counts[s], v);
-        }
+        int32_t *dst = partial + (int64_t)blockIdx.x * 2 * size;
+        for (int32_t s = tid; s < 2 * size; s += kFT) dst[s] = lds_hist[s];
     }
     if (tid == 0 && lds_overflow) atomicAdd(&counts[2 * A.size], lds_overflow);
+    // EDGE: counts[2*size+1 .. +2] hold the number of double re-checks (64-bit, low word first)
+    if (EDGE && tid == 0 && lds_rechecked)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&counts[2 * A.size + 2]), lds_rechecked);
+}
+
+// counts[s] += sum over the workgroup partials p in this split of partial[p][s]
+__global__ void __launch_bounds__(256) k_sum_partials(const int32_t *__restrict__ partial, int32_t nparts,
+                                                      int64_t len, int32_t per_split,
+                                                      int32_t *__restrict__ counts) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= len) return;
+    const int32_t p0 = blockIdx.y * per_split;
+    const int32_t p1 = min(nparts, p0 + per_split);
+    int32_t acc = 0;
+    for (int32_t p = p0; p < p1; p++) acc += partial[(int64_t)p * len + s];
+    if (acc) atomicAdd(&counts[s], acc);
 }
 
 // --------------------------------------------------------------- from dists
@@ -317,67 +409,29 @@ static int launch_multi(const int32_t *indexes, int32_t nidx, const float *coord
 
 // The orthorhombic fast path needs the basis on the host; callers that only
 // hold a device basis read it back once (36 bytes).
-static int read_ortho(const float *basis, int isPBC, hipStream_t st, bool &ortho) {
+static int read_basis(const float *basis, int isPBC, hipStream_t st, Basis &B, bool &ortho) {
     ortho = false;
+    for (int q = 0; q < 9; q++) B.b[q] = 0.f;
     if (!isPBC) return FRMC_OK;
-    float b[9];
-    FRMC_CHECK_HIP(hipMemcpyAsync(b, basis, sizeof(b), hipMemcpyDeviceToHost, st));
+    FRMC_CHECK_HIP(hipMemcpyAsync(B.b, basis, sizeof(B.b), hipMemcpyDeviceToHost, st));
     FRMC_CHECK_HIP(hipStreamSynchronize(st));
+    const float *b = B.b;
     ortho = b[1] == 0.f && b[2] == 0.f && b[3] == 0.f && b[5] == 0.f && b[6] == 0.f && b[7] == 0.f;
     return FRMC_OK;
 }
 
-template <int TILE>
-static int launch_full_t(const float *coords, int32_t n, const float *basis, int isPBC, bool ortho,
-                         const int32_t *mol, const int32_t *elem, const HistArgs &A, int32_t *counts,
-                         hipStream_t st) {
-    FullSched S;
-    S.nb = (n + TILE - 1) / TILE;
-    S.chunk = 8;
-    // row-start prefix of work items (host-computed, uploaded)
-    int64_t *h_start = new int64_t[S.nb + 1];
-    int64_t acc = 0;
-    for (int32_t bi = 0; bi < S.nb; bi++) {
-        h_start[bi] = acc;
-        acc += (S.nb - bi + S.chunk - 1) / S.chunk;
-    }
-    h_start[S.nb] = acc;
-    S.items = acc;
-    int64_t *d_start = nullptr;
-    hipError_t e = hipMallocAsync((void **)&d_start, sizeof(int64_t) * (S.nb + 1), st);
-    if (e != hipSuccess) {
-        delete[] h_start;
-        set_error("hipMallocAsync failed: %s", hipGetErrorString(e));
-        return FRMC_ENOMEM;
-    }
-    e = hipMemcpyAsync(d_start, h_start, sizeof(int64_t) * (S.nb + 1), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);  // h_start is pageable and freed below
-    delete[] h_start;
-    if (e != hipSuccess) {
-        (void)hipFreeAsync(d_start, st);
-        set_error("upload of the tile schedule failed: %s", hipGetErrorString(e));
-        return FRMC_EHIP;
-    }
-    const size_t lds_bytes = sizeof(int32_t) * 2 * A.size;
-    const bool lds = lds_bytes <= kMaxLdsHistBytes;
-    int dev = 0, ncu = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    // 160 KB of LDS per CU: as many resident blocks as the private bins allow
-    int per_cu = lds ? (int)((150 * 1024) / (lds_bytes + 6 * 1024)) : 4;
-    if (per_cu < 1) per_cu = 1;
-    if (per_cu > 4) per_cu = 4;
-    int64_t blocks = (int64_t)ncu * per_cu;
-    if (blocks > S.items) blocks = S.items;
-    if (blocks < 1) blocks = 1;
-    size_t shm = lds ? lds_bytes : 0;
+template <int R, bool EDGE>
+static int launch_full_r(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
+                         const int32_t *mol, const int32_t *elem, const HistArgs &A, FullSched S, const Edge64 &E,
+                         int64_t blocks, bool lds, int32_t *partial, int32_t *counts, hipStream_t st) {
+    const size_t shm = lds ? sizeof(int32_t) * 2 * A.size : 0;
 #define FRMC_FULL_LAUNCH(P, O, L)                                                                               \
     do {                                                                                                        \
         if (shm > 48 * 1024)                                                                                    \
-            (void)hipFuncSetAttribute((const void *)k_full_pairs<TILE, P, O, L>,                                \
+            (void)hipFuncSetAttribute((const void *)k_full_pairs<R, P, O, L, EDGE>,                             \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                    \
-        k_full_pairs<TILE, P, O, L><<<(unsigned)blocks, TILE, shm, st>>>(coords, n, basis, mol, elem, A, S,   \
-                                                                         d_start, counts);                      \
+        k_full_pairs<R, P, O, L, EDGE><<<(unsigned)blocks, kFT, shm, st>>>(coords, n, B, mol, elem, A, S, E,   \
+                                                                           partial, counts);                    \
     } while (0)
     if (isPBC) {
         if (ortho) { if (lds) FRMC_FULL_LAUNCH(true, true, true); else FRMC_FULL_LAUNCH(true, true, false); }
@@ -386,17 +440,91 @@ static int launch_full_t(const float *coords, int32_t n, const float *basis, int
         if (lds) FRMC_FULL_LAUNCH(false, false, true); else FRMC_FULL_LAUNCH(false, false, false);
     }
 #undef FRMC_FULL_LAUNCH
-    hipError_t le = hipGetLastError();
-    (void)hipFreeAsync(d_start, st);
-    if (le != hipSuccess) {
-        set_error("k_full_pairs launch failed: %s", hipGetErrorString(le));
-        return FRMC_EHIP;
-    }
+    FRMC_CHECK_LAUNCH();
     return FRMC_OK;
 }
 
-int launch_full(const float *coords, int32_t n, const float *basis, int isPBC, bool ortho, const int32_t *mol,
+static FullSched full_sched(int32_t n, int32_t R, int64_t blocks) {
+    FullSched S;
+    S.R = R;
+    S.nct = (n + kFT - 1) / kFT;
+    S.nrb = (n + R * kFT - 1) / (R * kFT);
+    S.items = row_block_start(S.nrb, S.nct, R);
+    if (blocks > S.items) blocks = S.items;
+    S.per_block = (S.items + blocks - 1) / blocks;
+    return S;
+}
+
+// All unordered pairs into counts ([2][nel][nel][H] + overflow, already
+// zeroed or holding counts to add to).  Workgroups count into LDS-private
+// bins and write them to per-workgroup partials, which k_sum_partials adds
+// into counts (no contention on the global bins); histograms too large for
+// LDS count straight into global memory with atomics.
+static int launch_full_t(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
+                         const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E,
+                         int32_t *counts, hipStream_t st) {
+    const size_t lds_bytes = sizeof(int32_t) * 2 * A.size;
+    const bool lds = lds_bytes <= kMaxLdsHistBytes;
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    // 160 KB of LDS per CU: as many resident workgroups as the private bins allow
+    int per_cu = lds ? (int)((156 * 1024) / (lds_bytes + 6 * 1024)) : 8;
+    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+    const int64_t max_blocks = (int64_t)ncu * per_cu;
+    // two row atoms per thread when that still leaves two items per workgroup
+    FullSched S = full_sched(n, 2, max_blocks);
+    if (S.items < 2 * max_blocks) S = full_sched(n, 1, max_blocks);
+    const int64_t blocks = (S.items + S.per_block - 1) / S.per_block;
+    int32_t *partial = nullptr;
+    if (lds) {
+        hipError_t e = hipMallocAsync((void **)&partial, lds_bytes * blocks, st);
+        if (e != hipSuccess) {
+            set_error("hipMallocAsync of %lld bytes of partial histograms failed: %s",
+                      (long long)(lds_bytes * blocks), hipGetErrorString(e));
+            return FRMC_ENOMEM;
+        }
+    }
+    Edge64 none;
+    memset(&none, 0, sizeof(none));
+    int rc;
+    if (E)
+        rc = S.R == 2 ? launch_full_r<2, true>(coords, n, B, isPBC, ortho, mol, elem, A, S, *E, blocks, lds, partial,
+                                               counts, st)
+                      : launch_full_r<1, true>(coords, n, B, isPBC, ortho, mol, elem, A, S, *E, blocks, lds, partial,
+                                               counts, st);
+    else
+        rc = S.R == 2 ? launch_full_r<2, false>(coords, n, B, isPBC, ortho, mol, elem, A, S, none, blocks, lds,
+                                                partial, counts, st)
+                      : launch_full_r<1, false>(coords, n, B, isPBC, ortho, mol, elem, A, S, none, blocks, lds,
+                                                partial, counts, st);
+    if (rc == FRMC_OK && lds) {
+        const int32_t per_split = 16;
+        const int64_t len = 2 * A.size;
+        dim3 grid(blocks_for(len, 256), (unsigned)((blocks + per_split - 1) / per_split));
+        k_sum_partials<<<grid, 256, 0, st>>>(partial, (int32_t)blocks, len, per_split, counts);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) {
+            set_error("k_sum_partials launch failed: %s", hipGetErrorString(le));
+            rc = FRMC_EHIP;
+        }
+    }
+    if (partial) (void)hipFreeAsync(partial, st);
+    return rc;
+}
+
+static int launch_full_timed(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
+                             const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E,
+                             int32_t *counts, hipStream_t st);
+
+int launch_full(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho, const int32_t *mol,
                 const int32_t *elem, const HistArgs &A, int32_t *counts, hipStream_t st) {
+    return launch_full_timed(coords, n, B, isPBC, ortho, mol, elem, A, nullptr, counts, st);
+}
+
+static int launch_full_timed(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
+                             const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E,
+                             int32_t *counts, hipStream_t st) {
     if (n < 2) return FRMC_OK;
     hipEvent_t a = nullptr, b = nullptr;
     const bool timing = timing_enabled();
@@ -405,7 +533,7 @@ int launch_full(const float *coords, int32_t n, const float *basis, int isPBC, b
         FRMC_CHECK_HIP(hipEventCreate(&b));
         FRMC_CHECK_HIP(hipEventRecord(a, st));
     }
-    int rc = launch_full_t<256>(coords, n, basis, isPBC, ortho, mol, elem, A, counts, st);
+    int rc = launch_full_t(coords, n, B, isPBC, ortho, mol, elem, A, E, counts, st);
     if (timing) {
         float ms = 0.f;
         hipError_t e = hipEventRecord(b, st);
@@ -420,6 +548,17 @@ int launch_full(const float *coords, int32_t n, const float *basis, int isPBC, b
         set_last_kernel_ms(ms);
     }
     return rc;
+}
+
+// largest |x| of n floats, as the bit pattern of a non-negative float (NaN
+// counts as infinity)
+__global__ void __launch_bounds__(256) k_absmax(const float *__restrict__ x, int64_t n, uint32_t *out) {
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float a = fabsf(x[i]);
+        m = (a != a) ? __int_as_float(0x7f800000) : fmaxf(m, a);
+    }
+    atomicMax(out, __float_as_uint(m));
 }
 
 }  // namespace frmc
@@ -460,7 +599,8 @@ int frmc_multiple_pairs_histograms_coords(const int32_t *indexes, int32_t nidx, 
     FRMC_REQUIRE(!isPBC || basis, "basis is NULL");
     hipStream_t st = as_stream(stream);
     bool ortho = false;
-    if ((rc = read_ortho(basis, isPBC, st, ortho))) return rc;
+    Basis hB;
+    if ((rc = read_basis(basis, isPBC, st, hB, ortho))) return rc;
     Scratch sc;
     if ((rc = alloc_counts(sc, A.size, st))) return rc;
     if ((rc = launch_multi(indexes, nidx, coords, n, basis, isPBC, mol, elem, A, allAtoms, sc.p, st, ortho)))
@@ -502,10 +642,11 @@ int frmc_full_pairs_histograms_coords(const float *coords, int32_t n, const floa
     FRMC_REQUIRE(!isPBC || basis, "basis is NULL");
     hipStream_t st = as_stream(stream);
     bool ortho = false;
-    if ((rc = read_ortho(basis, isPBC, st, ortho))) return rc;
+    Basis hB;
+    if ((rc = read_basis(basis, isPBC, st, hB, ortho))) return rc;
     Scratch sc;
     if ((rc = alloc_counts(sc, A.size, st))) return rc;
-    if ((rc = launch_full(coords, n, basis, isPBC, ortho, mol, elem, A, sc.p, st))) return rc;
+    if ((rc = launch_full(coords, n, hB, isPBC, ortho, mol, elem, A, sc.p, st))) return rc;
     return finish_counts(sc, A.size, hintra, hinter, /*add=*/0, st);
 }
 
@@ -565,10 +706,71 @@ int frmc_full_pairs_counts(const float *coords, int32_t n, const float *basis, i
     FRMC_REQUIRE(!isPBC || basis, "basis is NULL");
     hipStream_t st = as_stream(stream);
     bool ortho = false;
-    if ((rc = read_ortho(basis, isPBC, st, ortho))) return rc;
+    Basis hB;
+    if ((rc = read_basis(basis, isPBC, st, hB, ortho))) return rc;
     k_fill_i32<<<1024, 256, 0, st>>>(counts, 2 * A.size + 1, 0);
     FRMC_CHECK_LAUNCH();
-    return launch_full(coords, n, basis, isPBC, ortho, mol, elem, A, counts, st);
+    return launch_full(coords, n, hB, isPBC, ortho, mol, elem, A, counts, st);
+}
+
+int frmc_full_pairs_counts_f64(const float *coords, int32_t n, const float *basis, int isPBC, const int32_t *mol,
+                               const int32_t *elem, int32_t nel, float minD, float maxD, float bin, int32_t H,
+                               int32_t *counts, int64_t *rechecked, void *stream) {
+    HistArgs A;
+    int rc = make_args(nel, H, minD, maxD, bin, A);
+    if (rc) return rc;
+    FRMC_REQUIRE(n >= 0 && counts, "bad arguments");
+    FRMC_REQUIRE(n == 0 || (coords && mol && elem), "NULL pointer argument");
+    FRMC_REQUIRE(!isPBC || basis, "basis is NULL");
+    hipStream_t st = as_stream(stream);
+    bool ortho = false;
+    Basis hB;
+    if ((rc = read_basis(basis, isPBC, st, hB, ortho))) return rc;
+    // largest coordinate magnitude bounds the float32 difference error
+    float amax = 0.0f;
+    if (n > 0) {
+        uint32_t *d_m = nullptr;
+        hipError_t e = hipMallocAsync((void **)&d_m, sizeof(uint32_t), st);
+        if (e != hipSuccess) {
+            set_error("hipMallocAsync failed: %s", hipGetErrorString(e));
+            return FRMC_ENOMEM;
+        }
+        k_fill_i32<<<1, 64, 0, st>>>((int32_t *)d_m, 1, 0);
+        k_absmax<<<256, 256, 0, st>>>(coords, 3 * (int64_t)n, d_m);
+        uint32_t bits = 0;
+        e = hipMemcpyAsync(&bits, d_m, sizeof(bits), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        (void)hipFreeAsync(d_m, st);
+        if (e != hipSuccess) {
+            set_error("coordinate range reduction failed: %s", hipGetErrorString(e));
+            return FRMC_EHIP;
+        }
+        memcpy(&amax, &bits, sizeof(amax));
+        FRMC_REQUIRE(amax < 1e30f, "coordinates are not finite");
+    }
+    Edge64 E;
+    E.mn = minD;
+    E.mx = maxD;
+    E.bn = bin;
+    double rows = 0.0;
+    for (int m = 0; m < 3; m++) {
+        E.rowsum[m] = isPBC ? fabsf(hB.b[3 * m]) + fabsf(hB.b[3 * m + 1]) + fabsf(hB.b[3 * m + 2]) : 1.0f;
+        rows += E.rowsum[m];
+    }
+    // widest per-pair error bound (twice the kernel's, for the window only)
+    const double big = 2.0 * 8.0 * 5.9604644775390625e-08 * 1.001 * (rows * (2.0 * amax + 1.0) + maxD);
+    const double lo = (double)minD - big, hi = (double)maxD + big;
+    E.s_lo = lo > 0.0 ? nextafterf((float)(lo * lo), 0.0f) : 0.0f;
+    E.s_hi = nextafterf((float)(hi * hi), INFINITY);
+    k_fill_i32<<<1024, 256, 0, st>>>(counts, 2 * A.size + 4, 0);
+    FRMC_CHECK_LAUNCH();
+    rc = launch_full_timed(coords, n, hB, isPBC, ortho, mol, elem, A, &E, counts, st);
+    if (rc) return rc;
+    if (rechecked) {
+        FRMC_CHECK_HIP(hipMemcpyAsync(rechecked, counts + 2 * A.size + 2, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        FRMC_CHECK_HIP(hipStreamSynchronize(st));
+    }
+    return FRMC_OK;
 }
 
 }  // extern "C"

@@ -157,6 +157,24 @@ int frmc_full_pairs_counts(const float *boxCoords, int32_t n, const float *basis
                            float minDistance, float maxDistance, float bin, int32_t histSize, int32_t *counts,
                            void *stream);
 
+/* Full histogram of the binary64 recompute (no reference counterpart: the
+   second oracle of the full-histogram path, "bit-exact against a float64
+   recompute").  Every unordered pair i<j once, slot [e(i)][e(j)], with the
+   distance evaluated in double from the float32 box coordinates and basis
+   (the reference's minimum-image rule and operand order, no FMA, correctly
+   rounded sqrt), range minDistance <= d < maxDistance and bin
+   (int)((d-minDistance)/bin) in double.  Distances run in float32; a pair
+   whose float32 distance is within a rigorous error bound of a bin edge or
+   range end (or whose fractional difference is within rounding of +-1/2) is
+   recomputed in double.  counts: int32 [2][nel][nel][H], then the overflow
+   counter, one pad word and a 64-bit count of re-checked pairs (2*size+4
+   ints).  *rechecked (host, may be NULL; forces a synchronisation) receives
+   that count. */
+int frmc_full_pairs_counts_f64(const float *boxCoords, int32_t n, const float *basis, int isPBC,
+                               const int32_t *moleculeIndex, const int32_t *elementIndex, int32_t numberOfElements,
+                               float minDistance, float maxDistance, float bin, int32_t histSize, int32_t *counts,
+                               int64_t *rechecked, void *stream);
+
 /* ------------------------------------------------------------------------
  * boundary_conditions_collection (Extensions/boundary_conditions_collection.pyx)
  * ---------------------------------------------------------------------- */
@@ -264,14 +282,16 @@ int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float
                                 float tolerance, frmc_mt19937 *py_state, frmc_mt19937 *np_state,
                                 frmc_run_stats *stats, int32_t *group_log, uint8_t *accept_log, float *tse_log);
 /* Per-kernel timing with HIP events recorded on the engine stream around
-   every launch (adds two event records per launch while enabled).
-   frmc_engine_profile(e,1) resets and enables, (e,0) disables;
-   frmc_engine_kernel_times fills FRMC_NKERNELS totals (ms) and counts. */
+   the launches of every `every`-th window (two event records per timed
+   launch).  frmc_engine_profile(e, every >= 1) resets and enables, (e, 0)
+   disables; frmc_engine_kernel_times fills FRMC_NKERNELS totals (ms) and
+   the number of timed launches. */
 #define FRMC_K_DELTA 0   /* k_delta: moved group vs all atoms, k proposals */
-#define FRMC_K_EVAL 1    /* k_eval: G(r), S(Q), chi-squares and total per proposal */
+#define FRMC_K_EVAL 1    /* k_eval_a: G(r), PDC chi-square, S(Q) per proposal */
 #define FRMC_K_COMMIT 2  /* k_commit: histograms and coordinates of the accepted move */
-#define FRMC_NKERNELS 3
-int frmc_engine_profile(frmc_engine *e, int enable);
+#define FRMC_K_EVAL_B 3  /* k_eval_b: SFC chi-squares, totals, completion flags */
+#define FRMC_NKERNELS 4
+int frmc_engine_profile(frmc_engine *e, int every);
 int frmc_engine_kernel_times(frmc_engine *e, double *ms_out, int64_t *launches_out);
 /* window size of frmc_engine_run_translation: adaptive (default; k is the
    maximum, the size follows the acceptance rate) or always k */

@@ -293,6 +293,70 @@ int64_t ref_multiple_pairs_histograms_dists(const int32_t *indexes, int32_t nidx
     return overflow;
 }
 
+/* Float64 recompute of the full histogram: every unordered pair i<j once,
+   slot [e(i)][e(j)], with the distance evaluated in binary64 from the float32
+   box coordinates and basis (differences, minimum-image rounding
+   x>0 ? floor(x+0.5) : ceil(x-0.5), basis product ((bx*B0k + by*B1k) + bz*B2k),
+   ((rx*rx + ry*ry) + rz*rz), correctly rounded sqrt, no FMA), range
+   minD <= d < maxD and bin (int)((d - minD)/bin) in binary64, the flat
+   (nel,nel,H) index aliasing like the reference's.  counts: int32
+   [2][nel][nel][H] (zeroed here); returns the number of past-the-end bins.
+   This is the second oracle: a GPU edge re-check mode must equal it exactly. */
+static inline double ref_round64(double x) { return x > 0.0 ? floor(x + 0.5) : ceil(x - 0.5); }
+
+int64_t ref_full_pairs_counts_f64(const float *coords, int32_t n, const float *B, int isPBC, const int32_t *mol,
+                                  const int32_t *elem, int32_t nel, float minD, float maxD, float bin, int32_t H,
+                                  int32_t *counts, int nthreads) {
+    const int64_t size = (int64_t)nel * nel * H;
+    int64_t overflow = 0;
+    double Bd[9];
+    for (int k = 0; k < 9; k++) Bd[k] = (double)B[k];
+    const double mn = minD, mx = maxD, bn = bin;
+    memset(counts, 0, sizeof(int32_t) * 2 * size);
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads) reduction(+ : overflow)
+    {
+        int32_t *pc = counts;
+#ifdef _OPENMP
+        if (omp_get_num_threads() > 1) pc = (int32_t *)calloc(2 * size, sizeof(int32_t));
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int32_t i = 0; i < n; i++) {
+            for (int32_t j = i + 1; j < n; j++) {
+                double dx = (double)coords[3 * (int64_t)i] - (double)coords[3 * (int64_t)j];
+                double dy = (double)coords[3 * (int64_t)i + 1] - (double)coords[3 * (int64_t)j + 1];
+                double dz = (double)coords[3 * (int64_t)i + 2] - (double)coords[3 * (int64_t)j + 2];
+                double rx, ry, rz;
+                if (isPBC) {
+                    const double bx = dx - ref_round64(dx), by = dy - ref_round64(dy), bz = dz - ref_round64(dz);
+                    rx = (bx * Bd[0] + by * Bd[3]) + bz * Bd[6];
+                    ry = (bx * Bd[1] + by * Bd[4]) + bz * Bd[7];
+                    rz = (bx * Bd[2] + by * Bd[5]) + bz * Bd[8];
+                } else {
+                    rx = dx; ry = dy; rz = dz;
+                }
+                const double d = sqrt((rx * rx + ry * ry) + rz * rz);
+                if (d < mn || d >= mx) continue;
+                const int64_t b = (int64_t)((d - mn) / bn);
+                const int64_t flat = ((int64_t)elem[i] * nel + elem[j]) * H + b;
+                if (flat >= size) {
+                    overflow++;
+                    continue;
+                }
+                pc[(mol[i] == mol[j] ? 0 : size) + flat]++;
+            }
+        }
+#ifdef _OPENMP
+        if (pc != counts) {
+#pragma omp critical
+            for (int64_t s = 0; s < 2 * size; s++) counts[s] += pc[s];
+            free(pc);
+        }
+#endif
+    }
+    return overflow;
+}
+
 /* ---------------------------------------------------- boundary conditions */
 
 /* get_reciprocal_basis (boundary_conditions_collection.pyx:45-80).  The

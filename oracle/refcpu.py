@@ -41,6 +41,7 @@ def lib():
                                                      _p, _p, _int], ctypes.c_int64),
             "ref_multiple_pairs_histograms_dists": ([_p, _i, _p, _i, _i, _p, _p, _i, _f, _f, _f, _i, _int, _p, _p],
                                                     ctypes.c_int64),
+            "ref_full_pairs_counts_f64": ([_p, _i, _p, _int, _p, _p, _i, _f, _f, _f, _i, _p, _int], ctypes.c_int64),
             "ref_get_reciprocal_basis": ([_p, _p], ctypes.c_double),
             "ref_transform_coordinates": ([_p, _p, _i, _p], None),
             "ref_box_coordinates_real_distances": ([_i, _p, _i, _p, _p, _p], None),
@@ -190,6 +191,21 @@ def full_pairs_histograms_coords(boxCoords, basis, isPBC, moleculeIndex, element
                                             moleculeIndex, elementIndex, numberOfElements, minDistance,
                                             maxDistance, bin, histSize, allAtoms=False, ncores=ncores,
                                             return_overflow=return_overflow)
+
+
+def full_pairs_counts_f64(boxCoords, basis, isPBC, moleculeIndex, elementIndex, numberOfElements, minDistance,
+                          maxDistance, bin, histSize, ncores=1):
+    """Float64 recompute of the full histogram (the second oracle):
+    returns (hintra, hinter) as int64 counts and the past-the-end count."""
+    c, b = _c(boxCoords, np.float32), _c(basis, np.float32)
+    m, e = _c(moleculeIndex, np.int32), _c(elementIndex, np.int32)
+    nel, H = int(numberOfElements), int(histSize)
+    counts = np.zeros(2 * nel * nel * H, np.int32)
+    ov = lib().ref_full_pairs_counts_f64(_ptr(c), c.shape[0], _ptr(b), int(bool(isPBC)), _ptr(m), _ptr(e), nel,
+                                         np.float32(minDistance), np.float32(maxDistance), np.float32(bin), H,
+                                         _ptr(counts), int(ncores))
+    counts = counts.astype(np.int64).reshape(2, nel, nel, H)
+    return counts[0], counts[1], ov
 
 
 def multiple_pairs_histograms_dists(indexes, distances, moleculeIndex, elementIndex, numberOfElements,

@@ -203,3 +203,56 @@ def test_config1_full_histogram_13500_atoms():
     assert same(g[0], c[0]) and same(g[1], c[1])
     n = len(box)
     assert g[1].sum() > 0.1 * n * (n - 1) / 2
+
+
+def test_full_histogram_two_rows_per_thread_and_partials():
+    """40,000 atoms (two row atoms per thread, thousands of tiles, LDS
+    partials summed per workgroup) on a small grid, with molecules of four
+    atoms: bit-exact against the oracle."""
+    s = systems.random_box(40000, 80.0, 1.2, seed=12, species=("Si", "O"), fractions=(1 / 3, 2 / 3))
+    lut = {e: i for i, e in enumerate(sorted(set(s["elements"])))}
+    elem = np.array([lut[e] for e in s["elements"]], dtype=np.int32)
+    mol = (np.arange(40000) // 4).astype(np.int32)
+    box = systems.box_coordinates(s["recip"], s["real"])
+    args = (box, s["basis"], True, mol, elem, 2, F32(0.0), F32(6.0), F32(0.05), 120)
+    g = gph.full_pairs_histograms_coords(*args)
+    c = refcpu.full_pairs_histograms_coords(*args, ncores=8)
+    assert same(g[0], c[0]) and same(g[1], c[1])
+    assert g[0].sum() > 0 and g[1].sum() > 0
+
+
+@pytest.mark.parametrize("case", ["cubic", "triclinic", "small_cell"])
+def test_full_histogram_float64_recompute(case, small_random, small_triclinic):
+    """The float64 mode equals the binary64 recompute oracle exactly, with a
+    non-zero number of double re-checks (pairs near edges exist).  small_cell:
+    a 10 A cube binned to 9 A, where pairs sit at fractional differences near
+    +-1/2 (minimum image not unique)."""
+    if case == "small_cell":
+        s = systems.random_box(150, 10.0, 1.0, seed=4)
+        nel, grid = 1, (0.0, 9.0, 0.005, 1800)
+        mol, el = s["molecules"], np.zeros(150, np.int32)
+        box = systems.box_coordinates(s["recip"], s["real"])
+    else:
+        s = small_random if case == "cubic" else small_triclinic
+        nel, grid = int(s["elem"].max()) + 1, (0.5, 8.0, 0.004, 1875)
+        mol, el, box = s["molecules"], s["elem"], s["box"]
+    args = (box, s["basis"], True, mol, el, nel, F32(grid[0]), F32(grid[1]), F32(grid[2]), grid[3])
+    gi, ge, rechecked = gph.full_pairs_histograms_coords_float64(*args)
+    ci, ce, ov = refcpu.full_pairs_counts_f64(*args, ncores=8)
+    assert same(gi, ci.astype(F32)) and same(ge, ce.astype(F32))
+    assert rechecked > 0
+    # the float32 (reference) histogram is a different oracle: same total, a few pairs moved
+    fi, fe = gph.full_pairs_histograms_coords(*args)
+    assert fi.sum() + fe.sum() == gi.sum() + ge.sum()
+
+
+def test_full_histogram_float64_recompute_40000_atoms():
+    s = systems.random_box(40000, 80.0, 1.2, seed=13, species=("Si", "O"), fractions=(1 / 3, 2 / 3))
+    lut = {e: i for i, e in enumerate(sorted(set(s["elements"])))}
+    elem = np.array([lut[e] for e in s["elements"]], dtype=np.int32)
+    box = systems.box_coordinates(s["recip"], s["real"])
+    args = (box, s["basis"], True, s["molecules"], elem, 2, F32(0.0), F32(6.0), F32(0.01), 600)
+    gi, ge, rechecked = gph.full_pairs_histograms_coords_float64(*args)
+    ci, ce, ov = refcpu.full_pairs_counts_f64(*args, ncores=8)
+    assert same(gi, ci.astype(F32)) and same(ge, ce.astype(F32))
+    assert 0 < rechecked < 0.05 * (gi.sum() + ge.sum())

@@ -174,6 +174,35 @@ def test_round_quirk_known_answer():
     assert d != F32(x * L)
 
 
+def _min_image_inputs():
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(-3, 3, 200000), rng.uniform(-1e4, 1e4, 20000)]).astype(F32)
+    halves = (np.arange(-40, 41) * 0.5).astype(F32)
+    near = np.concatenate([halves, np.nextafter(halves, F32(np.inf)), np.nextafter(halves, F32(-np.inf))])
+    special = np.array([0.0, -0.0, 0.49999997, -0.49999997, 1.0, -1.0, 1e-45, -1e-45, 8388607.5, -8388607.5],
+                       dtype=F32)
+    return np.concatenate([x, near, special]).astype(F32)
+
+
+def test_min_image_four_op_form_is_the_reference_round():
+    """The kernels compute b = x - round(x) as x - copysign(floor(|x|+0.5f), x)
+    and, where only b*b is used, as |x| - floor(|x|+0.5f).  Both equal the
+    reference's x - (x>0 ? floor(x+0.5f) : ceil(x-0.5f)) (pairs_distances.pyx:31-32,
+    :373-375): the first bit for bit, the second up to sign, on random values and
+    on every half-integer and its float neighbours."""
+    x = _min_image_inputs()
+    half = F32(0.5)
+    with np.errstate(all="ignore"):
+        ref = (x - np.where(x > 0, np.floor(x + half), np.ceil(x - half))).astype(F32)
+        four = (x - np.copysign(np.floor(np.abs(x) + half), x)).astype(F32)
+        pm = (np.abs(x) - np.floor(np.abs(x) + half)).astype(F32)
+    assert ref.tobytes() == four.tobytes()
+    assert np.abs(ref).tobytes() == np.abs(pm).tobytes()
+    assert (ref * ref).tobytes() == (pm * pm).tobytes()
+    # a plausible slip, rounding half to even, is told apart
+    assert (x - np.rint(x)).astype(F32).tobytes() != ref.tobytes()
+
+
 def test_no_fma_known_answer():
     """A squared norm whose FMA-contracted value differs from the separately
     rounded one: the oracle must return the non-contracted result."""
@@ -376,3 +405,41 @@ def test_chi_square_is_pairwise_and_close_to_fsum():
     c = chain.chi_square(e, m)
     assert c.tobytes() == _pairwise(((e - m) ** 2).astype(F32)).tobytes()
     assert c == pytest.approx(math.fsum(((e.astype(np.float64) - m) ** 2)), rel=1e-5)
+
+
+def _f64_brute_force(box, basis, mol, elem, nel, mn, mx, bn, H):
+    """Vectorised binary64 enumeration of all pairs i<j (numpy elementwise
+    float64 operations: separately rounded, no fused multiply-add)."""
+    c = box.astype(np.float64)
+    B = basis.astype(np.float64)
+    i, j = np.triu_indices(len(c), 1)
+    d = c[i] - c[j]
+    b = d - np.where(d > 0, np.floor(d + 0.5), np.ceil(d - 0.5))
+    r = (b[:, 0:1] * B[0] + b[:, 1:2] * B[1]) + b[:, 2:3] * B[2]
+    dist = np.sqrt((r[:, 0] * r[:, 0] + r[:, 1] * r[:, 1]) + r[:, 2] * r[:, 2])
+    mn, mx, bn = float(F32(mn)), float(F32(mx)), float(F32(bn))
+    ok = (dist >= mn) & (dist < mx)
+    k = ((dist[ok] - mn) / bn).astype(np.int64)
+    flat = (elem[i[ok]].astype(np.int64) * nel + elem[j[ok]]) * H + k
+    intra = mol[i[ok]] == mol[j[ok]]
+    size = nel * nel * H
+    keep = flat < size
+    out = np.zeros(2 * size, np.int64)
+    np.add.at(out, flat[keep] + np.where(intra[keep], 0, size), 1)
+    return out[:size].reshape(nel, nel, H), out[size:].reshape(nel, nel, H), int((~keep).sum())
+
+
+def test_float64_recompute_oracle_vs_brute_force(small_triclinic):
+    """The float64 full-histogram oracle equals an independent vectorised
+    binary64 enumeration, and differs from the float32 (reference) histogram
+    only by pairs moved across a bin edge (same total)."""
+    s = small_triclinic
+    args = (s["box"], s["basis"], True, s["molecules"], s["elem"], 3, 0.0, 9.0, 0.01, 900)
+    hi, he, ov = refcpu.full_pairs_counts_f64(*args, ncores=4)
+    bi, be, bov = _f64_brute_force(s["box"], s["basis"], s["molecules"], s["elem"], 3, 0.0, 9.0, 0.01, 900)
+    assert np.array_equal(hi, bi) and np.array_equal(he, be) and ov == bov
+    fi, fe = refcpu.full_pairs_histograms_coords(s["box"], s["basis"], True, s["molecules"], s["elem"], 3,
+                                                 F32(0.0), F32(9.0), F32(0.01), 900)
+    assert hi.sum() + he.sum() == fi.sum() + fe.sum()
+    moved = np.abs(hi - fi).sum() + np.abs(he - fe).sum()
+    assert moved <= 0.002 * (hi.sum() + he.sum())
