@@ -224,9 +224,17 @@ def kernel_roofline(name, avg_s, n, props_per_launch):
     if not avg_s > 0:
         return None
     tf = flops / avg_s / 1e12
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic_engine_b2_13500.json")) as f:
+            traffic = json.load(f)["kernels"][name]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        pass
     return {"bound": "mfma", "compute": "fp32 VALU (no matrix instructions; MI355X fp32 vector and dense "
             "matrix peaks are both 157.3 TFLOP/s)", "achieved": round(tf, 4), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32_TFLOPS, 6), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32_TFLOPS, 6), "traffic": traffic,
+            "traffic_source": "profiles/hbm_traffic_engine_b2_13500.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                              "copy-calibrated)" if traffic is not None else None,
             "kernel": name, "avg_launch_us": round(avg_s * 1e6, 2), "flops_per_launch": flops}
 
 
@@ -296,6 +304,8 @@ def main():
     ap.add_argument("--sio2", action=argparse.BooleanOptionalAction, default=True,
                     help="also run the 200,000-atom SiO2 box (full histograms and k=128 moves)")
     ap.add_argument("--sio2-moves", type=int, default=3000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the pair path runs on a single MI355X; "
@@ -307,6 +317,14 @@ def main():
     from fullrmc_amd._device import require_gpu
     require_gpu()
     L = _lib.lib()
+
+    if args.copy_calibration:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.int32, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     start, elem, pdc, sfc = build_workload()
     n = len(elem)

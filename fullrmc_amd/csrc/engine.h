@@ -28,7 +28,8 @@ namespace frmc {
 
 constexpr int kMaxC = 4;   // constraints per engine
 constexpr int kMaxG = 64;  // atoms per group
-constexpr int kQC = 64;    // Q points per k_eval_a work unit (one per lane of its wave)
+constexpr int kQC = 64;    // Q points per k_eval_a wave (one per lane)
+constexpr int kEW = 1;     // waves per k_eval_a block: a work unit is kEW*kQC Q points
 
 struct DevC {
     int32_t kind;   // FRMC_PDC or FRMC_SFC
@@ -43,7 +44,7 @@ struct DevC {
     const int32_t *prog;    // postfix combine program over the leaves
     int32_t nleaves, nprog;
     int32_t sameGrid;       // 1: same histogram grid as the previous constraint (bins reused)
-    int32_t uOff;           // first k_eval_a work unit (PDC: 1 unit, SFC: ceil(M/kQC) units)
+    int32_t uOff;           // first k_eval_a work unit (PDC: 1 unit, SFC: ceil(M/(kEW*kQC)) units)
     int32_t *base;          // [T][H]
     int32_t *slot;          // [2][nel][nel][H] + overflow counter
     int32_t dnOff;          // offset in a proposal's delta block
@@ -75,11 +76,9 @@ struct View {
     int32_t *cnt;                  // [kmax] blocks finished per proposal (reset by the last)
     float *sqbuf;                  // [kmax][C][Mmax]
     float *chi2d;                  // [kmax][C] device partials
-    float *out_tse;                // [kmax] host-mapped
-    float *out_chi2;               // [kmax][C] host-mapped
-    volatile uint32_t *out_done;   // [kmax] host-mapped completion sequence numbers
+    unsigned long long *out_res;   // [kmax][C+1] host-mapped: (seq << 32 | float bits) of total, chi-squares
     float *tse_dev;                // [kmax] totals in device memory (read by k_commit)
-    volatile int32_t *out_commit;  // [2][2] host-mapped: proposal committed by k_commit, its window's seq
+    unsigned long long *out_commit;  // [2] host-mapped: (seq << 32 | proposal committed by k_commit)
     uint32_t seq;
 };
 
@@ -143,10 +142,10 @@ struct frmc_engine {
     float *sqbuf = nullptr, *chi2d = nullptr;
     frmc::Stage stage[2];
     int cur_stage = 0;     // staging buffer of the last evaluated window
-    float *h_tse = nullptr, *h_chi2 = nullptr;
-    uint32_t *h_done = nullptr;
+    unsigned long long *h_res = nullptr;  // host-mapped results [kmax][C+1]
+    std::vector<float> h_tse, h_chi2;     // results of the last window, unpacked
     float *tse_dev = nullptr;
-    int32_t *h_commit = nullptr;     // [2][2] written by k_commit (slot = window parity)
+    unsigned long long *h_commit = nullptr;  // [2] written by k_commit (slot = window parity)
     bool device_decided = false;     // the last window's commit was queued on the device
     int32_t pending_check = -2;      // decision of the last device-decided window (-2: none)
     uint32_t pending_seq = 0;

@@ -49,9 +49,7 @@ View make_view(frmc_engine *e, int s) {
     V.cnt = nullptr;
     V.sqbuf = e->sqbuf;
     V.chi2d = e->chi2d;
-    V.out_tse = e->h_tse;
-    V.out_chi2 = e->h_chi2;
-    V.out_done = e->h_done;
+    V.out_res = e->h_res;
     V.tse_dev = e->tse_dev;
     V.out_commit = e->h_commit;
     V.seq = e->seq;
@@ -150,21 +148,23 @@ static double now_s() {
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
-// Wait for the completion flags of proposals [0,k) of sequence e->seq.  A
-// spin on pinned memory returns within a microsecond of k_eval_b's stores;
-// after 60 s it synchronises the stream to surface a device error.
+// Wait for the results of proposals [0,k) of sequence e->seq: every word of
+// the host-mapped result block carries the window's number.  A spin on pinned
+// memory returns within a microsecond of k_eval_b's stores; after 60 s it
+// synchronises the stream to surface a device error.
 static int wait_done(frmc_engine *e, int32_t k) {
-    volatile uint32_t *done = e->h_done;
+    volatile unsigned long long *res = e->h_res;
     const uint32_t s = e->seq;
+    const int32_t W = e->C + 1;
     double t0 = 0.0;
     uint64_t spins = 0;
-    for (int32_t p = 0; p < k; p++) {
-        while (done[p] != s) {
+    for (int32_t q = 0; q < k * W; q++) {
+        while ((uint32_t)(res[q] >> 32) != s) {
             if ((++spins & 0xffff) == 0) {
                 if (t0 == 0.0) t0 = now_s();
                 else if (now_s() - t0 > 60.0) {
                     FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
-                    if (done[p] != s) {
+                    if ((uint32_t)(res[q] >> 32) != s) {
                         set_error("window evaluation did not complete");
                         return FRMC_EHIP;
                     }
@@ -173,6 +173,15 @@ static int wait_done(frmc_engine *e, int32_t k) {
         }
     }
     __sync_synchronize();
+    for (int32_t p = 0; p < k; p++) {
+        const unsigned long long *w = (const unsigned long long *)res + (size_t)p * W;
+        uint32_t bits = (uint32_t)w[0];
+        memcpy(&e->h_tse[p], &bits, 4);
+        for (int ci = 0; ci < e->C; ci++) {
+            bits = (uint32_t)w[1 + ci];
+            memcpy(&e->h_chi2[(size_t)p * e->C + ci], &bits, 4);
+        }
+    }
     return FRMC_OK;
 }
 
@@ -180,10 +189,12 @@ static int wait_done(frmc_engine *e, int32_t k) {
 // has completed once a later window's flags are set (same stream).
 static int check_device_decision(frmc_engine *e) {
     if (e->pending_check == -2) return FRMC_OK;
-    volatile int32_t *hc = e->h_commit + 2 * (e->pending_seq & 1);
-    if ((uint32_t)hc[1] != e->pending_seq || hc[0] != e->pending_check) {
-        set_error("device acceptance decision (proposal %d, window %u) differs from the host's (%d, %u)", hc[0],
-                  (unsigned)hc[1], e->pending_check, (unsigned)e->pending_seq);
+    const unsigned long long w = ((volatile unsigned long long *)e->h_commit)[e->pending_seq & 1];
+    const int32_t dp = (int32_t)(uint32_t)w;
+    const uint32_t dseq = (uint32_t)(w >> 32);
+    if (dseq != e->pending_seq || dp != e->pending_check) {
+        set_error("device acceptance decision (proposal %d, window %u) differs from the host's (%d, %u)", dp,
+                  (unsigned)dseq, e->pending_check, (unsigned)e->pending_seq);
         e->pending_check = -2;
         return FRMC_ESTATE;
     }
@@ -310,7 +321,7 @@ int eval_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, con
                         const float *u, float tol, float *tse_out) {
     int rc = stage_and_eval(e, k, group_ids, new_box, u, tol);
     if (rc) return rc;
-    memcpy(tse_out, e->h_tse, sizeof(float) * k);
+    memcpy(tse_out, e->h_tse.data(), sizeof(float) * k);
     return FRMC_OK;
 }
 
@@ -483,7 +494,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         int32_t units = 0;
         for (int ci = 0; ci < e->C; ci++) {
             e->c[ci].uOff = units;
-            units += e->c[ci].kind == FRMC_PDC ? 1 : (e->c[ci].M + kQC - 1) / kQC;
+            units += e->c[ci].kind == FRMC_PDC ? 1 : (e->c[ci].M + kEW * kQC - 1) / (kEW * kQC);
         }
         e->nunits = units;
         {
@@ -501,12 +512,13 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         }
         ECHK(dalloc(e, &e->sqbuf, K * e->C * e->Mmax));
         ECHK(dalloc(e, &e->chi2d, K * e->C));
-        ECHK(halloc(e, &e->h_tse, K));
-        ECHK(halloc(e, &e->h_chi2, K * e->C));
-        ECHK(halloc(e, &e->h_done, K));
+        ECHK(halloc(e, &e->h_res, K * (e->C + 1)));
+        for (size_t q = 0; q < K * (size_t)(e->C + 1); q++) e->h_res[q] = 0;
+        e->h_tse.assign(K, 0.0f);
+        e->h_chi2.assign(K * e->C, 0.0f);
         ECHK(dalloc(e, &e->tse_dev, K));
-        ECHK(halloc(e, &e->h_commit, 4));
-        for (int q = 0; q < 4; q++) e->h_commit[q] = q % 2 ? 0 : -1;
+        ECHK(halloc(e, &e->h_commit, 2));
+        e->h_commit[0] = e->h_commit[1] = 0xffffffffull;
         ECHK(hipDeviceSynchronize());
     }
 #undef ECHK
@@ -564,8 +576,8 @@ int frmc_engine_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const 
     FRMC_REQUIRE(group_ids && new_box && tse_out, "NULL argument");
     int rc = stage_and_eval(e, k, group_ids, new_box);
     if (rc) return rc;
-    memcpy(tse_out, e->h_tse, sizeof(float) * k);
-    if (chi2_out) memcpy(chi2_out, e->h_chi2, sizeof(float) * k * e->C);
+    memcpy(tse_out, e->h_tse.data(), sizeof(float) * k);
+    if (chi2_out) memcpy(chi2_out, e->h_chi2.data(), sizeof(float) * k * e->C);
     return FRMC_OK;
 }
 

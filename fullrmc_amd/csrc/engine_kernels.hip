@@ -29,10 +29,11 @@
 
 namespace frmc {
 
-// Loads of data that an earlier kernel of the window (or the host) wrote go
-// through coherent vector loads: a uniform address would otherwise be read
-// through the scalar data cache, which is not guaranteed to be refreshed
-// between back-to-back kernels of a stream.
+// Uniform (wave-invariant) loads of data that an earlier kernel of the window
+// or the host wrote go through coherent vector loads: a plain load from a
+// uniform address is compiled to a scalar load through the scalar data cache,
+// which has returned stale values here between back-to-back kernels of a
+// stream.  Lane-varying loads use the vector path and need no such care.
 template <typename T>
 __device__ __forceinline__ T load_dev(const T *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -140,8 +141,7 @@ __global__ void __launch_bounds__(256) k_delta(View V, Inline I, int32_t k) {
     const int32_t j = blockIdx.x * blockDim.x + tid;
     const bool jok = j < V.n;
     const int32_t jj = min(j, V.n - 1);  // clamped: loads stay in bounds
-    const float cx = load_dev(V.xyz + 3 * (int64_t)jj), cy = load_dev(V.xyz + 3 * (int64_t)jj + 1),
-                cz = load_dev(V.xyz + 3 * (int64_t)jj + 2);
+    const float cx = V.xyz[3 * (int64_t)jj], cy = V.xyz[3 * (int64_t)jj + 1], cz = V.xyz[3 * (int64_t)jj + 2];
     const int32_t ej = V.elem[jj];
     for (int32_t q = 0; q < np; q++) {
         const int32_t p = p0 + q;
@@ -230,39 +230,42 @@ __device__ float run_prog(const int32_t *prog, int32_t nprog, const float *leafv
 
 constexpr int kStack = 40;  // postfix stack depth (tree depth of n <= 2^30)
 
-// Pairwise sum of a[0..M) by one wave: lanes sum the leaves, lane 0 runs the
-// combine program; every lane returns the sum.  scratch: nleaves + kStack
-// floats; lprog: nprog ints (a postfix program over L leaves has 2L-1 entries).
-__device__ float wave_pairwise(const float *a, const DevC &c, float *scratch, int32_t *lprog) {
-    const int lane = threadIdx.x % 64;
-    for (int32_t k = lane; k < c.nprog; k += 64) lprog[k] = c.prog[k];
-    for (int32_t l = lane; l < c.nleaves; l += 64) scratch[l] = leaf_sum(a + c.leaves[2 * l], c.leaves[2 * l + 1]);
+// Pairwise sum of a[0..M) by a block: threads sum the leaves, thread 0 runs
+// the combine program and returns the sum (all threads must call it).
+// scratch: nleaves + kStack floats; lprog: nprog ints (a postfix program over
+// L leaves has 2L-1 entries).
+__device__ float block_pairwise(const float *a, const DevC &c, float *scratch, int32_t *lprog) {
+    for (int32_t k = threadIdx.x; k < c.nprog; k += blockDim.x) lprog[k] = c.prog[k];
+    for (int32_t l = threadIdx.x; l < c.nleaves; l += blockDim.x)
+        scratch[l] = leaf_sum(a + c.leaves[2 * l], c.leaves[2 * l + 1]);
     __syncthreads();
     float res = 0.0f;
-    if (lane == 0) res = run_prog(lprog, c.nprog, scratch, scratch + c.nleaves);
-    return __shfl(res, 0);
+    if (threadIdx.x == 0) res = run_prog(lprog, c.nprog, scratch, scratch + c.nleaves);
+    return res;
 }
 
 // --------------------------------------------------------------- k_eval_a
 
-// One wave per (work unit, proposal): grid (nunits, k), 64 threads.  A work
-// unit is one PDC constraint, or 64 Q points of one SFC constraint.  The wave
-// rebuilds the proposal's G(r) from base + delta in the reference's float32
-// operation order (PairDistributionConstraints.py:854-895) into LDS, then
+// One block of kEW waves per (work unit, proposal): grid (nunits, k).  A work
+// unit is one PDC constraint, or kEW*64 Q points of one SFC constraint.  The
+// block rebuilds the proposal's G(r) from base + delta in the reference's
+// float32 operation order (PairDistributionConstraints.py:854-895) into LDS,
+// its waves sharing the bins, then
 //   PDC: squared deviations and their NumPy pairwise sum (:810-838) -> chi2d;
-//   SFC: S(Q) of its 64 Q points, one per lane, as the sequential float32 sum
-//        over r of G[r]*M[r,q] (StructureFactorConstraints.py:772-773), M
-//        read straight from HBM/L2 (coalesced rows, a batch of rows ahead in
-//        registers) and G broadcast from LDS -> squared deviations in sqbuf.
+//   SFC: S(Q) of its Q points, one per lane, as the sequential float32 sum
+//        over r of G[r]*M[r,q] (StructureFactorConstraints.py:772-773), the
+//        matrix column streamed from HBM/L2 (transposed, a batch of rows
+//        ahead in registers) and G broadcast from LDS -> squared deviations.
 // The S(Q) sum is a chain of H dependent adds per Q: the kernel is bound by
-// that latency, so every (Q chunk, proposal) runs on its own wave.
-__global__ void __launch_bounds__(64) k_eval_a(View V, int32_t k) {
+// that latency, so each wave carries one chain per lane and the grid spreads
+// the (Q chunk, proposal) pairs over the CUs.
+__global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
     extern __shared__ float sm[];
     const int32_t Hp = (V.Hmax + 3) & ~3;        // keeps float4 reads aligned
     float *Gs = sm;                              // [Hp] G(r) or PDC squared deviations
     float *scratch = Gs + Hp;                    // [maxLeaves + kStack]
     int32_t *lprog = (int32_t *)(scratch + V.maxLeaves + kStack);  // [2*maxLeaves]
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
     const int32_t p = blockIdx.y;
     const int32_t u = blockIdx.x;
     int32_t ci = 0;
@@ -273,24 +276,56 @@ __global__ void __launch_bounds__(64) k_eval_a(View V, int32_t k) {
     const bool pdc = c.kind == FRMC_PDC;
     const int32_t *dn = V.dn + (int64_t)p * V.dnStride + c.dnOff;
     const int32_t *base = c.base;
+    // S(Q) for q = qc*kQC + lane (SFC units): the first rows of the matrix
+    // column and the experimental value load now, under the G(r) phase
+    const int32_t q = ((u - c.uOff) * kEW + w) * kQC + lane;
+    const bool qok = !pdc && q < M;
+    const float *mcol = c.sqm + (int64_t)min(q, M - 1) * c.hpb;  // M[r][q] at mcol[r], 16-byte aligned
+    constexpr int B = 32;                        // rows per register batch (hpb is a multiple of B)
+    float ma[B], mb[B], ga[B], gb[B];
+    auto load_m = [&](float *m, int32_t r0) {
+        // a batch past the end (a prefetch that is never used) rereads the last one
+        const float *src = mcol + min(r0, c.hpb - B);
+#pragma unroll
+        for (int i4 = 0; i4 < B / 4; i4++) {
+            const float4 v = *reinterpret_cast<const float4 *>(src + 4 * i4);
+            m[4 * i4] = v.x; m[4 * i4 + 1] = v.y; m[4 * i4 + 2] = v.z; m[4 * i4 + 3] = v.w;
+        }
+    };
+    float ex_q = 0.0f, dw_q = 1.0f;
+    if (!pdc) {
+        load_m(ma, 0);
+        ex_q = c.expv[min(q, M - 1)];
+        if (c.hasW) dw_q = c.dataW[min(q, M - 1)];
+    }
     {
         // U bins per lane per pass; every load is unconditional (clamped index,
         // valid pointer): a load under a divergent branch is followed by its
         // own wait, which serialises the batch
-        constexpr int U = 16;
+        constexpr int NT = 64 * kEW;
+        constexpr int U = (1024 + NT - 1) / NT;  // bins per thread per pass: 1024 per pass
         const float *expv = pdc ? c.expv : c.shellV;                 // SFC: unused (expv has M entries)
         const float *dataW = (pdc && c.hasW) ? c.dataW : c.shellV;   // unused unless hasW
-        for (int32_t r0 = 0; r0 < H; r0 += 64 * U) {
+        for (int32_t r0 = 0; r0 < H; r0 += NT * U) {
             int32_t rr[U];
 #pragma unroll
-            for (int i = 0; i < U; i++) rr[i] = min(r0 + lane + 64 * i, H - 1);
+            for (int i = 0; i < U; i++) rr[i] = min(r0 + tid + NT * i, H - 1);
             float Gr[U];
             int32_t bv[U], dv[U];
+            float sv[U], pf[U], ex[U], dw[U];
 #pragma unroll
             for (int i = 0; i < U; i++) {
                 Gr[i] = 0.0f;
-                bv[i] = load_dev(base + rr[i]);
-                dv[i] = load_dev(dn + rr[i]);
+                bv[i] = base[rr[i]];
+                dv[i] = dn[rr[i]];
+            }
+            // the per-bin constants load with the first counts
+#pragma unroll
+            for (int i = 0; i < U; i++) {
+                sv[i] = c.shellV[rr[i]];
+                pf[i] = c.pref[rr[i]];
+                ex[i] = expv[rr[i]];
+                dw[i] = dataW[rr[i]];
             }
             // pair types in order; the next type's counts load while this one is summed
             for (int32_t t = 0; t < T; t++) {
@@ -298,8 +333,8 @@ __global__ void __launch_bounds__(64) k_eval_a(View V, int32_t k) {
                 int32_t nb[U], nd[U];
 #pragma unroll
                 for (int i = 0; i < U; i++) {
-                    nb[i] = load_dev(base + tn * H + rr[i]);
-                    nd[i] = load_dev(dn + tn * H + rr[i]);
+                    nb[i] = base[tn * H + rr[i]];
+                    nd[i] = dn[tn * H + rr[i]];
                 }
                 const float wt = c.w[t], Dt = c.D[t];
 #pragma unroll
@@ -313,14 +348,6 @@ __global__ void __launch_bounds__(64) k_eval_a(View V, int32_t k) {
                     dv[i] = nd[i];
                 }
             }
-            float sv[U], pf[U], ex[U], dw[U];
-#pragma unroll
-            for (int i = 0; i < U; i++) {
-                sv[i] = c.shellV[rr[i]];
-                pf[i] = c.pref[rr[i]];
-                ex[i] = expv[rr[i]];
-                dw[i] = dataW[rr[i]];
-            }
 #pragma unroll
             for (int i = 0; i < U; i++) {
                 float g = Gr[i] / sv[i];           // Gr /= shellVolumes
@@ -333,62 +360,57 @@ __global__ void __launch_bounds__(64) k_eval_a(View V, int32_t k) {
                     if (c.hasW) v = dw[i] * v;
                     out = v;
                 }
-                if (r0 + lane + 64 * i < H) Gs[r0 + lane + 64 * i] = out;
+                if (r0 + tid + NT * i < H) Gs[r0 + tid + NT * i] = out;
             }
         }
     }
     __syncthreads();
     if (pdc) {
-        const float chi = wave_pairwise(Gs, c, scratch, lprog);
-        if (lane == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
+        const float chi = block_pairwise(Gs, c, scratch, lprog);
+        if (tid == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
         return;
     }
-    // S(Q) for q = qc*kQC + lane.  The sum starts from +0 rather than the first
-    // product: the two differ only in the sign of a zero partial sum, which no
-    // later addend and not the final +1 can see.
-    const int32_t q = (u - c.uOff) * kQC + lane;
-    const bool qok = q < M;
-    const float *mcol = c.sqm + (int64_t)min(q, M - 1) * c.hpb;  // M[r][q] at mcol[r], 16-byte aligned
-    constexpr int B = 32;                        // rows per register batch (hpb is a multiple of B)
+    // The S(Q) sum starts from +0 rather than the first product: the two
+    // differ only in the sign of a zero partial sum, which no later addend and
+    // not the final +1 can see.
     float acc = 0.0f;
-    float ma[B], mb[B];
-    auto load_batch = [&](float *m, int32_t r0) {
-        // a batch past the column (a prefetch that is never used) reads the last one
-        const float *src = mcol + min(r0, c.hpb - B);
+    // two register batches (matrix column and G rows) in turn: one batch loads
+    // while the other's chain runs, which hides both the HBM/L2 and the LDS latency
+    auto load_g = [&](float *gg, int32_t r0) {
+        const float *gsrc = Gs + min(r0, Hp - B > 0 ? Hp - B : 0);
 #pragma unroll
         for (int i4 = 0; i4 < B / 4; i4++) {
-            const float4 v = *reinterpret_cast<const float4 *>(src + 4 * i4);
-            m[4 * i4] = v.x; m[4 * i4 + 1] = v.y; m[4 * i4 + 2] = v.z; m[4 * i4 + 3] = v.w;
+            const float4 gv = *reinterpret_cast<const float4 *>(gsrc + 4 * i4);
+            gg[4 * i4] = gv.x; gg[4 * i4 + 1] = gv.y; gg[4 * i4 + 2] = gv.z; gg[4 * i4 + 3] = gv.w;
         }
     };
-    auto chain = [&](const float *m, int32_t r0) {
-        const float *g = Gs + r0;  // 16-byte aligned (r0 and Hp multiples of 4)
+    auto chain = [&](const float *m, const float *gg, int32_t r0) {
         if (r0 + B <= H) {
+            // all products first: the adds then run back to back, each waiting
+            // only on the previous add
+            float pr[B];
 #pragma unroll
-            for (int i4 = 0; i4 < B / 4; i4++) {
-                const float4 gv = *reinterpret_cast<const float4 *>(g + 4 * i4);
-                const float p0 = gv.x * m[4 * i4], p1 = gv.y * m[4 * i4 + 1];
-                const float p2 = gv.z * m[4 * i4 + 2], p3 = gv.w * m[4 * i4 + 3];
-                acc = acc + p0;
-                acc = acc + p1;
-                acc = acc + p2;
-                acc = acc + p3;
-            }
+            for (int i = 0; i < B; i++) pr[i] = gg[i] * m[i];
+            __builtin_amdgcn_sched_barrier(0);  // keep the scheduler from interleaving the two loops
+#pragma unroll
+            for (int i = 0; i < B; i++) acc = acc + pr[i];
         } else {
+            const float *g = Gs + r0;
             for (int i = 0; i < H - r0; i++) {
                 const float prod = g[i] * m[i];
                 acc = acc + prod;
             }
         }
     };
-    // two register batches in turn: the rows of one load while the other's chain runs
-    load_batch(ma, 0);
+    load_g(ga, 0);
     for (int32_t r0 = 0; r0 < H; r0 += 2 * B) {
-        load_batch(mb, r0 + B);
-        chain(ma, r0);
+        load_m(mb, r0 + B);
+        load_g(gb, r0 + B);
+        chain(ma, ga, r0);
         if (r0 + B >= H) break;
-        load_batch(ma, r0 + 2 * B);
-        chain(mb, r0 + B);
+        load_m(ma, r0 + 2 * B);
+        load_g(ga, r0 + 2 * B);
+        chain(mb, gb, r0 + B);
     }
     if (qok) {
         float S = acc + 1.0f;
@@ -397,14 +419,22 @@ __global__ void __launch_bounds__(64) k_eval_a(View V, int32_t k) {
             t = c.sf * t;
             S = t + 1.0f;
         }
-        const float diff = c.expv[q] - S;
+        const float diff = ex_q - S;
         float v = diff * diff;
-        if (c.hasW) v = c.dataW[q] * v;
+        if (c.hasW) v = dw_q * v;
         V.sqbuf[((int64_t)p * V.C + ci) * V.Mmax + q] = v;
     }
 }
 
 // --------------------------------------------------------------- k_eval_b
+
+// A result for the host: the value and the window's sequence number in one
+// 64-bit store to host memory, so the host needs no ordering between stores
+// (it waits until every word carries the current window's number).
+__device__ __forceinline__ void publish(unsigned long long *w, float v, uint32_t seq) {
+    const unsigned long long word = ((unsigned long long)seq << 32) | (unsigned long long)__float_as_uint(v);
+    __hip_atomic_store(w, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
     extern __shared__ float sm[];  // Mmax + maxLeaves + kStack, then 2*maxLeaves ints
@@ -421,7 +451,7 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
         float chi;
         if (c.kind == FRMC_SFC) {
             const float *src = V.sqbuf + ((int64_t)p * V.C + ci) * V.Mmax;
-            for (int32_t q = threadIdx.x; q < c.M; q += blockDim.x) sm[q] = load_dev(src + q);
+            for (int32_t q = threadIdx.x; q < c.M; q += blockDim.x) sm[q] = src[q];
             for (int32_t q = threadIdx.x; q < c.nprog; q += blockDim.x) lprog[q] = c.prog[q];
             __syncthreads();
             for (int32_t l = threadIdx.x; l < c.nleaves; l += blockDim.x)
@@ -436,14 +466,12 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
         // than 8 terms are summed sequentially from 0
         if (threadIdx.x == 0) {
             tse = tse + chi;
-            V.out_chi2[(int64_t)p * V.C + ci] = chi;
+            publish(V.out_res + (int64_t)p * (V.C + 1) + 1 + ci, chi, V.seq);
         }
     }
     if (threadIdx.x == 0) {
         V.tse_dev[p] = tse;
-        V.out_tse[p] = tse;
-        __threadfence_system();
-        V.out_done[p] = V.seq;
+        publish(V.out_res + (int64_t)p * (V.C + 1), tse, V.seq);
     }
 }
 
@@ -488,12 +516,10 @@ __global__ void __launch_bounds__(256) k_commit(View V, Inline I, CommitCtl C) {
         p = m ? __ffsll((long long)m) - 1 : -1;
     }
     if (blockIdx.x == 0 && tid == 0) {
-        // two slots by window parity: the host checks this window's decision
-        // while the next window's commit may already have run
-        volatile int32_t *slot = V.out_commit + 2 * (V.seq & 1);
-        slot[0] = p;
-        __threadfence_system();
-        slot[1] = (int32_t)V.seq;
+        // (seq << 32 | p) in one of two slots by window parity: the host checks
+        // this window's decision while the next window's commit may have run
+        const unsigned long long word = ((unsigned long long)V.seq << 32) | (uint32_t)p;
+        __hip_atomic_store(V.out_commit + (V.seq & 1), word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     if (p < 0) return;
     if (I.k > 0) {
@@ -516,7 +542,7 @@ __global__ void __launch_bounds__(256) k_commit(View V, Inline I, CommitCtl C) {
     const int32_t *dn = C.dn + (int64_t)p * V.dnStride;
     for (int32_t ci = 0; ci < V.C; ci++) {
         const DevC &c = V.c[ci];
-        for (int64_t s = gt; s < (int64_t)V.T * c.H; s += gs) c.base[s] = load_dev(c.base + s) + load_dev(dn + c.dnOff + s);
+        for (int64_t s = gt; s < (int64_t)V.T * c.H; s += gs) c.base[s] += dn[c.dnOff + s];
     }
     const int32_t g = G.g;
     const int32_t j = (int32_t)gt;
@@ -524,8 +550,7 @@ __global__ void __launch_bounds__(256) k_commit(View V, Inline I, CommitCtl C) {
         bool member = false;
         for (int32_t a = 0; a < g; a++) member |= (G.id[a] == j);
         if (!member) {
-            const float cx = load_dev(V.xyz + 3 * (int64_t)j), cy = load_dev(V.xyz + 3 * (int64_t)j + 1),
-                        cz = load_dev(V.xyz + 3 * (int64_t)j + 2);
+            const float cx = V.xyz[3 * (int64_t)j], cy = V.xyz[3 * (int64_t)j + 1], cz = V.xyz[3 * (int64_t)j + 2];
             const int32_t ej = V.elem[j], mj = V.mol[j];
             for (int32_t a = 0; a < g; a++) {
                 const float so = pair_s<PBC, ORTHO>(G.o[a][0], G.o[a][1], G.o[a][2], cx, cy, cz, V.B);
@@ -596,7 +621,7 @@ int launch_eval(const View &V, int32_t k, hipStream_t st) {
     const size_t shm_a = eval_lds_bytes(V);
     if (shm_a > 48 * 1024)
         (void)hipFuncSetAttribute((const void *)k_eval_a, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_a);
-    k_eval_a<<<dim3((unsigned)V.nunits, (unsigned)k), 64, shm_a, st>>>(V, k);
+    k_eval_a<<<dim3((unsigned)V.nunits, (unsigned)k), 64 * kEW, shm_a, st>>>(V, k);
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;
 }
