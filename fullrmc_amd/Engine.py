@@ -312,17 +312,9 @@ class Engine:
 
     def _run_native(self, h, nsteps, k, gen, log):
         L = _lib.lib()
-        py, npst = _lib.MT19937(), _lib.MT19937()
         pys = random.getstate()
-        for q in range(624):
-            py.mt[q] = pys[1][q]
-        py.pos = pys[1][624]
         nps = np.random.get_state()
-        if nps[3]:
-            raise ValueError("numpy.random holds a cached Gaussian; the native generator cannot carry it")
-        for q in range(624):
-            npst.mt[q] = int(nps[1][q])
-        npst.pos = int(nps[2])
+        py, npst = _mt_from_python(pys), _mt_from_numpy(nps)
         st = _lib.RunStats()
         glog = np.empty(nsteps, np.int32) if log is not None else None
         alog = np.empty(nsteps, np.uint8) if log is not None else None
@@ -333,9 +325,8 @@ class Engine:
             glog.ctypes.data_as(_lib.c_i32p) if glog is not None else None,
             alog.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)) if alog is not None else None,
             tlog.ctypes.data_as(_lib.c_f32p) if tlog is not None else None))
-        random.setstate((pys[0], tuple(int(py.mt[q]) for q in range(624)) + (int(py.pos),), pys[2]))
-        np.random.set_state(("MT19937", np.array([npst.mt[q] for q in range(624)], dtype=np.uint32),
-                             int(npst.pos), 0, 0.0))
+        random.setstate(_mt_to_python(py, pys))
+        np.random.set_state(_mt_to_numpy(npst))
         self.generated += st.generated
         self.tried += st.tried
         self.accepted += st.accepted
@@ -404,3 +395,86 @@ class Engine:
         r = np.ascontiguousarray(real, dtype=np.float32)
         _lib.check(L.frmc_engine_set_real(h, r.ctypes.data_as(_lib.c_f32p)))
         return dict(generated=step, accepted=accepted, windows=windows, evaluated=evaluated)
+
+
+# ---------------------------------------------------------------- replicas
+
+def _mt_from_python(state):
+    """frmc_mt19937 from a random.getstate() tuple."""
+    m = _lib.MT19937()
+    for q in range(624):
+        m.mt[q] = state[1][q]
+    m.pos = state[1][624]
+    return m
+
+
+def _mt_from_numpy(state):
+    """frmc_mt19937 from a numpy.random.get_state() tuple."""
+    if state[3]:
+        raise ValueError("numpy.random holds a cached Gaussian; the native generator cannot carry it")
+    m = _lib.MT19937()
+    for q in range(624):
+        m.mt[q] = int(state[1][q])
+    m.pos = int(state[2])
+    return m
+
+
+def _mt_to_python(m, like):
+    return (like[0], tuple(int(m.mt[q]) for q in range(624)) + (int(m.pos),), like[2])
+
+
+def _mt_to_numpy(m):
+    return ("MT19937", np.array([m.mt[q] for q in range(624)], dtype=np.uint32), int(m.pos), 0, 0.0)
+
+
+def run_replicas(engines, numberOfSteps=100, k=32, seeds=None):
+    """Advance independent engines (replicas) by numberOfSteps generated
+    moves each, with every replica's windows evaluated concurrently on the
+    GPU.  Each replica runs exactly as ``random.seed(s); numpy.random.seed(s);
+    engine.run_device(numberOfSteps, k)`` would run it alone, but on its own
+    pair of random streams: ``seeds`` (one per engine) starts them, and
+    without seeds each engine continues the streams of its previous replica
+    run.  Every engine needs translation moves on all groups (the native
+    loop) and the same generator amplitude and tolerance.  Returns one stats
+    dict per engine."""
+    engines = list(engines)
+    if not engines:
+        raise ValueError("no engines")
+    if seeds is not None and len(seeds) != len(engines):
+        raise ValueError("one seed per engine is required")
+    gen = engines[0].generators[0]
+    amp = (float(gen.amplitude[0]), float(gen.amplitude[1]))
+    tol = float(engines[0].tolerance)
+    for E in engines:
+        g0 = E.generators[0]
+        if not (all(g is g0 for g in E.generators) and getattr(g0, "native_kind", None) == "translation"):
+            raise ValueError("replicas need translation moves on every group")
+        if (float(g0.amplitude[0]), float(g0.amplitude[1])) != amp or float(E.tolerance) != tol:
+            raise ValueError("replicas need the same move amplitude and tolerance")
+    R = len(engines)
+    handles = (ctypes.c_void_p * R)()
+    pys, nps = (_lib.MT19937 * R)(), (_lib.MT19937 * R)()
+    stats = (_lib.RunStats * R)()
+    for r, E in enumerate(engines):
+        handles[r] = E._device_engine(max(k, 1)).value
+        if seeds is not None:
+            E._replica_rng = (random.Random(int(seeds[r])).getstate(),
+                              np.random.RandomState(int(seeds[r])).get_state())
+        elif getattr(E, "_replica_rng", None) is None:
+            raise ValueError("engine %d has no replica random streams: pass seeds" % r)
+        pys[r] = _mt_from_python(E._replica_rng[0])
+        nps[r] = _mt_from_numpy(E._replica_rng[1])
+    _lib.check(_lib.lib().frmc_engines_run_translation(handles, R, int(numberOfSteps), int(k), amp[0], amp[1], tol,
+                                                       pys, nps, stats))
+    out = []
+    for r, E in enumerate(engines):
+        E._replica_rng = (_mt_to_python(pys[r], E._replica_rng[0]), _mt_to_numpy(nps[r]))
+        st = stats[r]
+        E.generated += st.generated
+        E.tried += st.tried
+        E.accepted += st.accepted
+        E.tolerated += st.tolerated
+        E._dev_dirty = True
+        out.append(dict(generated=st.generated, accepted=st.accepted, windows=st.windows, evaluated=st.evaluated))
+    return out
+

@@ -106,6 +106,8 @@ _PROTOS = {
                                     ctypes.POINTER(MT19937), ctypes.POINTER(MT19937),
                                     ctypes.POINTER(RunStats), c_i32p,
                                     ctypes.POINTER(ctypes.c_uint8), c_f32p],
+    "frmc_engines_run_translation": [ctypes.POINTER(c_vp), c_i32, c_i64, c_i32, c_f32, c_f32, c_f32,
+                                     ctypes.POINTER(MT19937), ctypes.POINTER(MT19937), ctypes.POINTER(RunStats)],
     "frmc_engine_get_real": [c_vp, c_f32p],
     "frmc_engine_set_real": [c_vp, c_f32p],
     "frmc_engine_set_counts": [c_vp, c_i32, c_i32p],

@@ -102,31 +102,56 @@ static int32_t window_size(int64_t generated, int64_t accepted, int32_t kmax) {
     return std::max(kmin, std::min(kmax, kk));
 }
 
-}  // namespace frmc
 
-using namespace frmc;
+// One engine's native translation run, advanced one window at a time:
+// launch() draws the window's proposals and launches their evaluation (and
+// the device-decided commit), finish() waits for the totals, walks them in
+// order with the reference's accept rule and rewinds both streams to just
+// after the committed proposal's draws.  Several runs can have windows in
+// flight at once (frmc_engines_run_translation).
+struct TranslationRun {
+    frmc_engine *e = nullptr;
+    int64_t nsteps = 0;
+    int32_t k = 0;
+    float minAmp = 0.f, maxAmp = 0.f, tolerance = 0.f;
+    frmc_mt19937 *py_state = nullptr, *np_state = nullptr;
+    frmc_run_stats *stats = nullptr;
+    int32_t *group_log = nullptr;
+    uint8_t *accept_log = nullptr;
+    float *tse_log = nullptr;
+    std::vector<int32_t> gids, aoff, py_pre, py_post, np_used;
+    std::vector<float> nbox, nreal, uf, tse;
+    std::vector<double> uacc;
+    int64_t step = 0, gen0 = 0, acc0 = 0;
+    int32_t kw = 0;
+    frmc_mt19937 py0, np0;
 
-extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float minAmp, float maxAmp,
-                                           float tolerance, frmc_mt19937 *py_state, frmc_mt19937 *np_state,
-                                           frmc_run_stats *stats, int32_t *group_log, uint8_t *accept_log,
-                                           float *tse_log) {
-    FRMC_REQUIRE(e && py_state && np_state && stats, "NULL argument");
-    FRMC_REQUIRE(e->have_real, "engine was created without real coordinates");
-    FRMC_REQUIRE(k >= 1 && k <= e->kmax, "k=%d out of range [1,%d]", k, e->kmax);
-    FRMC_REQUIRE(py_state->pos >= 0 && py_state->pos <= 624 && np_state->pos >= 0 && np_state->pos <= 624,
-                 "bad MT19937 position");
-    std::vector<int32_t> gids(k), aoff(k + 1);
-    std::vector<float> nbox((size_t)k * e->gmax * 3), nreal((size_t)k * e->gmax * 3);
-    std::vector<double> uacc(k);
-    std::vector<float> uf(k);
-    std::vector<int32_t> py_pre(k), py_post(k), np_used(k);
-    std::vector<float> tse(k);
-    int64_t step = 0;
-    int64_t gen0 = 0, acc0 = 0;  // acceptance statistics of this call
-    while (step < nsteps) {
-        int32_t kw = e->adaptive ? window_size(gen0, acc0, k) : k;
+    int init() {
+        FRMC_REQUIRE(e && py_state && np_state && stats, "NULL argument");
+        FRMC_REQUIRE(e->have_real, "engine was created without real coordinates");
+        FRMC_REQUIRE(k >= 1 && k <= e->kmax, "k=%d out of range [1,%d]", k, e->kmax);
+        FRMC_REQUIRE(py_state->pos >= 0 && py_state->pos <= 624 && np_state->pos >= 0 && np_state->pos <= 624,
+                     "bad MT19937 position");
+        gids.resize(k);
+        aoff.resize(k + 1);
+        py_pre.resize(k);
+        py_post.resize(k);
+        np_used.resize(k);
+        nbox.resize((size_t)k * e->gmax * 3);
+        nreal.resize((size_t)k * e->gmax * 3);
+        uf.resize(k);
+        tse.resize(k);
+        uacc.resize(k);
+        return FRMC_OK;
+    }
+
+    bool done() const { return step >= nsteps; }
+
+    int launch() {
+        kw = e->adaptive ? window_size(gen0, acc0, k) : k;
         kw = (int32_t)std::min<int64_t>(kw, nsteps - step);
-        const frmc_mt19937 py0 = *py_state, np0 = *np_state;
+        py0 = *py_state;
+        np0 = *np_state;
         Stream py{py_state, 0}, np_{np_state, 0};
         int32_t off = 0;
         for (int32_t p = 0; p < kw; p++) {
@@ -156,7 +181,11 @@ extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32
         }
         aoff[kw] = off;
         for (int32_t p = 0; p < kw; p++) uf[p] = (float)uacc[p];
-        int rc = eval_device_decided(e, kw, gids.data(), nbox.data(), uf.data(), tolerance, tse.data());
+        return launch_device_decided(e, kw, gids.data(), nbox.data(), uf.data(), tolerance);
+    }
+
+    int finish() {
+        int rc = finish_device_decided(e, kw, tse.data());
         if (rc) return rc;
         stats->windows++;
         stats->evaluated += kw;
@@ -201,9 +230,85 @@ extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32
                 const size_t id = (size_t)e->h_gatoms[g0 + a];
                 for (int d = 0; d < 3; d++) e->h_real[3 * id + d] = nreal[3 * (size_t)(aoff[acc_p] + a) + d];
             }
+            return FRMC_OK;
         }
         // without an acceptance both streams already stand after the window
-        else if ((rc = settle_window(e, -1))) return rc;
+        return settle_window(e, -1);
+    }
+};
+
+}  // namespace frmc
+
+using namespace frmc;
+
+extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float minAmp, float maxAmp,
+                                           float tolerance, frmc_mt19937 *py_state, frmc_mt19937 *np_state,
+                                           frmc_run_stats *stats, int32_t *group_log, uint8_t *accept_log,
+                                           float *tse_log) {
+    TranslationRun run;
+    run.e = e;
+    run.nsteps = nsteps;
+    run.k = k;
+    run.minAmp = minAmp;
+    run.maxAmp = maxAmp;
+    run.tolerance = tolerance;
+    run.py_state = py_state;
+    run.np_state = np_state;
+    run.stats = stats;
+    run.group_log = group_log;
+    run.accept_log = accept_log;
+    run.tse_log = tse_log;
+    int rc = run.init();
+    if (rc) return rc;
+    while (!run.done()) {
+        if ((rc = run.launch())) return rc;
+        if ((rc = run.finish())) return rc;
     }
     return finish_windows(e);
+}
+
+extern "C" int frmc_engines_run_translation(frmc_engine **engines, int32_t nengines, int64_t nsteps, int32_t k,
+                                            float minAmp, float maxAmp, float tolerance, frmc_mt19937 *py_states,
+                                            frmc_mt19937 *np_states, frmc_run_stats *stats) {
+    FRMC_REQUIRE(engines && py_states && np_states && stats && nengines >= 1, "NULL or empty argument");
+    std::vector<TranslationRun> runs(nengines);
+    for (int32_t r = 0; r < nengines; r++) {
+        for (int32_t q = 0; q < r; q++)
+            FRMC_REQUIRE(engines[q] != engines[r], "engine %d appears twice", r);
+        TranslationRun &run = runs[r];
+        run.e = engines[r];
+        run.nsteps = nsteps;
+        run.k = k;
+        run.minAmp = minAmp;
+        run.maxAmp = maxAmp;
+        run.tolerance = tolerance;
+        run.py_state = &py_states[r];
+        run.np_state = &np_states[r];
+        run.stats = &stats[r];
+        int rc = run.init();
+        if (rc) return rc;
+    }
+    // every engine has one window in flight on its own stream; the host
+    // draws and launches all windows, then finishes them in the same order
+    bool active = true;
+    while (active) {
+        active = false;
+        for (auto &run : runs)
+            if (!run.done()) {
+                int rc = run.launch();
+                if (rc) return rc;
+                active = true;
+            }
+        for (auto &run : runs) {
+            if (run.kw <= 0) continue;
+            int rc = run.finish();
+            if (rc) return rc;
+            run.kw = 0;
+        }
+    }
+    for (auto &run : runs) {
+        int rc = finish_windows(run.e);
+        if (rc) return rc;
+    }
+    return FRMC_OK;
 }

@@ -175,8 +175,9 @@ struct frmc_engine {
 
 namespace frmc {
 View make_view(frmc_engine *e, int stage_idx);
-int eval_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
-                        const float *u, float tol, float *tse_out);
+int launch_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
+                          const float *u, float tol);
+int finish_device_decided(frmc_engine *e, int32_t k, float *tse_out);
 int settle_window(frmc_engine *e, int32_t p);
 int finish_windows(frmc_engine *e);
 void collect_times(frmc_engine *e);

@@ -203,8 +203,9 @@ static int check_device_decision(frmc_engine *e) {
 }
 
 // launch delta + eval for the k proposals in staging buffer s (and, with C,
-// the commit of the proposal the device accepts) and wait for the totals
-static int run_window(frmc_engine *e, int s, int32_t k, const Inline *I, const CommitCtl *C = nullptr) {
+// the commit of the proposal the device accepts); finish_window waits for
+// the totals
+static int launch_window(frmc_engine *e, int s, int32_t k, const Inline *I, const CommitCtl *C = nullptr) {
     const bool with_delta = I != nullptr;
     e->seq++;
     View V = make_view(e, s);
@@ -229,11 +230,15 @@ static int run_window(frmc_engine *e, int s, int32_t k, const Inline *I, const C
         rc = launch_commit(V, *I, *C, e->st);
         if (rc) return rc;
     }
-    // k_eval_b zeroed the other buffer for the next window
+    // k_eval_b zeroes the other buffer for the next window
     e->dn_used[1 - e->cur_dn] = 0;
     e->dn_used[e->cur_dn] = with_delta ? k : 0;
     e->cur_dn = 1 - e->cur_dn;
-    rc = wait_done(e, k);
+    return FRMC_OK;
+}
+
+static int finish_window(frmc_engine *e, int32_t k) {
+    int rc = wait_done(e, k);
     if (rc) return rc;
     rc = check_device_decision(e);
     if (rc) return rc;
@@ -241,12 +246,19 @@ static int run_window(frmc_engine *e, int s, int32_t k, const Inline *I, const C
     return FRMC_OK;
 }
 
+static int run_window(frmc_engine *e, int s, int32_t k, const Inline *I, const CommitCtl *C = nullptr) {
+    int rc = launch_window(e, s, k, I, C);
+    if (rc) return rc;
+    return finish_window(e, k);
+}
+
 // Stage k proposals (group ids + new box coordinates; old ones from the host
 // mirror) into the staging buffer not read by an in-flight commit, and
-// evaluate them.  With u (acceptance draws) the device also commits the
-// proposal the reference's rule accepts, queued behind the evaluation.
-static int stage_and_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
-                          const float *u = nullptr, float tol = 0.0f) {
+// launch their evaluation.  With u (acceptance draws) the device also
+// commits the proposal the reference's rule accepts, queued behind the
+// evaluation.  complete_eval waits for the totals.
+static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
+                            const float *u = nullptr, float tol = 0.0f) {
     const int s = 1 - e->cur_stage;
     Stage &S = e->stage[s];
     int32_t off = 0;
@@ -289,15 +301,27 @@ static int stage_and_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, c
         C.tol = tol;
         C.dn = e->dnbuf[e->cur_dn];  // the buffer this window's deltas go to
         for (int32_t p = 0; p < k; p++) C.u[p] = u[p];
-        rc = run_window(e, s, k, &I, &C);
+        rc = launch_window(e, s, k, &I, &C);
         e->device_decided = true;
     } else {
-        rc = run_window(e, s, k, &I);
+        rc = launch_window(e, s, k, &I);
         e->device_decided = false;
     }
+    return rc;
+}
+
+static int complete_eval(frmc_engine *e, int32_t k) {
+    int rc = finish_window(e, k);
     if (rc) return rc;
     e->last_k = k;
     return FRMC_OK;
+}
+
+static int stage_and_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
+                          const float *u = nullptr, float tol = 0.0f) {
+    int rc = stage_and_launch(e, k, group_ids, new_box, u, tol);
+    if (rc) return rc;
+    return complete_eval(e, k);
 }
 
 // host mirror of a committed proposal p of the last window (the group's new
@@ -315,11 +339,17 @@ static void note_commit(frmc_engine *e, int32_t p) {
     e->last_k = 0;  // the window now describes a stale state
 }
 
-// Evaluate a window whose commit the device decides; the host applies the
-// same rule to the returned totals, then calls settle_window.
-int eval_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
-                        const float *u, float tol, float *tse_out) {
-    int rc = stage_and_eval(e, k, group_ids, new_box, u, tol);
+// A window whose commit the device decides, in two halves so that several
+// engines can have windows in flight: launch_device_decided stages and
+// launches it, finish_device_decided waits and returns the totals.  The host
+// applies the same rule to them, then calls settle_window.
+int launch_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
+                          const float *u, float tol) {
+    return stage_and_launch(e, k, group_ids, new_box, u, tol);
+}
+
+int finish_device_decided(frmc_engine *e, int32_t k, float *tse_out) {
+    int rc = complete_eval(e, k);
     if (rc) return rc;
     memcpy(tse_out, e->h_tse.data(), sizeof(float) * k);
     return FRMC_OK;
@@ -484,8 +514,12 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             dc.nprog = (int32_t)pg.size();
             ECHK(dalloc(e, &dc.base, (size_t)e->T * dc.H));
             ECHK(dalloc(e, &dc.slot, 2 * (size_t)e->nel * e->nel * dc.H + 1));
-            dc.dnOff = off;
-            off += e->T * dc.H;
+            if (dc.sameGrid) {
+                dc.dnOff = e->c[ci - 1].dnOff;  // identical deltas: one block serves both constraints
+            } else {
+                dc.dnOff = off;
+                off += e->T * dc.H;
+            }
             e->Hmax = std::max(e->Hmax, dc.H);
             e->Mmax = std::max(e->Mmax, dc.M);
             e->maxLeaves = std::max(e->maxLeaves, dc.nleaves);

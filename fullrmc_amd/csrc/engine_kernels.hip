@@ -104,16 +104,15 @@ __device__ __forceinline__ void load_group(const View &V, int32_t p, GroupLds &G
 // ----------------------------------------------------------------- k_delta
 
 // -1 at the old and +1 at the new bin of one (group atom, other atom) pair,
-// per constraint grid (bins are reused when a grid repeats the previous one)
+// per distinct constraint grid (a constraint whose grid repeats the previous
+// one shares its delta block)
 template <bool PBC, bool ORTHO>
 __device__ __forceinline__ void delta_pair(const View &V, int32_t *out, float so, float sn, int32_t ea, int32_t eb) {
-    int32_t bo = -1, bn = -1;
     for (int32_t ci = 0; ci < V.C; ci++) {
         const DevC &c = V.c[ci];
-        if (!c.sameGrid) {
-            bo = in_range(so, c.g) ? bin_of(so, c.g) : -1;
-            bn = in_range(sn, c.g) ? bin_of(sn, c.g) : -1;
-        }
+        if (c.sameGrid) continue;
+        const int32_t bo = in_range(so, c.g) ? bin_of(so, c.g) : -1;
+        const int32_t bn = in_range(sn, c.g) ? bin_of(sn, c.g) : -1;
         if (bo >= 0) {
             const int32_t t = type_index(bo, ea, eb, V.nel, c.H);
             if (t >= 0) atomicSub(&out[c.dnOff + t], 1);

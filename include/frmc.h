@@ -281,6 +281,17 @@ int frmc_engine_get_state(frmc_engine *e, float *box_out, int32_t constraint, in
 int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float minAmp, float maxAmp,
                                 float tolerance, frmc_mt19937 *py_state, frmc_mt19937 *np_state,
                                 frmc_run_stats *stats, int32_t *group_log, uint8_t *accept_log, float *tse_log);
+/* Independent replicas (no reference counterpart: many separate runs of
+   Engine.run, Engine.py:3377-3516, each with its own engine and its own pair
+   of random streams): every engine advances by nsteps generated moves
+   exactly as frmc_engine_run_translation would advance it alone, while the
+   windows of all engines are in flight together on their own streams (the
+   host draws and launches every engine's window, then finishes them in
+   turn).  py_states, np_states and stats are arrays of nengines entries
+   (stats accumulate). */
+int frmc_engines_run_translation(frmc_engine **engines, int32_t nengines, int64_t nsteps, int32_t k, float minAmp,
+                                 float maxAmp, float tolerance, frmc_mt19937 *py_states, frmc_mt19937 *np_states,
+                                 frmc_run_stats *stats);
 /* Per-kernel timing with HIP events recorded on the engine stream around
    the launches of every `every`-th window (two event records per timed
    launch).  frmc_engine_profile(e, every >= 1) resets and enables, (e, 0)

@@ -210,3 +210,51 @@ def test_config1_size_properties():
         chis.append(c.compute_standard_error(c._model({"intra": fi, "inter": fe})))
         assert np.float32(chis[-1]).tobytes() == np.float32(c.standardError).tobytes()
     assert np.float32(np.sum(chis)).tobytes() == np.float32(E.totalStandardError).tobytes()
+
+
+def test_replicas_run_concurrently_and_match_independent_runs():
+    """Three replicas of different sizes advanced together (windows of all
+    engines in flight on their own streams) end in exactly the state of three
+    independent single-engine runs with the same seeds, and replica 0 in the
+    oracle's state."""
+    from fullrmc_amd.Engine import run_replicas
+    seeds, nsteps = [11, 12, 13], 150
+    boxes = []
+    for i, n in enumerate((300, 500, 420)):
+        L = (n / 0.05) ** (1 / 3)
+        s = systems.random_box(n, L, 2.0, seed=40 + i)
+        s["exp"] = pdc_target(s, 0.1, 50 + i, 0.025, 0.05, 150)
+        boxes.append(s)
+
+    def engine(s):
+        E = device_engine(s, [PairDistributionConstraint(s["exp"])])
+        E.set_move_generator(TranslationGenerator(0.2))
+        return E
+
+    reps = [engine(s) for s in boxes]
+    stats = run_replicas(reps, nsteps, k=16, seeds=seeds)
+    assert all(st["generated"] == nsteps for st in stats)
+    for E, s, seed in zip(reps, boxes, seeds):
+        E.sync_from_device()
+        random.seed(seed)
+        np.random.seed(seed)
+        S = engine(s)
+        S.run_device(nsteps, k=16)
+        assert E.realCoordinates.tobytes() == S.realCoordinates.tobytes()
+        assert E.boxCoordinates.tobytes() == S.boxCoordinates.tobytes()
+        assert np.float32(E.totalStandardError).tobytes() == np.float32(S.totalStandardError).tobytes()
+        assert E.accepted == S.accepted
+        for c, cs in zip(E.constraints, S.constraints):
+            assert c.data["intra"].tobytes() == cs.data["intra"].tobytes()
+            assert c.data["inter"].tobytes() == cs.data["inter"].tobytes()
+        assert E._replica_rng[0] == random.getstate()
+        assert np.array_equal(E._replica_rng[1][1], np.random.get_state()[1])
+    s = boxes[0]
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(len(s["real"]))], oeng.Translation(0.2), _pdc_cons_of(s),
+                                    nsteps, seeds[0])
+    assert reps[0].realCoordinates.tobytes() == oe.s.real.tobytes()
+    assert np.float32(reps[0].totalStandardError).tobytes() == np.float32(oe.tse).tobytes()
+
+
+def _pdc_cons_of(s):
+    return lambda osys, w: [oeng.PDC(osys, s["exp"], w)]
