@@ -238,6 +238,103 @@ def kernel_roofline(name, avg_s, n, props_per_launch):
             "kernel": name, "avg_launch_us": round(avg_s * 1e6, 2), "flops_per_launch": flops}
 
 
+def _replica_boxes(args):
+    from fullrmc_amd import systems
+    r, n, seed = args
+    L = float((n / 0.05) ** (1.0 / 3.0))
+    start = systems.random_box(int(n), L, 2.0, seed + 1 + 2 * r)
+    target = systems.jitter(start, 0.1, seed + 2 + 2 * r)
+    return start, target
+
+
+def build_replica_boxes(nrep, seed=2040):
+    """Independent single-element boxes: N log-uniform in [1000, 20000],
+    cubic edge for a number density of 0.05 / A^3, 2.0 A minimum distance,
+    and a sigma = 0.1 A jittered copy of each as its target structure.  Built
+    by a process pool; call before the GPU is initialised."""
+    import multiprocessing
+    from fullrmc_amd import systems
+    sizes = systems.replica_sizes(nrep, 1000, 20000, seed)
+    jobs = [(r, int(n), seed) for r, n in enumerate(sizes)]
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    with multiprocessing.get_context("fork").Pool(workers) as pool:
+        return pool.map(_replica_boxes, jobs)
+
+
+def build_replicas(boxes):
+    """Each replica's float64 target G(r) on r in [0, 15) @ 0.05 (GPU pair
+    counts of its target structure)."""
+    out = []
+    for start, target in boxes:
+        elem, pdc, _ = target_curves(start, target, 0.05, 300, np.array([1.0]))
+        out.append((start, pdc))
+    return out
+
+
+def replicas_section(args, boxes):
+    """Independent replicas advanced together (run_replicas), beside the
+    oracle engine on a sample of the same replicas."""
+    import torch
+    from fullrmc_amd.Constraints import PairDistributionConstraint
+    from fullrmc_amd.Engine import Engine, run_replicas
+    from fullrmc_amd.Generators import TranslationGenerator
+    t0 = time.perf_counter()
+    reps = build_replicas(boxes)
+    engines = []
+    for start, pdc in reps:
+        E = Engine()
+        E.set_system(start["real"], start["elements"], start["molecules"], start["basis"], start["recip"])
+        E.set_move_generator(TranslationGenerator(0.1))
+        E.add_constraints([PairDistributionConstraint(pdc)])
+        engines.append(E)
+    seeds = [3000 + r for r in range(len(engines))]
+    run_replicas(engines, 20, k=32, seeds=seeds)  # engine creation and warm-up
+    torch.cuda.synchronize()
+    print("bench: %d replicas built (%.1f s)" % (len(engines), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    stats = run_replicas(engines, args.replica_moves, k=32)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    moves = sum(st["generated"] for st in stats)
+    sizes = [len(start["real"]) for start, _ in reps]
+    out = {"workload": "%d independent single-element boxes, N log-uniform in [1000, 20000] (mean %.0f), density "
+                       "0.05/A^3, PDC r[0,15]@0.05, translation 0.1 A, k<=32, own random streams per replica"
+                       % (len(reps), float(np.mean(sizes))),
+           "moves_per_s": round(moves / el, 1), "moves": moves, "seconds": round(el, 3),
+           "accepted_fraction": round(sum(st["accepted"] for st in stats) / moves, 4),
+           "windows": sum(st["windows"] for st in stats)}
+    print("bench: replicas done (%.1f moves/s)" % out["moves_per_s"], file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        # the oracle engine on four replicas spread over the size range, equal time each
+        order = np.argsort(sizes)
+        pick = [int(order[int(q * (len(order) - 1))]) for q in (0.125, 0.375, 0.625, 0.875)]
+        cm = ct = 0.0
+        for r in pick:
+            start, pdc = reps[r]
+            oe = cpu_engine_pdc(start, pdc)
+            rate, n, el_r = cpu_moves_per_s(oe, args.cpu_seconds / 8)
+            cm += n
+            ct += el_r
+        out["cpu_baseline"] = {"value": round(cm / ct, 2), "unit": "moves/s", "cores": 1, "kind": "port",
+                               "sample": "%d moves on 4 replicas at the 12.5/37.5/62.5/87.5 %% size quantiles "
+                                         "(oracle, 1 thread, equal time each) in %.1f s" % (cm, ct)}
+        out["speedup_vs_cpu"] = round(out["moves_per_s"] / (cm / ct), 1)
+    return out
+
+
+def cpu_engine_pdc(start, pdc, seed=7):
+    from oracle import chain
+    from oracle import engine as oeng
+    from fullrmc_amd.Constraints.PairDistributionConstraints import ATOMIC_NUMBER
+    random.seed(seed)
+    np.random.seed(seed)
+    osys = oeng.System(start["real"], start["elements"], start["molecules"], start["basis"], start["recip"])
+    w = chain.normalized_weighting(osys.counts, {e: ATOMIC_NUMBER[e] for e in osys.elements})
+    oe = oeng.Engine(osys, [[i] for i in range(len(start["real"]))], oeng.Translation(0.1), [oeng.PDC(osys, pdc, w)])
+    oe.initialize()
+    return oe
+
+
 def sio2_section(args):
     """200,000-atom SiO2 box: full histogram (float32 reference binning and
     the binary64-recompute mode) and device moves with k <= 128, beside the
@@ -304,6 +401,8 @@ def main():
     ap.add_argument("--sio2", action=argparse.BooleanOptionalAction, default=True,
                     help="also run the 200,000-atom SiO2 box (full histograms and k=128 moves)")
     ap.add_argument("--sio2-moves", type=int, default=3000)
+    ap.add_argument("--replicas", type=int, default=256, help="independent replicas run together (0: skip)")
+    ap.add_argument("--replica-moves", type=int, default=200, help="generated moves per replica")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -311,6 +410,13 @@ def main():
         print("bench.py measures one GPU: the pair path runs on a single MI355X; "
               "a sharded multi-GPU run is not implemented.", file=sys.stderr)
         sys.exit(2)
+
+    replica_boxes = None
+    if args.replicas > 0:
+        t0 = time.perf_counter()
+        replica_boxes = build_replica_boxes(args.replicas)  # before the GPU is initialised (fork pool)
+        print("bench: %d replica boxes built (%.1f s)" % (len(replica_boxes), time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
 
     import torch
     from fullrmc_amd import _lib
@@ -400,6 +506,8 @@ def main():
 
     if args.sio2:
         result["sio2_200k"] = sio2_section(args)
+    if args.replicas > 0:
+        result["replicas"] = replicas_section(args, replica_boxes)
 
     if not args.no_cpu:
         oe = cpu_engine(start, pdc, sfc, 0.02)

@@ -93,11 +93,13 @@ def _rsa(n, basis, rmin, rng, max_tries=200):
     return (pts @ B).astype(F32)
 
 
-def _rsa_cubic_batched(n, L, rmin, rng, batch=40000):
+def _rsa_cubic_batched(n, L, rmin, rng, batch=None):
     """Random insertion in a periodic cube, vectorised over batches of
     candidates (periodic k-d trees): a candidate is kept if no kept atom lies
     within rmin and no earlier candidate of its batch does."""
     from scipy.spatial import cKDTree
+    if batch is None:
+        batch = int(min(40000, max(2000, n // 4)))
     pts = np.empty((0, 3))
     while len(pts) < n:
         cand = rng.random((batch, 3)) * L
@@ -117,7 +119,7 @@ def random_box(n, L, rmin, seed, species=("Ar",), fractions=None):
     closer than rmin; species assigned by fractions (default all the first)."""
     rng = np.random.default_rng(seed)
     basis = cubic_basis(L)
-    real = _rsa_cubic_batched(n, L, rmin, rng) if n > 20000 else _rsa(n, basis, rmin, rng)
+    real = _rsa_cubic_batched(n, L, rmin, rng) if n > 2000 else _rsa(n, basis, rmin, rng)
     if fractions is None:
         elements = [species[0]] * n
     else:
