@@ -342,19 +342,26 @@ class Engine:
         step = 0
         windows = evaluated = accepted = 0
         C = len(self.constraints)
+        def propose():
+            gi = self.selector.select_index(len(self.groups))
+            before = np.array(real[self.groups[gi]], dtype=FLOAT_TYPE)
+            return gi, self.generators[gi].move(before)
+
+        gen0 = acc0 = 0
         while step < nsteps:
-            kw = min(k, nsteps - step)
+            # proposals drawn by the Python generators cost more than a window
+            # on the device: about one expected acceptance per window
+            p_acc = (acc0 + 1.0) / (gen0 + 2.0)
+            kw = min(k, nsteps - step, max(1, int(np.ceil(1.0 / p_acc))))
             gids = np.empty(kw, np.int32)
-            boxes, reals, u, st_pre, st_post = [], [], [], [], []
+            boxes, reals, u = [], [], []
+            # the streams are saved once per window; an accepted proposal's
+            # position is reached again by replaying the window's draws
+            st0 = (random.getstate(), np.random.get_state())
             for p in range(kw):
-                gi = self.selector.select_index(len(self.groups))
-                rel = self.groups[gi]
-                before = np.array(real[rel], dtype=FLOAT_TYPE)
-                mr = self.generators[gi].move(before)
+                gi, mr = propose()
                 mb = box_coordinates(self.reciprocalBasisVectors, mr)
-                st_pre.append((random.getstate(), np.random.get_state()))
                 u.append(random.random())
-                st_post.append((random.getstate(), np.random.get_state()))
                 gids[p] = gi
                 boxes.append(mb)
                 reals.append(mr)
@@ -369,6 +376,7 @@ class Engine:
             for p in range(kw):
                 self.generated += 1
                 self.tried += 1
+                gen0 += 1
                 ok, drew = True, False
                 if tse[p] > cur:
                     drew = True
@@ -381,11 +389,15 @@ class Engine:
                     log.append((int(gids[p]), ok, float(tse[p] if ok else cur)))
                 if ok:
                     acc = p
+                    acc0 += 1
                     break
             if acc >= 0:
-                pys, nps = (st_post if drew else st_pre)[acc]
-                random.setstate(pys)
-                np.random.set_state(nps)
+                random.setstate(st0[0])
+                np.random.set_state(st0[1])
+                for p in range(acc + 1):
+                    propose()
+                    if p < acc or drew:
+                        random.random()
                 _lib.check(L.frmc_engine_commit(h, acc))
                 self.accepted += 1
                 accepted += 1

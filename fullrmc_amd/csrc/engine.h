@@ -50,12 +50,18 @@ struct DevC {
     int32_t dnOff;          // offset in a proposal's delta block
 };
 
-// Window staging in pinned host memory (device-accessible).
+// Window staging: filled by the host in pinned memory (one block, in this
+// order), copied to the device block before the window's kernels (one
+// asynchronous copy on the engine stream), which read only the device copy.
 struct Stage {
     int32_t *pgroup;  // [kmax]
     int32_t *poff;    // [kmax+1]
     float *pold;      // [kmax*gmax*3]
     float *pnew;      // [kmax*gmax*3]
+    void *host = nullptr, *dev = nullptr;  // the two blocks
+    size_t bytes = 0;
+    int32_t *d_pgroup = nullptr, *d_poff = nullptr;
+    float *d_pold = nullptr, *d_pnew = nullptr;
 };
 
 struct View {

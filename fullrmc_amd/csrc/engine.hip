@@ -39,10 +39,10 @@ View make_view(frmc_engine *e, int s) {
     V.Mmax = e->Mmax;
     V.maxLeaves = e->maxLeaves;
     V.nunits = e->nunits;
-    V.pgroup = e->stage[s].pgroup;
-    V.poff = e->stage[s].poff;
-    V.pold = e->stage[s].pold;
-    V.pnew = e->stage[s].pnew;
+    V.pgroup = e->stage[s].d_pgroup;
+    V.poff = e->stage[s].d_poff;
+    V.pold = e->stage[s].d_pold;
+    V.pnew = e->stage[s].d_pnew;
     V.dn = e->dnbuf[e->cur_dn];
     V.dn_zero = e->dnbuf[1 - e->cur_dn];
     V.zero_len = (int64_t)e->dn_used[1 - e->cur_dn] * e->dnStride;
@@ -276,7 +276,8 @@ static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids,
     }
     S.poff[k] = off;
     memcpy(S.pnew, new_box, sizeof(float) * 3 * (size_t)off);
-    __sync_synchronize();
+    // the kernels read the device copy; the copy is ordered before them on the stream
+    FRMC_CHECK_HIP(hipMemcpyAsync(S.dev, S.host, S.bytes, hipMemcpyHostToDevice, e->st));
     e->cur_stage = s;
     Inline &I = e->last_inline;
     I.k = 0;
@@ -539,10 +540,22 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         for (int b = 0; b < 2; b++) {
             ECHK(dalloc(e, &e->dnbuf[b], K * e->dnStride));
             ECHK(hipMemset(e->dnbuf[b], 0, sizeof(int32_t) * K * e->dnStride));
-            ECHK(halloc(e, &e->stage[b].pgroup, K));
-            ECHK(halloc(e, &e->stage[b].poff, K + 1));
-            ECHK(halloc(e, &e->stage[b].pold, K * e->gmax * 3));
-            ECHK(halloc(e, &e->stage[b].pnew, K * e->gmax * 3));
+            Stage &S = e->stage[b];
+            const size_t ni = 2 * K + 2, nf = 2 * K * (size_t)e->gmax * 3;  // K+1 offsets padded to even
+            S.bytes = 4 * (ni + nf);
+            char *hb = nullptr, *db = nullptr;
+            ECHK(halloc(e, &hb, S.bytes));
+            ECHK(dalloc(e, &db, S.bytes));
+            S.host = hb;
+            S.dev = db;
+            S.pgroup = (int32_t *)hb;
+            S.poff = S.pgroup + K;
+            S.pold = (float *)(hb + 4 * ni);
+            S.pnew = S.pold + K * (size_t)e->gmax * 3;
+            S.d_pgroup = (int32_t *)db;
+            S.d_poff = S.d_pgroup + K;
+            S.d_pold = (float *)(db + 4 * ni);
+            S.d_pnew = S.d_pold + K * (size_t)e->gmax * 3;
         }
         ECHK(dalloc(e, &e->sqbuf, K * e->C * e->Mmax));
         ECHK(dalloc(e, &e->chi2d, K * e->C));

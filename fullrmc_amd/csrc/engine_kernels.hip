@@ -29,18 +29,14 @@
 
 namespace frmc {
 
-// Uniform (wave-invariant) loads of data that an earlier kernel of the window
-// or the host wrote go through coherent vector loads: a plain load from a
+// Uniform (wave-invariant) loads of data that an earlier kernel or copy of
+// the window wrote go through coherent vector loads: a plain load from a
 // uniform address is compiled to a scalar load through the scalar data cache,
 // which has returned stale values here between back-to-back kernels of a
 // stream.  Lane-varying loads use the vector path and need no such care.
 template <typename T>
 __device__ __forceinline__ T load_dev(const T *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T load_host(const T *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __device__ __forceinline__ int32_t pair_type(int32_t a, int32_t b, int32_t nel) {
@@ -84,9 +80,9 @@ struct GroupLds {
 
 // proposal p of the staged window into G (all threads of the block call it)
 __device__ __forceinline__ void load_group(const View &V, int32_t p, GroupLds &G) {
-    const int32_t gi = load_host(V.pgroup + p);
-    const int32_t a0 = load_host(V.poff + p);
-    const int32_t g = load_host(V.poff + p + 1) - a0;
+    const int32_t gi = load_dev(V.pgroup + p);
+    const int32_t a0 = load_dev(V.poff + p);
+    const int32_t g = load_dev(V.poff + p + 1) - a0;
     const int32_t *atoms = V.gatoms + V.goff[gi];
     if (threadIdx.x == 0) G.g = g;
     for (int32_t a = threadIdx.x; a < g; a += blockDim.x) {
@@ -95,8 +91,8 @@ __device__ __forceinline__ void load_group(const View &V, int32_t p, GroupLds &G
         G.el[a] = V.elem[id];
         G.mol[a] = V.mol[id];
         for (int d = 0; d < 3; d++) {
-            G.o[a][d] = load_host(V.pold + 3 * (int64_t)(a0 + a) + d);
-            G.w[a][d] = load_host(V.pnew + 3 * (int64_t)(a0 + a) + d);
+            G.o[a][d] = V.pold[3 * (int64_t)(a0 + a) + d];
+            G.w[a][d] = V.pnew[3 * (int64_t)(a0 + a) + d];
         }
     }
 }

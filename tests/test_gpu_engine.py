@@ -258,3 +258,25 @@ def test_replicas_run_concurrently_and_match_independent_runs():
 
 def _pdc_cons_of(s):
     return lambda osys, w: [oeng.PDC(osys, s["exp"], w)]
+
+
+def test_molecular_triclinic_full_size():
+    """500 rigid 13-atom rings (6,500 atoms) in the 42/44/46 A, 80/95/105 deg
+    cell, molecule translation + rotation through the Python generators:
+    the device trajectory equals the oracle's."""
+    s = systems.ring_box(nmol=500, cell=(42.0, 44.0, 46.0, 80.0, 95.0, 105.0), seed=61, min_centre=4.5)
+    s["elem"] = element_index(s["elements"])
+    exp = pdc_target(s, 0.05, 62, 0.025, 0.05, 300)
+    mols = {}
+    for i, m in enumerate(s["molecules"]):
+        mols.setdefault(int(m), []).append(i)
+    groups = list(mols.values())
+    ogen = oeng.Combined([oeng.Translation(0.2), oeng.Rotation(10)])
+    oe, olog, pys, nps = oracle_run(s, groups, ogen, lambda osys, w: [oeng.PDC(osys, exp, w)], 40, 63)
+    random.seed(63)
+    np.random.seed(63)
+    gen = MoveGeneratorCombinator([TranslationGenerator(0.2), RotationGenerator(10)])
+    E = device_engine(s, [PairDistributionConstraint(exp)], groups=groups, gen=gen)
+    dlog = []
+    E.run_device(40, k=8, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
