@@ -1,6 +1,7 @@
 """PairDistributionConstraint with the reference's float32 semantics
 (Constraints/PairDistributionConstraints.py), histograms through the gfx950
-drop-in ``fullrmc_amd.Core.pairs_histograms``.
+drop-in ``fullrmc_amd.Core.pairs_histograms``.  PairCorrelationConstraint
+(Constraints/PairCorrelationConstraints.py) lives in PairCorrelationConstraints.
 
 Supported options: experimental data (n,2), data weights, weighting scheme
 or explicit atoms weights, constant scale factor.  Shape functions, window
@@ -241,7 +242,7 @@ class PairDistributionConstraint(_HistogramConstraint):
             Nij = ni * (ni - 1) / 2.0 if a == b else ni * nj
             w[t] = wij
             Dd[t] = FLOAT_TYPE(Nij / e.volume)
-        return dict(kind=0, H=int(self.histogramSize), minD=self.minimumDistance, maxD=self.maximumDistance,
+        return dict(kind=self.kind, H=int(self.histogramSize), minD=self.minimumDistance, maxD=self.maximumDistance,
                     bin=self.bin, nQ=0, w=w, D=Dd, shellVolumes=np.ascontiguousarray(self.shellVolumes),
                     prefactor=np.ascontiguousarray(self._prefactor(), dtype=FLOAT_TYPE),
                     experimental=np.ascontiguousarray(self._experimental), dataWeights=self._usedDataWeights,

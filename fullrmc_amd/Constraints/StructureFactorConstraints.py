@@ -1,6 +1,7 @@
 """StructureFactorConstraint with the reference's float32 semantics
 (Constraints/StructureFactorConstraints.py): its own histogram grid
-(rmin, rmax, dr), G(r) on that grid, S(Q) = sum_r G(r) * dr sin(rQ)/Q + 1.
+(rmin, rmax, dr), G(r) on that grid, S(Q) = sum_r G(r) * dr sin(rQ)/Q + 1;
+ReducedStructureFactorConstraint: the same sum without the +1.
 """
 import numpy as np
 
@@ -60,14 +61,24 @@ class StructureFactorConstraint(_HistogramConstraint):
     def _prefactor(self):
         return FLOAT_TYPE(4.) * PI * self.shellCenters * self.engine.numberDensity
 
+    def _get_Sq_from_Gr(self, Gr):
+        """StructureFactorConstraints.py:772-773"""
+        return np.sum(Gr.reshape((-1, 1)) * self.Gr2SqMatrix, axis=0) + 1
+
+    def _apply_scale_factor(self, Sq, scaleFactor):
+        """StructureFactorConstraints.py:775-778"""
+        if scaleFactor != 1:
+            Sq = scaleFactor * (Sq - 1) + 1
+        return Sq
+
     def _model(self, data):
+        """__get_total_Sq (StructureFactorConstraints.py:780-822) without
+        scale-factor fitting, multiframe prior or window."""
         Gr = self._gr_sum(data, cast_density=False)
         Gr /= self.shellVolumes
         Gr = (FLOAT_TYPE(4.) * PI * self.shellCenters * self.engine.numberDensity) * (Gr - 1)
-        Sq = np.sum(Gr.reshape((-1, 1)) * self.Gr2SqMatrix, axis=0) + 1
-        if self.scaleFactor != 1:
-            Sq = self.scaleFactor * (Sq - 1) + 1
-        return Sq
+        Sq = self._get_Sq_from_Gr(Gr)
+        return self._apply_scale_factor(Sq, FLOAT_TYPE(self.scaleFactor))
 
     def _device_arrays(self):
         e = self.engine
@@ -80,10 +91,29 @@ class StructureFactorConstraint(_HistogramConstraint):
             Nij = ni * (ni - 1) / 2.0 if a == b else ni * nj
             w[t] = wij
             Dd[t] = Nij / e.volume
-        return dict(kind=1, H=int(self.histogramSize), minD=self.minimumDistance, maxD=self.maximumDistance,
+        return dict(kind=self.kind, H=int(self.histogramSize), minD=self.minimumDistance, maxD=self.maximumDistance,
                     bin=self.bin, nQ=len(self.experimentalQValues), w=w, D=Dd,
                     shellVolumes=np.ascontiguousarray(self.shellVolumes),
                     prefactor=np.ascontiguousarray(self._prefactor(), dtype=FLOAT_TYPE),
                     experimental=np.ascontiguousarray(self._experimental), dataWeights=self._usedDataWeights,
                     sqMatrix=np.ascontiguousarray(self.Gr2SqMatrix, dtype=FLOAT_TYPE),
                     scaleFactor=FLOAT_TYPE(self.scaleFactor))
+
+
+class ReducedStructureFactorConstraint(StructureFactorConstraint):
+    """Reduced S(Q) constraint (StructureFactorConstraints.py:1237-1261):
+    S(Q) normalised to 0, the sine transform of G(r) without the +1, and a
+    scale factor that multiplies S(Q) itself.  ``experimentalData`` is (n,2):
+    Q (>0) and S(Q)-1."""
+
+    kind = 3
+
+    def _get_Sq_from_Gr(self, Gr):
+        """StructureFactorConstraints.py:1255-1256"""
+        return np.sum(Gr.reshape((-1, 1)) * self.Gr2SqMatrix, axis=0)
+
+    def _apply_scale_factor(self, Sq, scaleFactor):
+        """StructureFactorConstraints.py:1258-1261"""
+        if scaleFactor != 1:
+            Sq = scaleFactor * Sq
+        return Sq

@@ -32,9 +32,10 @@ constexpr int kQC = 64;    // Q points per k_eval_a wave (one per lane)
 constexpr int kEW = 1;     // waves per k_eval_a block: a work unit is kEW*kQC Q points
 
 struct DevC {
-    int32_t kind;   // FRMC_PDC or FRMC_SFC
+    int32_t kind;   // FRMC_PDC, FRMC_SFC, FRMC_PCF or FRMC_RSFC
+    int32_t rspace; // 1: model on the histogram grid (PDC, PCF); 0: S(Q) (SFC, RSFC)
     Grid g;
-    int32_t H, M;   // histogram bins, model points (H for PDC, nQ for SFC)
+    int32_t H, M;   // histogram bins, model points (H in r space, nQ for S(Q))
     int32_t hasW, hasSF;
     float sf;
     const float *w, *D, *shellV, *pref, *expv, *dataW;
@@ -44,7 +45,7 @@ struct DevC {
     const int32_t *prog;    // postfix combine program over the leaves
     int32_t nleaves, nprog;
     int32_t sameGrid;       // 1: same histogram grid as the previous constraint (bins reused)
-    int32_t uOff;           // first k_eval_a work unit (PDC: 1 unit, SFC: ceil(M/(kEW*kQC)) units)
+    int32_t uOff;           // first k_eval_a work unit (r space: 1 unit, S(Q): ceil(M/(kEW*kQC)) units)
     int32_t *base;          // [T][H]
     int32_t *slot;          // [2][nel][nel][H] + overflow counter
     int32_t dnOff;          // offset in a proposal's delta block

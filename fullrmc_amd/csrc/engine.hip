@@ -460,12 +460,14 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             const frmc_constraint_desc &cd = d->constraints[ci];
             DevC &dc = e->c[ci];
             memset(&dc, 0, sizeof(dc));
-            if (cd.kind != FRMC_PDC && cd.kind != FRMC_SFC) EFAIL(FRMC_EINVAL, "constraint %d: unknown kind", ci);
+            if (cd.kind != FRMC_PDC && cd.kind != FRMC_SFC && cd.kind != FRMC_PCF && cd.kind != FRMC_RSFC)
+                EFAIL(FRMC_EINVAL, "constraint %d: unknown kind", ci);
             if (cd.histSize < 1) EFAIL(FRMC_EINVAL, "constraint %d: histSize must be >= 1", ci);
             if (!(cd.bin > 0.f)) EFAIL(FRMC_EINVAL, "constraint %d: bin must be > 0", ci);
             dc.kind = cd.kind;
+            dc.rspace = cd.kind == FRMC_PDC || cd.kind == FRMC_PCF;
             dc.H = cd.histSize;
-            dc.M = cd.kind == FRMC_PDC ? cd.histSize : cd.nQ;
+            dc.M = dc.rspace ? cd.histSize : cd.nQ;
             if (dc.M < 1) EFAIL(FRMC_EINVAL, "constraint %d: no model points", ci);
             if ((int64_t)e->nel * e->nel * cd.histSize * 2 + 1 > INT32_MAX)
                 EFAIL(FRMC_EINVAL, "constraint %d: nel^2*histSize too large", ci);
@@ -480,7 +482,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             dc.sf = cd.scaleFactor;
             if (!cd.pairWeights || !cd.pairDensity || !cd.shellVolumes || !cd.prefactor || !cd.experimental)
                 EFAIL(FRMC_EINVAL, "constraint %d: NULL array", ci);
-            if (cd.kind == FRMC_SFC && !cd.sqMatrix) EFAIL(FRMC_EINVAL, "constraint %d: SFC needs sqMatrix", ci);
+            if (!dc.rspace && !cd.sqMatrix) EFAIL(FRMC_EINVAL, "constraint %d: S(Q) needs sqMatrix", ci);
             float *wp, *Dp, *sv, *pf, *ex, *dw = nullptr, *sm = nullptr;
             ECHK(dupload(e, &wp, cd.pairWeights, (size_t)e->T));
             ECHK(dupload(e, &Dp, cd.pairDensity, (size_t)e->T));
@@ -488,7 +490,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             ECHK(dupload(e, &pf, cd.prefactor, (size_t)dc.H));
             ECHK(dupload(e, &ex, cd.experimental, (size_t)dc.M));
             if (cd.dataWeights) ECHK(dupload(e, &dw, cd.dataWeights, (size_t)dc.M));
-            if (cd.kind == FRMC_SFC) {
+            if (!dc.rspace) {
                 // k_eval_a streams one Q column per lane: the matrix is kept transposed,
                 // each column zero-padded to a multiple of the kernel's 32-row batches
                 dc.hpb = (dc.H + 31) / 32 * 32;
@@ -529,7 +531,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         int32_t units = 0;
         for (int ci = 0; ci < e->C; ci++) {
             e->c[ci].uOff = units;
-            units += e->c[ci].kind == FRMC_PDC ? 1 : (e->c[ci].M + kEW * kQC - 1) / (kEW * kQC);
+            units += e->c[ci].rspace ? 1 : (e->c[ci].M + kEW * kQC - 1) / (kEW * kQC);
         }
         e->nunits = units;
         {

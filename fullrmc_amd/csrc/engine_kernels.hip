@@ -242,15 +242,22 @@ __device__ float block_pairwise(const float *a, const DevC &c, float *scratch, i
 // --------------------------------------------------------------- k_eval_a
 
 // One block of kEW waves per (work unit, proposal): grid (nunits, k).  A work
-// unit is one PDC constraint, or kEW*64 Q points of one SFC constraint.  The
-// block rebuilds the proposal's G(r) from base + delta in the reference's
-// float32 operation order (PairDistributionConstraints.py:854-895) into LDS,
-// its waves sharing the bins, then
-//   PDC: squared deviations and their NumPy pairwise sum (:810-838) -> chi2d;
+// unit is one r-space constraint (PDC, PCF), or kEW*64 Q points of one S(Q)
+// constraint (SFC, RSFC).  The block rebuilds the proposal's G(r) from
+// base + delta in the reference's float32 operation order
+// (PairDistributionConstraints.py:854-895) into LDS, its waves sharing the
+// bins, then
+//   PDC: squared deviations of G(r) and their NumPy pairwise sum (:810-838)
+//        -> chi2d;
+//   PCF: the same for g(r), the G(r) sum over the shell volumes, with the
+//        scale factor applied through G(r) (PairCorrelationConstraints.py:
+//        126-169);
 //   SFC: S(Q) of its Q points, one per lane, as the sequential float32 sum
-//        over r of G[r]*M[r,q] (StructureFactorConstraints.py:772-773), the
+//        over r of G[r]*M[r,q] (StructureFactorConstraints.py:772-778), the
 //        matrix column streamed from HBM/L2 (transposed, a batch of rows
-//        ahead in registers) and G broadcast from LDS -> squared deviations.
+//        ahead in registers) and G broadcast from LDS -> squared deviations;
+//   RSFC: the same sum without the +1, scaled without the shift
+//        (StructureFactorConstraints.py:1255-1261).
 // The S(Q) sum is a chain of H dependent adds per Q: the kernel is bound by
 // that latency, so each wave carries one chain per lane and the grid spreads
 // the (Q chunk, proposal) pairs over the CUs.
@@ -268,13 +275,13 @@ __global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
     // constraint fields in registers: keeps every load below unconditional
     const DevC c = V.c[ci];
     const int32_t H = c.H, M = c.M, T = V.T;
-    const bool pdc = c.kind == FRMC_PDC;
+    const bool rsp = c.rspace;
     const int32_t *dn = V.dn + (int64_t)p * V.dnStride + c.dnOff;
     const int32_t *base = c.base;
     // S(Q) for q = qc*kQC + lane (SFC units): the first rows of the matrix
     // column and the experimental value load now, under the G(r) phase
     const int32_t q = ((u - c.uOff) * kEW + w) * kQC + lane;
-    const bool qok = !pdc && q < M;
+    const bool qok = !rsp && q < M;
     const float *mcol = c.sqm + (int64_t)min(q, M - 1) * c.hpb;  // M[r][q] at mcol[r], 16-byte aligned
     constexpr int B = 32;                        // rows per register batch (hpb is a multiple of B)
     float ma[B], mb[B], ga[B], gb[B];
@@ -288,7 +295,7 @@ __global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
         }
     };
     float ex_q = 0.0f, dw_q = 1.0f;
-    if (!pdc) {
+    if (!rsp) {
         load_m(ma, 0);
         ex_q = c.expv[min(q, M - 1)];
         if (c.hasW) dw_q = c.dataW[min(q, M - 1)];
@@ -299,8 +306,8 @@ __global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
         // own wait, which serialises the batch
         constexpr int NT = 64 * kEW;
         constexpr int U = (1024 + NT - 1) / NT;  // bins per thread per pass: 1024 per pass
-        const float *expv = pdc ? c.expv : c.shellV;                 // SFC: unused (expv has M entries)
-        const float *dataW = (pdc && c.hasW) ? c.dataW : c.shellV;   // unused unless hasW
+        const float *expv = rsp ? c.expv : c.shellV;                 // S(Q): unused (expv has M entries)
+        const float *dataW = (rsp && c.hasW) ? c.dataW : c.shellV;   // unused unless hasW
         for (int32_t r0 = 0; r0 < H; r0 += NT * U) {
             int32_t rr[U];
 #pragma unroll
@@ -348,9 +355,19 @@ __global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
                 float g = Gr[i] / sv[i];           // Gr /= shellVolumes
                 float G = pf[i] * (g - 1.0f);      // (4*PI*r*rho0)*(Gr-1)
                 float out = G;
-                if (pdc) {
-                    if (c.hasSF) G = G * c.sf;
-                    const float diff = ex[i] - G;
+                if (rsp) {
+                    float model;
+                    if (c.kind == FRMC_PCF) {
+                        // g(r); a scale factor acts on G(r): g = 1 + (sf*G)/(4*PI*r*rho0)
+                        model = g;
+                        if (c.hasSF) {
+                            const float Gsf = G * c.sf;
+                            model = 1.0f + Gsf / pf[i];
+                        }
+                    } else {
+                        model = c.hasSF ? G * c.sf : G;
+                    }
+                    const float diff = ex[i] - model;
                     float v = diff * diff;
                     if (c.hasW) v = dw[i] * v;
                     out = v;
@@ -360,14 +377,14 @@ __global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
         }
     }
     __syncthreads();
-    if (pdc) {
+    if (rsp) {
         const float chi = block_pairwise(Gs, c, scratch, lprog);
         if (tid == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
         return;
     }
     // The S(Q) sum starts from +0 rather than the first product: the two
-    // differ only in the sign of a zero partial sum, which no later addend and
-    // not the final +1 can see.
+    // differ only in the sign of a zero partial sum, which no later addend,
+    // not the final +1 and (reduced S(Q)) not the squared deviation can see.
     float acc = 0.0f;
     // two register batches (matrix column and G rows) in turn: one batch loads
     // while the other's chain runs, which hides both the HBM/L2 and the LDS latency
@@ -408,11 +425,16 @@ __global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
         chain(mb, gb, r0 + B);
     }
     if (qok) {
-        float S = acc + 1.0f;
-        if (c.hasSF) {
-            float t = S - 1.0f;
-            t = c.sf * t;
-            S = t + 1.0f;
+        float S;
+        if (c.kind == FRMC_RSFC) {
+            S = c.hasSF ? c.sf * acc : acc;
+        } else {
+            S = acc + 1.0f;
+            if (c.hasSF) {
+                float t = S - 1.0f;
+                t = c.sf * t;
+                S = t + 1.0f;
+            }
         }
         const float diff = ex_q - S;
         float v = diff * diff;
@@ -444,7 +466,7 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
     for (int32_t ci = 0; ci < V.C; ci++) {
         const DevC &c = V.c[ci];
         float chi;
-        if (c.kind == FRMC_SFC) {
+        if (!c.rspace) {
             const float *src = V.sqbuf + ((int64_t)p * V.C + ci) * V.Mmax;
             for (int32_t q = threadIdx.x; q < c.M; q += blockDim.x) sm[q] = src[q];
             for (int32_t q = threadIdx.x; q < c.nprog; q += blockDim.x) lprog[q] = c.prog[q];

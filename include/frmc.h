@@ -197,27 +197,32 @@ int frmc_box_coordinates_real_distances(int32_t atomIndex, const int32_t *indexe
  * Device-resident engine: the Engine.run step (Engine.py:3166-3344) with the
  * constraint chain of PairDistributionConstraint / StructureFactorConstraint
  * (PairDistributionConstraints.py:847-895, 1044-1166;
- * StructureFactorConstraints.py:772-834, 1011-1096) on the GPU, evaluating
+ * StructureFactorConstraints.py:772-834, 1011-1096), PairCorrelationConstraint
+ * (PairCorrelationConstraints.py:126-169, 305-392) and
+ * ReducedStructureFactorConstraint (StructureFactorConstraints.py:1237-1261)
+ * on the GPU, evaluating
  * k proposals per window.  Descriptor arrays are HOST pointers, copied at
  * creation.  No reference counterpart as a single call: the reference
  * reaches the same results through per-move Python calls.
  * ---------------------------------------------------------------------- */
 
-#define FRMC_PDC 0   /* model = G(r) on the histogram grid */
+#define FRMC_PDC 0   /* model = G(r) = prefactor*(g(r)-1) on the histogram grid */
 #define FRMC_SFC 1   /* model = S(Q) = sum_r G(r) M[r,q] + 1 */
+#define FRMC_PCF 2   /* model = g(r) on the histogram grid */
+#define FRMC_RSFC 3  /* model = reduced S(Q) = sum_r G(r) M[r,q] */
 
 typedef struct {
-    int32_t kind;               /* FRMC_PDC or FRMC_SFC */
+    int32_t kind;               /* FRMC_PDC, FRMC_SFC, FRMC_PCF or FRMC_RSFC */
     int32_t histSize;           /* H */
     float minDistance, maxDistance, bin;
-    int32_t nQ;                 /* SFC: number of Q points */
+    int32_t nQ;                 /* SFC/RSFC: number of Q points */
     const float *pairWeights;   /* [T] w_ij, T = nel(nel+1)/2 pairs in sorted-element order */
     const float *pairDensity;   /* [T] D_ij = N_ij / V as float32 */
     const float *shellVolumes;  /* [H] */
     const float *prefactor;     /* [H] (4*PI*shellCenters)*rho0 as float32 */
-    const float *experimental;  /* [H] (PDC) or [nQ] (SFC) */
+    const float *experimental;  /* [H] (PDC, PCF) or [nQ] (SFC, RSFC) */
     const float *dataWeights;   /* same length or NULL */
-    const float *sqMatrix;      /* SFC: [H][nQ] dr*sin(r*Q)/Q as float32 */
+    const float *sqMatrix;      /* SFC/RSFC: [H][nQ] dr*sin(r*Q)/Q as float32 */
     float scaleFactor;          /* constant scale factor (1 = none) */
 } frmc_constraint_desc;
 

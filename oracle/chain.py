@@ -10,7 +10,11 @@ expressions, so the restatement keeps their operand order and dtypes.
   sq_matrix         StructureFactorConstraints.py:302-312
   total_gr          PairDistributionConstraints.py:847-895 (no shape function,
                     no window, constant scale factor)
-  total_sq          StructureFactorConstraints.py:780-822, 772-773
+  total_gofr        PairCorrelationConstraints.py:126-169 (no shape function,
+                    no window, constant scale factor)
+  total_sq          StructureFactorConstraints.py:780-822, 772-778;
+                    reduced=True: ReducedStructureFactorConstraint,
+                    StructureFactorConstraints.py:1255-1261
   chi_square        PairDistributionConstraints.py:810-838,
                     StructureFactorConstraints.py:742-770
   total_error       Engine.py:3000-3029 (varianceSquared = 1, Core/Constraint.py:39)
@@ -87,12 +91,29 @@ def total_gr(intra, inter, elements, counts, weights, volume, rho0, shellVolumes
     return Gr
 
 
+def total_gofr(intra, inter, elements, counts, weights, volume, rho0, shellVolumes, centers, scaleFactor=1.0):
+    """PCF model g(r): the G(r) sum divided by the shell volumes; a scale
+    factor acts on 4*PI*r*rho0*(g-1) and is mapped back to g."""
+    gr = _gr_sum(intra, inter, elements, counts, weights, volume, cast_density=False)
+    gr /= shellVolumes
+    if scaleFactor != 1:
+        Gr = (4. * PI * centers * rho0) * (gr - 1)
+        Gr *= F32(scaleFactor)
+        gr = 1. + Gr / (4. * PI * centers * rho0)
+    return gr
+
+
 def total_sq(intra, inter, elements, counts, weights, volume, rho0, shellVolumes, centers, Gr2Sq,
-             scaleFactor=1.0):
-    """SFC model S(Q)."""
+             scaleFactor=1.0, reduced=False):
+    """SFC model S(Q); reduced=True: the reduced S(Q) (no +1, plain scaling)."""
     Gr = _gr_sum(intra, inter, elements, counts, weights, volume, cast_density=False)
     Gr /= shellVolumes
     Gr = (F32(4.) * PI * centers * rho0) * (Gr - 1)
+    if reduced:
+        Sq = np.sum(Gr.reshape((-1, 1)) * Gr2Sq, axis=0)
+        if scaleFactor != 1:
+            Sq = F32(scaleFactor) * Sq
+        return Sq
     Sq = np.sum(Gr.reshape((-1, 1)) * Gr2Sq, axis=0) + 1
     if scaleFactor != 1:
         Sq = scaleFactor * (Sq - 1) + 1

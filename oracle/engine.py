@@ -12,6 +12,8 @@ constraint chain (chain.py), the accept rule, and accept/reject updates.
   Combinator          Core/MoveGenerator.py:792-811 (no shuffle)
   PDC / SFC update    PairDistributionConstraints.py:1044-1166,
                       StructureFactorConstraints.py:975-1096
+  PCF / reduced SFC   PairCorrelationConstraints.py:305-392 (same update,
+                      g(r) model), StructureFactorConstraints.py:1237-1261
 """
 import random
 
@@ -177,6 +179,24 @@ class SFC(PDC):
         s = self.s
         return chain.total_sq(intra, inter, s.elements, s.counts, self.w, s.volume, s.rho0, self.vols,
                               self.centers, self.M, self.sf)
+
+
+class PCF(PDC):
+    """PairCorrelationConstraint on the oracle path: PDC grid, g(r) model."""
+
+    def model(self, intra, inter):
+        s = self.s
+        return chain.total_gofr(intra, inter, s.elements, s.counts, self.w, s.volume, s.rho0, self.vols,
+                                self.centers, self.sf)
+
+
+class RSFC(SFC):
+    """ReducedStructureFactorConstraint on the oracle path."""
+
+    def model(self, intra, inter):
+        s = self.s
+        return chain.total_sq(intra, inter, s.elements, s.counts, self.w, s.volume, s.rho0, self.vols,
+                              self.centers, self.M, self.sf, reduced=True)
 
 
 class Engine:

@@ -398,6 +398,102 @@ def test_sq_matrix_vs_float64():
     assert np.max(np.abs(M - ref)) < 5e-6
 
 
+def _ideal_gas_histograms(n_a, n_b, L, mn, bn, H, seed):
+    """Uniform random atoms (one per molecule) in a cubic periodic box."""
+    rng = np.random.default_rng(seed)
+    n = n_a + n_b
+    box = rng.random((n, 3)).astype(F32)
+    basis = np.diag([L, L, L]).astype(F32)
+    elem = np.array([0] * n_a + [1] * n_b, np.int32)
+    mol = np.arange(n, dtype=np.int32)
+    hi, he = refcpu.full_pairs_histograms_coords(box, basis, True, mol, elem, 2, mn, F32(mn + bn * H), bn, H)
+    return hi, he, systems.volume(basis)
+
+
+def test_chain_gofr_is_one_for_an_ideal_gas():
+    """g(r) of uniformly random atoms is 1 at every r (pins the pair counts
+    N_ij, the volume, the shell volumes and the weights' normalisation
+    independently of G(r)); statistical bound from the expected pair count."""
+    counts = {"A": 600, "B": 1200}
+    x = (3.05 + 0.1 * np.arange(80)).astype(F32)
+    exp = np.stack([x, np.zeros_like(x)], 1)
+    bn, mn, mx, centers, H, vols = chain.pdc_grid(exp)
+    hi, he, V = _ideal_gas_histograms(600, 1200, F32(24.0), mn, bn, H, 11)
+    w = chain.normalized_weighting(counts, {"A": 8, "B": 14})
+    g = chain.total_gofr(hi, he, ["A", "B"], counts, w, V, F32(1800) / V, vols, centers)
+    # expected pairs per bin ~ N^2/2 * v/V; the weighted mix has at most ~2x the Poisson variance
+    npairs = 1800 * 1799 / 2 * vols.astype(np.float64) / float(V)
+    sigma = np.sqrt(2.0 / npairs)
+    assert np.all(np.abs(g - 1) < 6 * sigma)
+    assert abs(np.mean(g) - 1) < 6 * np.sqrt(np.mean(sigma ** 2) / H)
+
+
+def test_chain_gofr_and_gr_are_one_transform(small_random):
+    """G(r) = 4*PI*r*rho0*(g(r)-1) exactly in float32 without a scale factor,
+    and a scale factor multiplies g(r)-1 (PairCorrelationConstraints.py:
+    26-40, 160-163), against float64."""
+    s = small_random
+    x = (0.025 + 0.05 * np.arange(160)).astype(F32)
+    exp = np.stack([x, np.zeros_like(x)], 1)
+    bn, mn, mx, centers, H, vols = chain.pdc_grid(exp)
+    hi, he = refcpu.full_pairs_histograms_coords(s["box"], s["basis"], True, s["molecules"], s["elem"], 2, mn, mx,
+                                                 bn, H)
+    counts = {"O": 400, "Si": 200}
+    w = chain.normalized_weighting(counts, {"O": 8, "Si": 14})
+    V = systems.volume(s["basis"])
+    rho0 = F32(600) / V
+    args = (hi, he, ["O", "Si"], counts, w, V, rho0, vols, centers)
+    g = chain.total_gofr(*args)
+    G = chain.total_gr(*args)
+    pf = 4. * chain.PI * centers * rho0
+    assert G.tobytes() == (pf * (g - 1)).astype(F32).tobytes()
+    gs = chain.total_gofr(*args, scaleFactor=1.25)
+    ref = 1 + 1.25 * (g.astype(np.float64) - 1)
+    assert np.max(np.abs(gs - ref) / (1 + np.abs(ref))) < 4e-6
+    assert not np.allclose(gs, 1.25 * g, atol=1e-3)  # not a plain multiply of g
+
+
+def test_chain_reduced_sq_is_sq_minus_one_and_vs_float64(small_random):
+    """Reduced S(Q) is the S(Q) sum without the +1, scaled without the shift
+    (StructureFactorConstraints.py:1255-1261); the transform against a float64
+    evaluation of sum_r G(r) dr sin(Qr)/Q from the same histograms."""
+    s = small_random
+    mn, mx, bn, centers, H, vols = chain.sfc_grid(0.0, 8.0, 0.05)
+    hi, he = refcpu.full_pairs_histograms_coords(s["box"], s["basis"], True, s["molecules"], s["elem"], 2, mn, mx,
+                                                 bn, H)
+    counts = {"O": 400, "Si": 200}
+    w = chain.normalized_weighting(counts, {"O": 8, "Si": 14})
+    V = systems.volume(s["basis"])
+    rho0 = F32(600) / V
+    q = np.linspace(0.5, 20, 90).astype(F32)
+    M = chain.sq_matrix(centers, q)
+    args = (hi, he, ["O", "Si"], counts, w, V, rho0, vols, centers, M)
+    S = chain.total_sq(*args)
+    R = chain.total_sq(*args, reduced=True)
+    assert (R + 1).astype(F32).tobytes() == S.tobytes()
+    Rs = chain.total_sq(*args, scaleFactor=0.8, reduced=True)
+    assert Rs.tobytes() == (F32(0.8) * R).astype(F32).tobytes()
+    Ss = chain.total_sq(*args, scaleFactor=0.8)
+    assert np.max(np.abs((Ss - 1) - Rs)) < 1e-5 * (1 + np.max(np.abs(Rs)))
+    # float64: G(r) from the histograms, then the sine transform
+    g = np.zeros(H)
+    for a, b in [("O", "O"), ("O", "Si"), ("Si", "Si")]:
+        i, j = ["O", "Si"].index(a), ["O", "Si"].index(b)
+        if i == j:
+            nij = hi[i, i] + he[i, i]
+            D = counts[a] * (counts[a] - 1) / 2 / float(V)
+        else:
+            nij = hi[i, j] + hi[j, i] + he[i, j] + he[j, i]
+            D = counts[a] * counts[b] / float(V)
+        g += float(w[a + "-" + b]) * nij.astype(np.float64) / D
+    e = np.arange(H + 1) * 0.05
+    r = (e[1:] + e[:-1]) / 2
+    G64 = 4 * np.pi * r * (600 / float(V)) * (g / (4 / 3 * np.pi * (e[1:] ** 3 - e[:-1] ** 3)) - 1)
+    q64 = q.astype(np.float64)
+    R64 = (G64[:, None] * 0.05 * np.sin(np.outer(r, q64)) / q64).sum(0)
+    assert np.max(np.abs(R - R64)) < 2e-4 * (1 + np.max(np.abs(R64)))
+
+
 def test_chi_square_is_pairwise_and_close_to_fsum():
     rng = np.random.default_rng(1)
     e = rng.random(1000).astype(F32)
