@@ -439,6 +439,68 @@ def sio2_section(args):
     return out
 
 
+def sq_transform_section(reps=50):
+    """G(r) -> S(Q) for k proposals at once: the exact float32 sequential
+    transform (bit-identical to the constraint) and the matrix-core
+    contraction, with their error against a float64 transform; kernel_us is
+    the median device time of the kernels (HIP events around them on the
+    call stream), us_per_call the mean of back-to-back calls through the
+    C ABI; the reference's NumPy expression per curve on the host beside
+    them."""
+    import torch
+    from fullrmc_amd import _device as D
+    from fullrmc_amd import _lib
+    L = _lib.lib()
+    out = {}
+    for k, H, nQ, label in [(64, 1000, 500, "k64_H1000_Q500"), (128, 2000, 1000, "k128_H2000_Q1000")]:
+        rng = np.random.default_rng(H)
+        r = (0.005 + 0.01 * np.arange(H)).astype(np.float32)
+        q = np.linspace(0.3, 30, nQ).astype(np.float32)
+        Rs = r.reshape((-1, 1)) * (np.ones((H, 1), dtype=np.float32) * q)
+        M = (r[1] - r[0]) * (np.sin(Rs) / q)
+        G = (rng.standard_normal((k, H)) * np.exp(-r / 6)).astype(np.float32)
+        d_g, d_m = D.upload(G), D.upload(M)
+        d_s = D.empty((k, nQ), np.float32)
+        S64 = G.astype(np.float64) @ M.astype(np.float64) + 1
+        sec = {}
+        for mode, mname in [(0, "exact"), (1, "mfma")]:
+            def call():
+                _lib.check(L.frmc_sq_transform(d_g.data_ptr(), k, H, d_m.data_ptr(), nQ, 1, mode, d_s.data_ptr(),
+                                               D.stream()))
+            call()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                call()
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / reps
+            S = D.download(d_s)
+            # device time of the transform kernels alone (HIP events on the call stream)
+            L.frmc_set_timing(1)
+            kus = []
+            for _ in range(11):
+                call()
+                kus.append(1e3 * L.frmc_last_kernel_ms())
+            L.frmc_set_timing(0)
+            kus = sorted(kus)[5]
+            tf = 2.0 * k * H * nQ / (kus * 1e-6) / 1e12
+            sec[mname] = {"kernel_us": round(kus, 2), "us_per_call": round(us, 2), "tflops": round(tf, 3),
+                          "max_abs_err_over_max_abs_S_vs_fp64": float("%.3g" % (np.max(np.abs(S - S64)) /
+                                                                                 np.max(np.abs(S64))))}
+        sec["mfma"]["roofline"] = {"bound": "mfma", "achieved": sec["mfma"]["tflops"], "peak": PEAK_FP32_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": round(sec["mfma"]["tflops"] / PEAK_FP32_TFLOPS, 4)}
+        t0 = time.perf_counter()
+        ncpu = 4
+        for p in range(ncpu):
+            np.sum(G[p].reshape((-1, 1)) * M, axis=0) + 1
+        sec["cpu_numpy_us_per_curve"] = round((time.perf_counter() - t0) * 1e6 / ncpu, 1)
+        sec["gpu_exact_us_per_curve"] = round(sec["exact"]["kernel_us"] / k, 3)
+        out[label] = sec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -557,6 +619,7 @@ def main():
         t2 = time.perf_counter()
         E2.run(30)
         result["dropin_per_move_loop_moves_per_s"] = round(30 / (time.perf_counter() - t2), 1)
+        result["sq_transform"] = sq_transform_section()
         print("bench: extras done", file=sys.stderr, flush=True)
 
     if args.sio2:

@@ -93,6 +93,11 @@ _PROTOS = {
     "frmc_get_reciprocal_basis": [c_vp, c_vp, ctypes.POINTER(ctypes.c_double), c_vp],
     "frmc_transform_coordinates": [c_vp, c_vp, c_i32, c_vp, c_vp],
     "frmc_box_coordinates_real_distances": [c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp],
+    # G(r) -> S(Q) transform, reciprocal_space
+    "frmc_sq_transform": [c_vp, c_i32, c_i32, c_vp, c_i32, c_int, c_int, c_vp, c_vp],
+    "frmc_Gr_to_sq": [c_vp, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp],
+    "frmc_gr_to_sq": [c_vp, c_vp, c_i32, c_vp, c_i32, c_f32, c_vp, c_vp],
+    "frmc_sq_to_Gr": [c_vp, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp],
     # engine
     "frmc_engine_create": [ctypes.POINTER(c_vp), ctypes.POINTER(EngineDesc)],
     "frmc_engine_destroy": [c_vp],

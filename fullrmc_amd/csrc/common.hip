@@ -1,5 +1,6 @@
 // common.hip -- error state, exact squared-norm thresholds, and the
 // boundary_conditions_collection entry points.
+#include <atomic>
 #include <math.h>
 #include <stdarg.h>
 
@@ -46,6 +47,22 @@ float sq_threshold(float t) {
     float r;
     memcpy(&r, &lo, 4);
     return r;
+}
+
+hipError_t pool_malloc(void **p, size_t bytes, hipStream_t st) {
+    static std::atomic<uint64_t> retained{0};  // bit d: device d's pool configured
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 64 && !(retained.load() & (1ull << dev))) {
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        retained.fetch_or(1ull << dev);
+    }
+    return hipMallocAsync(p, bytes, st);
 }
 
 Grid make_grid(float minD, float maxD, float bin, int32_t H) {
@@ -130,7 +147,7 @@ int frmc_get_reciprocal_basis(const float *basis, float *rbasis, double *vol, vo
     FRMC_REQUIRE(basis && rbasis && vol, "NULL pointer argument");
     hipStream_t st = as_stream(stream);
     double *dvol = nullptr;
-    FRMC_CHECK_HIP(hipMallocAsync((void **)&dvol, sizeof(double), st));
+    FRMC_CHECK_HIP(pool_malloc((void **)&dvol, sizeof(double), st));
     k_reciprocal_basis<<<1, 64, 0, st>>>(basis, rbasis, dvol);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(vol, dvol, sizeof(double), hipMemcpyDeviceToHost, st);

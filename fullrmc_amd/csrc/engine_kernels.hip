@@ -29,16 +29,6 @@
 
 namespace frmc {
 
-// Uniform (wave-invariant) loads of data that an earlier kernel or copy of
-// the window wrote go through coherent vector loads: a plain load from a
-// uniform address is compiled to a scalar load through the scalar data cache,
-// which has returned stale values here between back-to-back kernels of a
-// stream.  Lane-varying loads use the vector path and need no such care.
-template <typename T>
-__device__ __forceinline__ T load_dev(const T *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ int32_t pair_type(int32_t a, int32_t b, int32_t nel) {
     int32_t lo = a < b ? a : b, hi = a < b ? b : a;
     return lo * (2 * nel - lo + 1) / 2 + (hi - lo);
@@ -382,9 +372,8 @@ __global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
         if (tid == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
         return;
     }
-    // The S(Q) sum starts from +0 rather than the first product: the two
-    // differ only in the sign of a zero partial sum, which no later addend,
-    // not the final +1 and (reduced S(Q)) not the squared deviation can see.
+    // The S(Q) sum starts from +0, as NumPy's add reduction does (a sum of
+    // -0 products is +0).
     float acc = 0.0f;
     // two register batches (matrix column and G rows) in turn: one batch loads
     // while the other's chain runs, which hides both the HBM/L2 and the LDS latency

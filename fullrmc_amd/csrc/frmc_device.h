@@ -10,6 +10,16 @@
 
 namespace frmc {
 
+// Uniform (wave-invariant) loads of data that an earlier kernel or copy
+// wrote go through coherent vector loads: a plain load from a uniform address
+// is compiled to a scalar load through the scalar data cache, which has
+// returned stale values here between back-to-back kernels of a stream.
+// Lane-varying loads use the vector path and need no such care.
+template <typename T>
+__device__ __forceinline__ T load_dev(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // pairs_distances.pyx:31-32: x>0 ? floor(x+0.5) : ceil(x-0.5).  floor/ceil of
 // a float are exact, so the float versions equal libm's double path.
 __device__ __forceinline__ float ref_round(float x) {

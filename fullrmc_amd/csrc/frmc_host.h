@@ -46,6 +46,11 @@ inline unsigned blocks_for(int64_t n, int threads) {
 // norm threshold of the reference's distance comparison against t.
 float sq_threshold(float t);
 
+// Stream-ordered scratch allocation (hipMallocAsync from the device's default
+// pool).  The pool keeps freed memory for reuse instead of returning it to
+// the driver at each synchronisation, so per-call scratch costs no mapping.
+hipError_t pool_malloc(void **p, size_t bytes, hipStream_t st);
+
 // Basis / grid helpers (copy a device 3x3 to the host once per call).
 struct Basis;
 struct Grid;

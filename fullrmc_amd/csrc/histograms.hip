@@ -361,7 +361,7 @@ struct Scratch {
 
 static int alloc_counts(Scratch &sc, int64_t size, hipStream_t st) {
     sc.st = st;
-    hipError_t e = hipMallocAsync((void **)&sc.p, sizeof(int32_t) * (2 * size + 1), st);
+    hipError_t e = pool_malloc((void **)&sc.p, sizeof(int32_t) * (2 * size + 1), st);
     if (e != hipSuccess) {
         set_error("hipMallocAsync of %lld histogram counts failed: %s", (long long)(2 * size + 1),
                   hipGetErrorString(e));
@@ -478,7 +478,7 @@ static int launch_full_t(const float *coords, int32_t n, const Basis &B, int isP
     const int64_t blocks = (S.items + S.per_block - 1) / S.per_block;
     int32_t *partial = nullptr;
     if (lds) {
-        hipError_t e = hipMallocAsync((void **)&partial, lds_bytes * blocks, st);
+        hipError_t e = pool_malloc((void **)&partial, lds_bytes * blocks, st);
         if (e != hipSuccess) {
             set_error("hipMallocAsync of %lld bytes of partial histograms failed: %s",
                       (long long)(lds_bytes * blocks), hipGetErrorString(e));
@@ -668,7 +668,7 @@ int frmc_full_pairs_histograms_dists(const float *distances, int32_t n, int32_t 
         FRMC_REQUIRE(distances && mol && elem, "NULL pointer argument");
         // column c is atom c: use an identity index list through atomIndex per column
         int32_t *idx = nullptr;
-        hipError_t e = hipMallocAsync((void **)&idx, sizeof(int32_t) * k, st);
+        hipError_t e = pool_malloc((void **)&idx, sizeof(int32_t) * k, st);
         if (e != hipSuccess) {
             set_error("hipMallocAsync failed: %s", hipGetErrorString(e));
             return FRMC_ENOMEM;
@@ -730,7 +730,7 @@ int frmc_full_pairs_counts_f64(const float *coords, int32_t n, const float *basi
     float amax = 0.0f;
     if (n > 0) {
         uint32_t *d_m = nullptr;
-        hipError_t e = hipMallocAsync((void **)&d_m, sizeof(uint32_t), st);
+        hipError_t e = pool_malloc((void **)&d_m, sizeof(uint32_t), st);
         if (e != hipSuccess) {
             set_error("hipMallocAsync failed: %s", hipGetErrorString(e));
             return FRMC_ENOMEM;

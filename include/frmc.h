@@ -194,6 +194,42 @@ int frmc_box_coordinates_real_distances(int32_t atomIndex, const int32_t *indexe
                                         void *stream);
 
 /* ------------------------------------------------------------------------
+ * G(r) -> S(Q) sine transform and reciprocal_space
+ * (Extensions/reciprocal_space.pyx)
+ * ---------------------------------------------------------------------- */
+
+#define FRMC_SQ_EXACT 0  /* float32 products summed sequentially over r: bit-identical */
+#define FRMC_SQ_MFMA 1   /* matrix-core contraction, float32 accumulation (not bit-identical) */
+
+/* The transform of the S(Q) constraints for k curves at once
+   (_get_Sq_from_Gr, StructureFactorConstraints.py:772-773 with addOne=1,
+   :1255-1256 with addOne=0): Sq[p][q] = sum_r Gr[p][r]*sqMatrix[r][q]
+   (+1).  Gr (k,H), sqMatrix (H,nQ) as the constraint builds it
+   (:302-312), Sq (k,nQ).  FRMC_SQ_EXACT reproduces np.sum(..., axis=0)
+   bit for bit; FRMC_SQ_MFMA runs on the matrix cores. */
+int frmc_sq_transform(const float *Gr, int32_t k, int32_t H, const float *sqMatrix, int32_t nQ, int addOne,
+                      int mode, float *Sq, void *stream);
+
+/* Gr_to_sq (reciprocal_space.pyx:89-122): sq (m,) = 1 + sum_r
+   dr*sin(q*r)/q*Gr[r], dr = distances[1]-distances[0], each term and each
+   accumulation in double, rounded to float32 after each accumulation as
+   the Cython loop does.  n >= 2. */
+int frmc_Gr_to_sq(const float *distances, const float *Gr, int32_t n, const float *qrange, int32_t m, float *sq,
+                  void *stream);
+
+/* gr_to_sq (reciprocal_space.pyx:42-80): sq = 1 + sum_r
+   4*PI*rho*dr*r*sin(q*r)/q*(gr[r]-1), same arithmetic rules.  n >= 2. */
+int frmc_gr_to_sq(const float *distances, const float *gr, int32_t n, const float *qrange, int32_t m, float rho,
+                  float *sq, void *stream);
+
+/* sq_to_Gr (reciprocal_space.pyx:131-162): Gr[r] = (2/PI) * sum_q
+   dq*Q*(S(Q)-1)*sin(Q*r), dq = qValues[1]-qValues[0], in double, rounded
+   once.  The reference's loop passes an array to the C sin and raises
+   TypeError for every input; this is the transform it documents.  m >= 2. */
+int frmc_sq_to_Gr(const float *qValues, const float *rValues, int32_t nr, const float *sq, int32_t m, float *Gr,
+                  void *stream);
+
+/* ------------------------------------------------------------------------
  * Device-resident engine: the Engine.run step (Engine.py:3166-3344) with the
  * constraint chain of PairDistributionConstraint / StructureFactorConstraint
  * (PairDistributionConstraints.py:847-895, 1044-1166;
