@@ -93,6 +93,14 @@ _PROTOS = {
     "frmc_get_reciprocal_basis": [c_vp, c_vp, ctypes.POINTER(ctypes.c_double), c_vp],
     "frmc_transform_coordinates": [c_vp, c_vp, c_i32, c_vp, c_vp],
     "frmc_box_coordinates_real_distances": [c_i32, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp],
+    # atomic_distances
+    "frmc_single_atomic_distances_dists": [c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32,
+                                           c_vp, c_vp, c_int, c_int, c_vp],
+    "frmc_multiple_atomic_distances_coords": [c_vp, c_i32, c_vp, c_i32, c_vp, c_int, c_vp, c_vp, c_i32, c_vp, c_vp,
+                                              c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "frmc_multiple_atomic_distances_dists": [c_vp, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp,
+                                             c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "frmc_pair_elements_stats": [c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp],
     # G(r) -> S(Q) transform, reciprocal_space
     "frmc_sq_transform": [c_vp, c_i32, c_i32, c_vp, c_i32, c_int, c_int, c_vp, c_vp],
     "frmc_Gr_to_sq": [c_vp, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp],

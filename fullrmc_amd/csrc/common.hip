@@ -65,6 +65,14 @@ hipError_t pool_malloc(void **p, size_t bytes, hipStream_t st) {
     return hipMallocAsync(p, bytes, st);
 }
 
+int load_basis(const float *dbasis, Basis &B, bool &ortho, hipStream_t st) {
+    FRMC_CHECK_HIP(hipMemcpyAsync(B.b, dbasis, sizeof(B.b), hipMemcpyDeviceToHost, st));
+    FRMC_CHECK_HIP(hipStreamSynchronize(st));
+    const float *b = B.b;
+    ortho = b[1] == 0.f && b[2] == 0.f && b[3] == 0.f && b[5] == 0.f && b[6] == 0.f && b[7] == 0.f;
+    return FRMC_OK;
+}
+
 Grid make_grid(float minD, float maxD, float bin, int32_t H) {
     Grid g;
     g.minD = minD;

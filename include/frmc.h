@@ -194,6 +194,59 @@ int frmc_box_coordinates_real_distances(int32_t atomIndex, const int32_t *indexe
                                         void *stream);
 
 /* ------------------------------------------------------------------------
+ * atomic_distances  (Extensions/atomic_distances.pyx)
+ * ---------------------------------------------------------------------- */
+
+/* flags of the atomic_distances entries (the reference's bint arguments) */
+#define FRMC_AD_INTER 1       /* interMolecular */
+#define FRMC_AD_INTRA 2       /* intraMolecular */
+#define FRMC_AD_WITHIN 4      /* countWithinLimits */
+#define FRMC_AD_RED_UPPER 8   /* reduceDistanceToUpper */
+#define FRMC_AD_RED_LOWER 16  /* reduceDistanceToLower */
+#define FRMC_AD_RED 32        /* reduceDistance */
+
+/* single_atomic_distances_dists (atomic_distances.pyx:241-318, the loop of
+   :42-118).  For i in [start,n) (start = 0 if allAtoms else atomIndex),
+   i != atomIndex, pairs passing the molecule filter and the limits
+   lowerLimit/upperLimit[elem[i]][elem[atomIndex]] add their (reduced)
+   distance to d{intra,inter}[elem[atomIndex]][elem[i]] and 1 to
+   n{intra,inter}[...] IN PLACE.  Float sums are added in the reference's
+   order (i ascending), so they are bit-identical.  lower/upper/d/n arrays
+   are (nel,nel) (the reference's (nel,nel,1)).  Synchronises once. */
+int frmc_single_atomic_distances_dists(int32_t atomIndex, const float *distances, int32_t n,
+                                       const int32_t *moleculeIndex, const int32_t *elementIndex, float *dintra,
+                                       float *dinter, int32_t *nintra, int32_t *ninter, int32_t numberOfElements,
+                                       const float *lowerLimit, const float *upperLimit, int flags, int allAtoms,
+                                       void *stream);
+
+/* multiple_atomic_distances_coords (atomic_distances.pyx:326-410): the same
+   for every atom of indexes in order, distances from pairs_distances on
+   the coordinates; full_atomic_distances_coords (:500-560) is this with
+   indexes = arange(n) and allAtoms = 0.  ADDS into the four outputs (the
+   reference returns them from zeros). */
+int frmc_multiple_atomic_distances_coords(const int32_t *indexes, int32_t nidx, const float *boxCoords, int32_t n,
+                                          const float *basis, int isPBC, const int32_t *moleculeIndex,
+                                          const int32_t *elementIndex, int32_t numberOfElements,
+                                          const float *lowerLimit, const float *upperLimit, int flags,
+                                          int allAtoms, int32_t *nintra, float *dintra, int32_t *ninter,
+                                          float *dinter, void *stream);
+
+/* multiple_atomic_distances_dists (atomic_distances.pyx:419-491): distances
+   (n,k), column c for atom indexes[c], c < nidx <= k;
+   full_atomic_distances_dists (:569-623) is this with indexes = arange(k),
+   allAtoms = 0.  ADDS into the four outputs. */
+int frmc_multiple_atomic_distances_dists(const int32_t *indexes, int32_t nidx, const float *distances, int32_t n,
+                                         int32_t k, const int32_t *moleculeIndex, const int32_t *elementIndex,
+                                         int32_t numberOfElements, const float *lowerLimit,
+                                         const float *upperLimit, int flags, int allAtoms, int32_t *nintra,
+                                         float *dintra, int32_t *ninter, float *dinter, void *stream);
+
+/* pair_elements_stats (atomic_distances.pyx:632-675): for every pair i<j,
+   nintra/ninter[elem[i]][elem[j]] += 1 by molecule.  Overwrites. */
+int frmc_pair_elements_stats(const int32_t *elementIndex, const int32_t *moleculeIndex, int32_t n,
+                             int32_t numberOfElements, int32_t *nintra, int32_t *ninter, void *stream);
+
+/* ------------------------------------------------------------------------
  * G(r) -> S(Q) sine transform and reciprocal_space
  * (Extensions/reciprocal_space.pyx)
  * ---------------------------------------------------------------------- */

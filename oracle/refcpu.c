@@ -19,6 +19,10 @@
  *     basis[0,1] typo at :59)
  *   transform_coordinates            Extensions/boundary_conditions_collection.pyx:88-110
  *   box_coordinates_real_distances   Extensions/boundary_conditions_collection.pyx:119-165
+ *   _single_atomic_distances_dists   Extensions/atomic_distances.pyx:42-118
+ *   multiple_atomic_distances_coords Extensions/atomic_distances.pyx:326-410
+ *   multiple_atomic_distances_dists  Extensions/atomic_distances.pyx:419-491
+ *   _pair_elements_stats             Extensions/atomic_distances.pyx:632-656
  *
  * Float semantics: every expression is evaluated left to right in IEEE
  * binary32 with round-to-nearest-even, no fused multiply-add (the recipe
@@ -364,6 +368,102 @@ int64_t ref_full_pairs_counts_f64(const float *coords, int32_t n, const float *B
    it is kept.  vol is a C double holding the float32 triple product (Cython
    infers double for the untyped float expression); the normalising divisions
    run in double and round to float32 on store. */
+/* --------------------------------------------------------- atomic_distances */
+
+#define AD_INTER 1
+#define AD_INTRA 2
+#define AD_WITHIN 4
+#define AD_RED_UPPER 8
+#define AD_RED_LOWER 16
+#define AD_RED 32
+
+/* _single_atomic_distances_dists (atomic_distances.pyx:42-118) with
+   ncores = 1: i ascending, float32 sums in that order.  Arrays (nel,nel). */
+void ref_single_atomic_distances_dists(int32_t atomIndex, const float *distances, int32_t n, const int32_t *mol,
+                                       const int32_t *elem, int32_t nel, float *dintra, float *dinter,
+                                       int32_t *nintra, int32_t *ninter, const float *lowerLimit,
+                                       const float *upperLimit, int flags, int allAtoms) {
+    int32_t atomMol = mol[atomIndex], atomEl = elem[atomIndex];
+    int32_t start = allAtoms ? 0 : atomIndex;
+    for (int32_t i = start; i < n; i++) {
+        if (i == atomIndex) continue;
+        int32_t molI = mol[i];
+        if (!(flags & AD_INTRA) && molI == atomMol) continue;
+        if (!(flags & AD_INTER) && !(molI == atomMol)) continue;
+        float distance = distances[i];
+        int32_t elI = elem[i];
+        float lower = lowerLimit[elI * nel + atomEl];
+        float upper = upperLimit[elI * nel + atomEl];
+        if (flags & AD_WITHIN) {
+            if (distance < lower) continue;
+            if (distance >= upper) continue;
+        } else if ((distance >= lower) && (distance < upper)) {
+            continue;
+        }
+        if (flags & AD_RED_UPPER) {
+            distance = (float)fabs((double)(upper - distance));
+        } else if (flags & AD_RED_LOWER) {
+            distance = (float)fabs((double)(lower - distance));
+        } else if (flags & AD_RED) {
+            if (distance > (lower + upper) / 2.0f)
+                distance = (float)fabs((double)(upper - distance));
+            else
+                distance = (float)fabs((double)(lower - distance));
+        }
+        if (molI == atomMol) {
+            dintra[atomEl * nel + elI] += distance;
+            nintra[atomEl * nel + elI] += 1;
+        } else {
+            dinter[atomEl * nel + elI] += distance;
+            ninter[atomEl * nel + elI] += 1;
+        }
+    }
+}
+
+/* multiple_atomic_distances_coords (atomic_distances.pyx:326-410): each atom
+   of indexes in order, distances by pairs_distances_to_indexcoords.  Adds
+   into the outputs. */
+void ref_multiple_atomic_distances_coords(const int32_t *indexes, int32_t nidx, const float *coords, int32_t n,
+                                          const float *B, int isPBC, const int32_t *mol, const int32_t *elem,
+                                          int32_t nel, const float *lowerLimit, const float *upperLimit, int flags,
+                                          int allAtoms, int32_t *nintra, float *dintra, int32_t *ninter,
+                                          float *dinter) {
+    float *d = (float *)malloc(sizeof(float) * (n > 0 ? n : 1));
+    for (int32_t c = 0; c < nidx; c++) {
+        ref_distances_to_indexcoords(indexes[c], coords, n, B, isPBC, allAtoms, d);
+        ref_single_atomic_distances_dists(indexes[c], d, n, mol, elem, nel, dintra, dinter, nintra, ninter,
+                                          lowerLimit, upperLimit, flags, allAtoms);
+    }
+    free(d);
+}
+
+/* multiple_atomic_distances_dists (atomic_distances.pyx:419-491): column c of
+   the (n,k) distances for atom indexes[c].  Adds into the outputs. */
+void ref_multiple_atomic_distances_dists(const int32_t *indexes, int32_t nidx, const float *distances, int32_t n,
+                                         int32_t k, const int32_t *mol, const int32_t *elem, int32_t nel,
+                                         const float *lowerLimit, const float *upperLimit, int flags, int allAtoms,
+                                         int32_t *nintra, float *dintra, int32_t *ninter, float *dinter) {
+    float *d = (float *)malloc(sizeof(float) * (n > 0 ? n : 1));
+    for (int32_t c = 0; c < nidx; c++) {
+        for (int32_t i = 0; i < n; i++) d[i] = distances[(int64_t)i * k + c];
+        ref_single_atomic_distances_dists(indexes[c], d, n, mol, elem, nel, dintra, dinter, nintra, ninter,
+                                          lowerLimit, upperLimit, flags, allAtoms);
+    }
+    free(d);
+}
+
+/* _pair_elements_stats (atomic_distances.pyx:632-656).  Adds. */
+void ref_pair_elements_stats(const int32_t *elem, const int32_t *mol, int32_t n, int32_t nel, int32_t *nintra,
+                             int32_t *ninter) {
+    for (int32_t i = 0; i < n - 1; i++)
+        for (int32_t j = i + 1; j < n; j++) {
+            if (mol[j] == mol[i])
+                nintra[elem[i] * nel + elem[j]] += 1;
+            else
+                ninter[elem[i] * nel + elem[j]] += 1;
+        }
+}
+
 double ref_get_reciprocal_basis(const float *b, float *r) {
     r[0] = b[4] * b[8] - b[7] * b[5];
     r[1] = -(b[1] * b[8] - b[7] * b[1]);

@@ -45,6 +45,13 @@ def lib():
             "ref_get_reciprocal_basis": ([_p, _p], ctypes.c_double),
             "ref_transform_coordinates": ([_p, _p, _i, _p], None),
             "ref_box_coordinates_real_distances": ([_i, _p, _i, _p, _p, _p], None),
+            "ref_single_atomic_distances_dists": ([_i, _p, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _int, _int],
+                                                  None),
+            "ref_multiple_atomic_distances_coords": ([_p, _i, _p, _i, _p, _int, _p, _p, _i, _p, _p, _int, _int,
+                                                      _p, _p, _p, _p], None),
+            "ref_multiple_atomic_distances_dists": ([_p, _i, _p, _i, _i, _p, _p, _i, _p, _p, _int, _int, _p, _p,
+                                                     _p, _p], None),
+            "ref_pair_elements_stats": ([_p, _p, _i, _i, _p, _p], None),
         }
         for name, (args, res) in protos.items():
             fn = getattr(L, name)
@@ -293,3 +300,84 @@ def box_coordinates_real_distances(atomIndex, indexes, boxCoords, basis):
     out = np.empty(idx.shape[0], np.float32)
     lib().ref_box_coordinates_real_distances(int(atomIndex), _ptr(idx), idx.shape[0], _ptr(c), _ptr(b), _ptr(out))
     return out
+
+
+# ------------------------------------------------------------ atomic_distances
+
+AD_INTER, AD_INTRA, AD_WITHIN, AD_RED_UPPER, AD_RED_LOWER, AD_RED = 1, 2, 4, 8, 16, 32
+
+
+def ad_flags(interMolecular=True, intraMolecular=True, countWithinLimits=True, reduceDistanceToUpper=False,
+             reduceDistanceToLower=False, reduceDistance=False):
+    f = 0
+    for on, bit in ((interMolecular, AD_INTER), (intraMolecular, AD_INTRA), (countWithinLimits, AD_WITHIN),
+                    (reduceDistanceToUpper, AD_RED_UPPER), (reduceDistanceToLower, AD_RED_LOWER),
+                    (reduceDistance, AD_RED)):
+        if on:
+            f |= bit
+    return f
+
+
+def single_atomic_distances_dists(atomIndex, distances, moleculeIndex, elementIndex, dintra, dinter, nintra,
+                                  ninter, lowerLimit, upperLimit, flags, allAtoms=True):
+    """atomic_distances.pyx:42-118 (in place on the (nel,nel,1) arrays)"""
+    d = _c(distances, np.float32)
+    m, e = _c(moleculeIndex, np.int32), _c(elementIndex, np.int32)
+    for a, t in ((dintra, np.float32), (dinter, np.float32), (nintra, np.int32), (ninter, np.int32)):
+        assert a.flags.c_contiguous and a.dtype == t
+    nel = dintra.shape[0]
+    lo, up = _c(lowerLimit, np.float32), _c(upperLimit, np.float32)
+    lib().ref_single_atomic_distances_dists(int(atomIndex), _ptr(d), d.shape[0], _ptr(m), _ptr(e), nel,
+                                            _ptr(dintra), _ptr(dinter), _ptr(nintra), _ptr(ninter), _ptr(lo),
+                                            _ptr(up), int(flags), int(bool(allAtoms)))
+
+
+def multiple_atomic_distances_coords(indexes, boxCoords, basis, isPBC, moleculeIndex, elementIndex,
+                                     numberOfElements, lowerLimit, upperLimit, flags, allAtoms=True):
+    """atomic_distances.pyx:326-410 -> (nintra, dintra, ninter, dinter)"""
+    idx = _c(indexes, np.int32)
+    c, b = _c(boxCoords, np.float32), _c(basis, np.float32)
+    m, e = _c(moleculeIndex, np.int32), _c(elementIndex, np.int32)
+    nel = int(numberOfElements)
+    lo, up = _c(lowerLimit, np.float32), _c(upperLimit, np.float32)
+    ni, ne = np.zeros((nel, nel, 1), np.int32), np.zeros((nel, nel, 1), np.int32)
+    di, de = np.zeros((nel, nel, 1), np.float32), np.zeros((nel, nel, 1), np.float32)
+    lib().ref_multiple_atomic_distances_coords(_ptr(idx), idx.shape[0], _ptr(c), c.shape[0], _ptr(b),
+                                               int(bool(isPBC)), _ptr(m), _ptr(e), nel, _ptr(lo), _ptr(up),
+                                               int(flags), int(bool(allAtoms)), _ptr(ni), _ptr(di), _ptr(ne),
+                                               _ptr(de))
+    return ni, di, ne, de
+
+
+def full_atomic_distances_coords(boxCoords, basis, isPBC, moleculeIndex, elementIndex, numberOfElements,
+                                 lowerLimit, upperLimit, flags):
+    """atomic_distances.pyx:500-560"""
+    n = len(boxCoords)
+    return multiple_atomic_distances_coords(np.arange(n, dtype=np.int32), boxCoords, basis, isPBC, moleculeIndex,
+                                            elementIndex, numberOfElements, lowerLimit, upperLimit, flags,
+                                            allAtoms=False)
+
+
+def multiple_atomic_distances_dists(indexes, distances, moleculeIndex, elementIndex, numberOfElements, lowerLimit,
+                                    upperLimit, flags, allAtoms=True):
+    """atomic_distances.pyx:419-491"""
+    idx = _c(indexes, np.int32)
+    d = _c(distances, np.float32)
+    m, e = _c(moleculeIndex, np.int32), _c(elementIndex, np.int32)
+    nel = int(numberOfElements)
+    lo, up = _c(lowerLimit, np.float32), _c(upperLimit, np.float32)
+    ni, ne = np.zeros((nel, nel, 1), np.int32), np.zeros((nel, nel, 1), np.int32)
+    di, de = np.zeros((nel, nel, 1), np.float32), np.zeros((nel, nel, 1), np.float32)
+    lib().ref_multiple_atomic_distances_dists(_ptr(idx), idx.shape[0], _ptr(d), d.shape[0], d.shape[1], _ptr(m),
+                                              _ptr(e), nel, _ptr(lo), _ptr(up), int(flags), int(bool(allAtoms)),
+                                              _ptr(ni), _ptr(di), _ptr(ne), _ptr(de))
+    return ni, di, ne, de
+
+
+def pair_elements_stats(elementIndex, moleculeIndex, numberOfElements):
+    """atomic_distances.pyx:632-675 -> (nintra, ninter)"""
+    e, m = _c(elementIndex, np.int32), _c(moleculeIndex, np.int32)
+    nel = int(numberOfElements)
+    ni, ne = np.zeros((nel, nel, 1), np.int32), np.zeros((nel, nel, 1), np.int32)
+    lib().ref_pair_elements_stats(_ptr(e), _ptr(m), e.shape[0], nel, _ptr(ni), _ptr(ne))
+    return ni, ne
