@@ -37,6 +37,7 @@ class Engine:
         self.groups = []
         self.generators = []
         self.constraints = []
+        self.rigidConstraints = []
         self.selector = RandomSelector()
         self.tolerance = FLOAT_TYPE(0)
         self.generated = self.tried = self.accepted = self.tolerated = 0
@@ -47,9 +48,10 @@ class Engine:
 
     # ------------------------------------------------------------ system
     def set_system(self, realCoordinates, allElements, moleculesIndex=None, basisVectors=None,
-                   reciprocalBasisVectors=None):
+                   reciprocalBasisVectors=None, allNames=None):
         """Coordinates (n,3), element name per atom, molecule index per atom
-        and periodic cell vectors as rows (number = cube edge).  Replaces
+        and periodic cell vectors as rows (number = cube edge); atom names
+        (default: the elements) for constraints typed by name.  Replaces
         Engine.set_pdb + set_boundary_conditions (Engine.py:2134-2298)."""
         real = np.array(realCoordinates, dtype=FLOAT_TYPE)
         if real.ndim != 2 or real.shape[1] != 3:
@@ -78,6 +80,15 @@ class Engine:
             self.numberOfAtomsPerElement[e] = self.numberOfAtomsPerElement.get(e, 0) + 1
         self.moleculesIndex = (np.arange(n, dtype=INT_TYPE) if moleculesIndex is None
                                else np.array(moleculesIndex, dtype=INT_TYPE))
+        self.allNames = list(self.allElements if allNames is None else allNames)
+        if len(self.allNames) != n:
+            raise ValueError("one name per atom is required")
+        self.names = sorted(set(self.allNames))
+        nlut = {e: i for i, e in enumerate(self.names)}
+        self.namesIndex = np.array([nlut[e] for e in self.allNames], dtype=INT_TYPE)
+        self.numberOfAtomsPerName = {}
+        for e in self.allNames:
+            self.numberOfAtomsPerName[e] = self.numberOfAtomsPerName.get(e, 0) + 1
         self.numberOfAtoms = n
         self.numberDensity = FLOAT_TYPE(n) / FLOAT_TYPE(self.volume)
         self.set_groups_as_atoms()
@@ -110,9 +121,12 @@ class Engine:
         self.tolerance = FLOAT_TYPE(t / 100.)
 
     def add_constraints(self, constraints):
+        """Experimental constraints enter the total standard error; rigid
+        ones (distance constraints) only veto moves before the experimental
+        constraints are evaluated (Engine.py:3040-3082, 3278-3296)."""
         for c in constraints if isinstance(constraints, (list, tuple)) else [constraints]:
             c._attach(self)
-            self.constraints.append(c)
+            (self.rigidConstraints if getattr(c, "rigid", False) else self.constraints).append(c)
         self._drop_device()
 
     # ----------------------------------------------------- standard error
@@ -121,7 +135,7 @@ class Engine:
         return np.sum([getattr(c, current) / c.varianceSquared for c in constraints])
 
     def initialize_used_constraints(self, force=False):
-        for c in self.constraints:
+        for c in self.constraints + self.rigidConstraints:
             if c.data is None or force:
                 c.compute_data()
         self.totalStandardError = self.compute_total_standard_error(self.constraints)
@@ -138,26 +152,39 @@ class Engine:
             before = np.array(self.realCoordinates[rel], dtype=FLOAT_TYPE)
             moved_real = self.generators[gi].move(before)
             moved_box = transform_coordinates(self.reciprocalBasisVectors, moved_real)
-            self.tried += 1
-            for c in self.constraints:
+            # rigid constraints first: any of them can reject the move untried
+            reject = False
+            for c in self.rigidConstraints:
                 c.compute_before_move(realIndexes=rel, relativeIndexes=rel)
                 c.compute_after_move(realIndexes=rel, relativeIndexes=rel, movedBoxCoordinates=moved_box)
-            new = self.compute_total_standard_error(self.constraints, current="afterMoveStandardError")
-            reject = False
-            if new > self.totalStandardError:
-                if random.random() > self.tolerance:
-                    reject = True
-                else:
-                    self.tolerated += 1
-                    self.totalStandardError = new
-            else:
-                self.totalStandardError = new
+                reject = c.should_step_get_rejected(c.afterMoveStandardError)
+                if reject:
+                    break
+            tried = not reject
             if reject:
-                for c in self.constraints:
+                for c in self.rigidConstraints:
                     c.reject_move(realIndexes=rel, relativeIndexes=rel)
             else:
-                self.accepted += 1
+                self.tried += 1
                 for c in self.constraints:
+                    c.compute_before_move(realIndexes=rel, relativeIndexes=rel)
+                    c.compute_after_move(realIndexes=rel, relativeIndexes=rel, movedBoxCoordinates=moved_box)
+                new = self.compute_total_standard_error(self.constraints, current="afterMoveStandardError")
+                if new > self.totalStandardError:
+                    if random.random() > self.tolerance:
+                        reject = True
+                    else:
+                        self.tolerated += 1
+                        self.totalStandardError = new
+                else:
+                    self.totalStandardError = new
+            if reject:
+                if tried:
+                    for c in self.constraints:
+                        c.reject_move(realIndexes=rel, relativeIndexes=rel)
+            else:
+                self.accepted += 1
+                for c in self.constraints + self.rigidConstraints:
                     c.accept_move(realIndexes=rel, relativeIndexes=rel)
                 self.realCoordinates[rel] = moved_real
                 self.boxCoordinates[rel] = moved_box
@@ -182,6 +209,9 @@ class Engine:
         """Create (or reuse) the device engine holding the current state."""
         from . import _device
         _device.require_gpu()
+        if self.rigidConstraints:
+            raise NotImplementedError("rigid (distance) constraints veto moves one at a time in Engine.run; "
+                                      "the device engine evaluates the experimental pair constraints only")
         if self._dev is not None and self._dev_kmax >= kmax and self._dev_synced:
             return self._dev
         self._pull_device()

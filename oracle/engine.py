@@ -14,6 +14,8 @@ constraint chain (chain.py), the accept rule, and accept/reject updates.
                       StructureFactorConstraints.py:975-1096
   PCF / reduced SFC   PairCorrelationConstraints.py:305-392 (same update,
                       g(r) model), StructureFactorConstraints.py:1237-1261
+  rigid constraints   Engine.py:3278-3296; InterMolecularDistanceConstraint
+                      DistanceConstraints.py:361-428, 552-772 (element types)
 """
 import random
 
@@ -199,17 +201,88 @@ class RSFC(SFC):
                               self.centers, self.M, self.sf, reduced=True)
 
 
+class IMD:
+    """InterMolecularDistanceConstraint on the oracle path, element types,
+    every pair limit = defaultDistance unless given in limits[(a, b)]."""
+
+    def __init__(self, sysm, defaultDistance=1.5, limits=None, flexible=True, rejectProbability=1.0,
+                 inter=True):
+        self.s = sysm
+        nel = sysm.nel
+        self.up = np.full((nel, nel, 1), F32(defaultDistance), F32)
+        for (a, b), d in (limits or {}).items():
+            i, j = sysm.elements.index(a), sysm.elements.index(b)
+            self.up[i, j, 0] = self.up[j, i, 0] = F32(d)
+        self.lo = np.zeros((nel, nel, 1), F32)
+        self.flags = refcpu.ad_flags(inter, not inter, True, True)
+        self.inter = inter
+        self.flexible = flexible
+        self.rp = F32(rejectProbability)
+        self.pairs = [(i, j) for i in range(nel) for j in range(i, nel)]
+
+    def _pick(self, r):
+        ni, di, ne, de = r
+        return (ne, de) if self.inter else (ni, di)
+
+    def _full(self, box, mol, elem):
+        return self._pick(refcpu.full_atomic_distances_coords(box, self.s.basis, True, mol, elem, self.s.nel,
+                                                              self.lo, self.up, self.flags))
+
+    def value(self, number, dsum):
+        idi = np.array([p[0] for p in self.pairs])
+        idj = np.array([p[1] for p in self.pairs])
+        numbers = (number[idi, idj] + number[idj, idi]).reshape(-1)
+        distances = (dsum[idi, idj] + dsum[idj, idi]).reshape(-1)
+        nz = np.where(numbers)
+        distances[nz] /= numbers[nz]
+        return F32(np.sum(distances))
+
+    def compute_data(self):
+        self.number, self.dsum = self._full(self.s.box, self.s.mol, self.s.elem)
+        self.sd = self.value(self.number, self.dsum)
+
+    def _group(self, idx, box):
+        nM, dM = self._pick(refcpu.multiple_atomic_distances_coords(idx, box, self.s.basis, True, self.s.mol,
+                                                                    self.s.elem, self.s.nel, self.lo, self.up,
+                                                                    self.flags, allAtoms=True))
+        nF, dF = self._full(box[idx], self.s.mol[idx], self.s.elem[idx])
+        return nM - nF, dM - dF
+
+    def before(self, idx):
+        self.bn, self.bd = self._group(idx, self.s.box)
+
+    def after(self, idx, moved_box):
+        saved = np.array(self.s.box[idx], dtype=F32)
+        self.s.box[idx] = moved_box
+        self.an, self.ad = self._group(idx, self.s.box)
+        self.s.box[idx] = saved
+        self.new_n = self.number - self.bn + self.an
+        self.new_d = self.dsum - self.bd + self.ad
+        self.after_sd = self.value(self.new_n, self.new_d)
+
+    def rejects(self):
+        if self.flexible:
+            if self.after_sd <= self.sd:
+                return False
+            return random.random() < self.rp
+        return bool(np.any(self.an > self.bn))
+
+    def accept(self):
+        self.number, self.dsum, self.sd = self.new_n, self.new_d, self.after_sd
+
+
 class Engine:
-    def __init__(self, sysm, groups, generator, constraints, tolerance=0.0):
+    def __init__(self, sysm, groups, generator, constraints, tolerance=0.0, rigid=()):
         self.s = sysm
         self.groups = [np.array(g, dtype=np.int32) for g in groups]
         self.generator = generator
         self.constraints = constraints
+        self.rigid = list(rigid)
         self.tolerance = F32(F32(tolerance) / 100.)
         self.generated = self.tried = self.accepted = self.tolerated = 0
 
     def initialize(self):
-        for c in self.constraints:
+        for c in self.constraints + self.rigid:
             c.compute_data()
         self.tse = chain.total_error([c.sd for c in self.constraints])
 
@@ -222,23 +295,30 @@ class Engine:
             before = np.array(s.real[idx], dtype=F32)
             moved_real = self.generator.move(before)
             moved_box = refcpu.transform_coordinates(s.recip, moved_real)
-            self.tried += 1
-            for c in self.constraints:
+            reject = False
+            for c in self.rigid:
                 c.before(idx)
                 c.after(idx, moved_box)
-            new = chain.total_error([c.after_sd for c in self.constraints])
-            reject = False
-            if new > self.tse:
-                if random.random() > self.tolerance:
-                    reject = True
+                reject = c.rejects()
+                if reject:
+                    break
+            if not reject:
+                self.tried += 1
+                for c in self.constraints:
+                    c.before(idx)
+                    c.after(idx, moved_box)
+                new = chain.total_error([c.after_sd for c in self.constraints])
+                if new > self.tse:
+                    if random.random() > self.tolerance:
+                        reject = True
+                    else:
+                        self.tolerated += 1
+                        self.tse = new
                 else:
-                    self.tolerated += 1
                     self.tse = new
-            else:
-                self.tse = new
             if not reject:
                 self.accepted += 1
-                for c in self.constraints:
+                for c in self.constraints + self.rigid:
                     c.accept()
                 s.real[idx] = moved_real
                 s.box[idx] = moved_box
