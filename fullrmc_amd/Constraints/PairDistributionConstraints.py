@@ -31,9 +31,10 @@ def elements_weight(elements, atomsWeight, weighting):
     for el in elements:
         w = (atomsWeight or {}).get(el)
         if w is None:
-            if weighting != "atomicNumber" or el not in ATOMIC_NUMBER:
+            key = el.capitalize()  # element symbols are matched without regard to case
+            if weighting != "atomicNumber" or key not in ATOMIC_NUMBER:
                 raise ValueError("no '%s' weight for element '%s'; pass atomsWeight" % (weighting, el))
-            w = ATOMIC_NUMBER[el]
+            w = ATOMIC_NUMBER[key]
         out[el] = FLOAT_TYPE(w)
     return out
 
@@ -64,7 +65,12 @@ class _HistogramConstraint:
             raise NotImplementedError("window and shape functions are not part of this path")
         if adjustScaleFactor is not None and adjustScaleFactor[0]:
             raise NotImplementedError("scale-factor fitting is not part of this path")
-        data = np.asarray(experimentalData)
+        # set_experimental_data (Core/Constraint.py:1240-1279): a path is read
+        # by np.loadtxt as float32; lists and arrays are cast
+        if isinstance(experimentalData, str):
+            data = np.loadtxt(experimentalData, dtype=FLOAT_TYPE)
+        else:
+            data = np.asarray(experimentalData)
         self.experimentalData = data.astype(FLOAT_TYPE)
         self._check_data(self.experimentalData)
         self.dataWeights = None if dataWeights is None else np.array(dataWeights, dtype=FLOAT_TYPE)

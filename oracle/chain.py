@@ -7,6 +7,8 @@ expressions, so the restatement keeps their operand order and dtypes.
 
   pdc_grid          PairDistributionConstraints.py:722, 748-758
   sfc_grid          StructureFactorConstraints.py:330-360
+  sfc_defaults      StructureFactorConstraints.py:502-575 (rmin, rmax, dr
+                    when not given)
   sq_matrix         StructureFactorConstraints.py:302-312
   total_gr          PairDistributionConstraints.py:847-895 (no shape function,
                     no window, constant scale factor)
@@ -51,6 +53,20 @@ def sfc_grid(rmin, rmax, dr):
     H = np.int32(len(edges) - 1)
     vols = F32(4.0 / 3.) * PI * ((edges[1:]) ** 3 - edges[0:-1] ** 3)
     return mn, mx, bin_, centers, H, vols
+
+
+def sfc_defaults(qvalues):
+    """rmin = 2*PI/qmax, rmax = 2*PI/dq, dr = 2*PI/qmax rounded down to a
+    multiple of 0.1, all in the float32 the reference computes them in."""
+    q = np.asarray(qvalues, F32)
+    qmax = q[-1]
+    rmin = F32(2. * PI / qmax)
+    rmax = F32(2. * PI / (q[1] - q[0]))
+    b = 2. * PI / qmax
+    rb = round(b, 1)
+    if rb > b:
+        rb -= 0.1
+    return rmin, rmax, F32(rb)
 
 
 def sq_matrix(centers, qvalues):
