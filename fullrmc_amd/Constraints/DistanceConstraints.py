@@ -62,11 +62,11 @@ class _MolecularDistanceConstraint:
         """set_type_definition (:195-263) and set_pairs_distance (:265-359)."""
         self.engine = engine
         if self.typeDefinition == "name":
-            self.types, self.typesIndex = list(engine.names), engine.namesIndex
-            self.numberOfAtomsPerType = engine.numberOfAtomsPerName
+            self.types, self.typesIndex = list(engine.names), np.array(engine.namesIndex)
+            self.numberOfAtomsPerType = dict(engine.numberOfAtomsPerName)
         else:
-            self.types, self.typesIndex = list(engine.elements), engine.elementsIndex
-            self.numberOfAtomsPerType = engine.numberOfAtomsPerElement
+            self.types, self.typesIndex = list(engine.elements), np.array(engine.elementsIndex)
+            self.numberOfAtomsPerType = dict(engine.numberOfAtomsPerElement)
         self.numberOfTypes = len(self.types)
         self.typePairs = sorted(itertools.combinations_with_replacement(self.types, 2))
         self.typePairsIndex = np.array([[self.types.index(a), self.types.index(b)] for a, b in self.typePairs],
@@ -214,6 +214,27 @@ class _MolecularDistanceConstraint:
         """:760-772"""
         self.activeAtomsDataBeforeMove = self.activeAtomsDataAfterMove = None
         self.afterMoveStandardError = None
+
+    # --- amputation (remove moves), :774-819 and :451-464
+    def compute_as_if_amputated(self, realIndex, relativeIndex):
+        pass
+
+    def accept_amputation(self, realIndex, relativeIndex):
+        """The removed atom's counted pairs leave the data."""
+        self.compute_before_move(realIndexes=realIndex, relativeIndexes=relativeIndex)
+        b = self.activeAtomsDataBeforeMove
+        self.data = {"number": self.data["number"] - b["number"],
+                     "distanceSum": self.data["distanceSum"] - b["distanceSum"]}
+        self.activeAtomsDataBeforeMove = None
+        self.standardError = self._compute_standard_error(self._get_constraint_value())
+
+    def reject_amputation(self, realIndex, relativeIndex):
+        pass
+
+    def _on_collector_collect_atom(self, realIndex, relativeIndex):
+        t = self.types[int(self.typesIndex[relativeIndex])]
+        self.typesIndex = np.delete(self.typesIndex, relativeIndex, axis=0)
+        self.numberOfAtomsPerType[t] -= 1
 
 
 class InterMolecularDistanceConstraint(_MolecularDistanceConstraint):

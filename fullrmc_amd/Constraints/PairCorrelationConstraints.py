@@ -15,15 +15,16 @@ class PairCorrelationConstraint(PairDistributionConstraint):
 
     kind = 2
 
-    def _model(self, data):
+    def _model(self, data, rho0=None, counts=None, weights=None):
         """__get_total_gr (PairCorrelationConstraints.py:126-169) without
         shape function, scale-factor fitting, multiframe prior or window:
         a scale factor acts on G(r) and is mapped back to g(r)."""
-        gr = self._gr_sum(data, cast_density=False)
+        rho0 = self.engine.numberDensity if rho0 is None else rho0
+        gr = self._gr_sum(data, cast_density=False, counts=counts, weights=weights)
         gr /= self.shellVolumes
         sf = FLOAT_TYPE(self.scaleFactor)
         if sf != 1:
-            Gr = (4. * PI * self.shellCenters * self.engine.numberDensity) * (gr - 1)
+            Gr = (4. * PI * self.shellCenters * rho0) * (gr - 1)
             Gr *= sf
-            gr = 1. + Gr / (4. * PI * self.shellCenters * self.engine.numberDensity)
+            gr = 1. + Gr / (4. * PI * self.shellCenters * rho0)
         return gr

@@ -132,14 +132,18 @@ class _HistogramConstraint:
                                               self.bin, self.histogramSize)
         return {"intra": iM - iF, "inter": eM - eF}
 
-    def _gr_sum(self, data, cast_density):
+    def _gr_sum(self, data, cast_density, counts=None, weights=None):
+        """sum over element pairs of w_ij n_ij / D_ij; counts and weights
+        default to the engine's atom counts and the weighting scheme."""
         e = self.engine
+        counts = e.numberOfAtomsPerElement if counts is None else counts
+        weights = self.weightingScheme if weights is None else weights
         Gr = np.zeros(self.histogramSize, dtype=FLOAT_TYPE)
         for a, b in self.elementsPairs:
-            wij = self.weightingScheme.get(a + "-" + b)
+            wij = weights.get(a + "-" + b)
             if wij is None:
-                wij = self.weightingScheme[b + "-" + a]
-            ni, nj = e.numberOfAtomsPerElement[a], e.numberOfAtomsPerElement[b]
+                wij = weights[b + "-" + a]
+            ni, nj = counts[a], counts[b]
             i, j = e.elements.index(a), e.elements.index(b)
             if i == j:
                 Nij = ni * (ni - 1) / 2.0
@@ -190,6 +194,39 @@ class _HistogramConstraint:
         self.activeAtomsDataBeforeMove = self.activeAtomsDataAfterMove = None
         self.afterMoveStandardError = None
 
+    # --- amputation (remove moves), PairDistributionConstraints.py:1168-1233,
+    #     StructureFactorConstraints.py:1098-1163, PairCorrelationConstraints.py:394-436
+    def compute_as_if_amputated(self, realIndex, relativeIndex):
+        """Data and standard error as if the atom were removed: its pairs
+        taken out, the atom counts, weighting scheme and number density of
+        the system without it."""
+        e = self.engine
+        self.compute_before_move(realIndexes=realIndex, relativeIndexes=relativeIndex)
+        b = self.activeAtomsDataBeforeMove
+        data = {"intra": self.data["intra"] - b["intra"], "inter": self.data["inter"] - b["inter"]}
+        el = e.allElements[relativeIndex[0]]
+        counts = dict(e.numberOfAtomsPerElement)
+        counts[el] -= 1
+        weights = normalized_weighting(counts, self._elementsWeight)
+        rho0 = FLOAT_TYPE((e.numberOfAtoms - 1) / e.volume)
+        self.amputationStandardError = self.compute_standard_error(self._model(data, rho0, counts, weights))
+        self.amputationData = {"data": data, "weightingScheme": weights}
+        self.activeAtomsDataBeforeMove = None
+
+    def accept_amputation(self, realIndex, relativeIndex):
+        self.data = self.amputationData["data"]
+        self.weightingScheme = self.amputationData["weightingScheme"]
+        self.standardError = self.amputationStandardError
+        self.amputationData = None
+        self.amputationStandardError = None
+
+    def reject_amputation(self, realIndex, relativeIndex):
+        self.amputationData = None
+        self.amputationStandardError = None
+
+    def _on_collector_collect_atom(self, realIndex, relativeIndex):
+        pass
+
     def compute_standard_error(self, modelData):
         diff = self._experimental - modelData
         if self._usedDataWeights is None:
@@ -229,10 +266,13 @@ class PairDistributionConstraint(_HistogramConstraint):
     def _prefactor(self):
         return 4. * PI * self.shellCenters * self.engine.numberDensity
 
-    def _model(self, data):
-        Gr = self._gr_sum(data, cast_density=True)
+    def _model(self, data, rho0=None, counts=None, weights=None):
+        """__get_total_Gr (PairDistributionConstraints.py:847-895) without
+        shape function, scale-factor fitting, multiframe prior or window."""
+        rho0 = self.engine.numberDensity if rho0 is None else rho0
+        Gr = self._gr_sum(data, cast_density=True, counts=counts, weights=weights)
         Gr /= self.shellVolumes
-        Gr = (4. * PI * self.shellCenters * self.engine.numberDensity) * (Gr - 1)
+        Gr = (4. * PI * self.shellCenters * rho0) * (Gr - 1)
         if self.scaleFactor != 1:
             Gr *= FLOAT_TYPE(self.scaleFactor)
         return Gr

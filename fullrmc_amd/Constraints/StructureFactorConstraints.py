@@ -71,12 +71,13 @@ class StructureFactorConstraint(_HistogramConstraint):
             Sq = scaleFactor * (Sq - 1) + 1
         return Sq
 
-    def _model(self, data):
+    def _model(self, data, rho0=None, counts=None, weights=None):
         """__get_total_Sq (StructureFactorConstraints.py:780-822) without
         scale-factor fitting, multiframe prior or window."""
-        Gr = self._gr_sum(data, cast_density=False)
+        rho0 = self.engine.numberDensity if rho0 is None else rho0
+        Gr = self._gr_sum(data, cast_density=False, counts=counts, weights=weights)
         Gr /= self.shellVolumes
-        Gr = (FLOAT_TYPE(4.) * PI * self.shellCenters * self.engine.numberDensity) * (Gr - 1)
+        Gr = (FLOAT_TYPE(4.) * PI * self.shellCenters * rho0) * (Gr - 1)
         Sq = self._get_Sq_from_Gr(Gr)
         return self._apply_scale_factor(Sq, FLOAT_TYPE(self.scaleFactor))
 

@@ -41,6 +41,10 @@ class Engine:
         self.selector = RandomSelector()
         self.tolerance = FLOAT_TYPE(0)
         self.generated = self.tried = self.accepted = self.tolerated = 0
+        self.removed = [0, 0]  # removals tried, accepted
+        self._last_accepted = False
+        self.collectedAtoms = []
+        self._collected_set = set()
         self.totalStandardError = None
         self._dev = None
         self._dev_kmax = 0
@@ -90,7 +94,11 @@ class Engine:
         for e in self.allNames:
             self.numberOfAtomsPerName[e] = self.numberOfAtomsPerName.get(e, 0) + 1
         self.numberOfAtoms = n
+        self.originalNumberOfAtoms = n
         self.numberDensity = FLOAT_TYPE(n) / FLOAT_TYPE(self.volume)
+        self._realIndex = np.arange(n, dtype=INT_TYPE)
+        self._collected_set = set()
+        self.collectedAtoms = []
         self.set_groups_as_atoms()
         self._drop_device()
 
@@ -119,9 +127,17 @@ class Engine:
         self.set_groups([np.array(v, dtype=INT_TYPE) for v in order.values()])
 
     def set_groups(self, groups, generator=None):
+        """Groups of atom indexes; ``generator`` is one move generator for
+        every group or a list with one per group (default: translations of
+        0.2 A, as the reference's groups get)."""
         self.groups = [np.array(g, dtype=INT_TYPE) for g in groups]
-        gen = generator if generator is not None else TranslationGenerator(amplitude=0.2)
-        self.generators = [gen] * len(self.groups)
+        if isinstance(generator, (list, tuple)):
+            if len(generator) != len(self.groups):
+                raise ValueError("one move generator per group is required")
+            self.generators = list(generator)
+        else:
+            gen = generator if generator is not None else TranslationGenerator(amplitude=0.2)
+            self.generators = [gen] * len(self.groups)
         self._drop_device()
 
     def set_move_generator(self, generator):
@@ -157,55 +173,138 @@ class Engine:
 
     # ------------------------------------------------------------- plain
     def run(self, numberOfSteps=100, log=None):
-        """The reference step loop, one move per drop-in call sequence."""
+        """The reference step loop (Engine.py:3377-3516), one move per
+        drop-in call sequence: group selection, move or removal, rigid
+        constraints, experimental constraints, the accept rule."""
         self._pull_device()
         self.initialize_used_constraints()
         for _ in range(int(numberOfSteps)):
             self.generated += 1
             gi = self.selector.select_index(len(self.groups))
-            rel = self.groups[gi]
-            before = np.array(self.realCoordinates[rel], dtype=FLOAT_TYPE)
-            moved_real = self.generators[gi].move(before)
-            moved_box = transform_coordinates(self.reciprocalBasisVectors, moved_real)
-            # rigid constraints first: any of them can reject the move untried
-            reject = False
-            for c in self.rigidConstraints:
-                c.compute_before_move(realIndexes=rel, relativeIndexes=rel)
-                c.compute_after_move(realIndexes=rel, relativeIndexes=rel, movedBoxCoordinates=moved_box)
-                reject = c.should_step_get_rejected(c.afterMoveStandardError)
-                if reject:
-                    break
-            tried = not reject
-            if reject:
-                for c in self.rigidConstraints:
-                    c.reject_move(realIndexes=rel, relativeIndexes=rel)
-            else:
-                self.tried += 1
-                for c in self.constraints:
-                    c.compute_before_move(realIndexes=rel, relativeIndexes=rel)
-                    c.compute_after_move(realIndexes=rel, relativeIndexes=rel, movedBoxCoordinates=moved_box)
-                new = self.compute_total_standard_error(self.constraints, current="afterMoveStandardError")
-                if new > self.totalStandardError:
-                    if random.random() > self.tolerance:
-                        reject = True
-                    else:
-                        self.tolerated += 1
-                        self.totalStandardError = new
+            gen = self.generators[gi]
+            if hasattr(gen, "pick_from_list"):
+                real = gen.pick_from_list(self)
+                if len(real):
+                    self._try_remove(real, self.relative_indexes(real))
+                if log is not None:
+                    log.append((int(gi), bool(len(real)) and self._last_accepted, float(self.totalStandardError)))
+                continue
+            real = self.groups[gi]
+            if self.collectedAtoms:
+                real = real[[not self.is_collected(int(i)) for i in real]]
+            if hasattr(gen, "get_ready_for_move") and len(real):
+                # swap moves act on the group plus the drawn partners (Engine.py:3192-3201)
+                if len(real) == gen.swapLength:
+                    real = gen.get_ready_for_move(real, self)
                 else:
-                    self.totalStandardError = new
-            if reject:
-                if tried:
-                    for c in self.constraints:
-                        c.reject_move(realIndexes=rel, relativeIndexes=rel)
-            else:
-                self.accepted += 1
-                for c in self.constraints + self.rigidConstraints:
-                    c.accept_move(realIndexes=rel, relativeIndexes=rel)
-                self.realCoordinates[rel] = moved_real
-                self.boxCoordinates[rel] = moved_box
+                    real = real[:0]
+            if not len(real):
+                # every atom of the group is gone: the move cannot be tried
+                if log is not None:
+                    log.append((int(gi), False, float(self.totalStandardError)))
+                continue
+            rel = self.relative_indexes(real)
+            before = np.array(self.realCoordinates[rel], dtype=FLOAT_TYPE)
+            moved_real = gen.move(before)
+            moved_box = transform_coordinates(self.reciprocalBasisVectors, moved_real)
+            self._try_move(real, rel, moved_real, moved_box)
             if log is not None:
-                log.append((int(gi), not reject, float(self.totalStandardError)))
+                log.append((int(gi), self._last_accepted, float(self.totalStandardError)))
         self._dev_synced = False
+
+    def _accept_rule(self, new):
+        """Engine.py:3300-3310: a worse total is kept with probability
+        `tolerance` (one Python random draw)."""
+        if new > self.totalStandardError:
+            if random.random() > self.tolerance:
+                return True
+            self.tolerated += 1
+        self.totalStandardError = new
+        return False
+
+    def _try_move(self, real, rel, moved_real, moved_box):
+        """Engine.py:3278-3344"""
+        reject = False
+        for c in self.rigidConstraints:
+            c.compute_before_move(realIndexes=real, relativeIndexes=rel)
+            c.compute_after_move(realIndexes=real, relativeIndexes=rel, movedBoxCoordinates=moved_box)
+            reject = c.should_step_get_rejected(c.afterMoveStandardError)
+            if reject:
+                break
+        tried = not reject
+        if reject:
+            for c in self.rigidConstraints:
+                c.reject_move(realIndexes=real, relativeIndexes=rel)
+        else:
+            self.tried += 1
+            for c in self.constraints:
+                c.compute_before_move(realIndexes=real, relativeIndexes=rel)
+                c.compute_after_move(realIndexes=real, relativeIndexes=rel, movedBoxCoordinates=moved_box)
+            reject = self._accept_rule(self.compute_total_standard_error(self.constraints,
+                                                                         current="afterMoveStandardError"))
+        if reject:
+            if tried:
+                for c in self.constraints:
+                    c.reject_move(realIndexes=real, relativeIndexes=rel)
+        else:
+            self.accepted += 1
+            for c in self.constraints + self.rigidConstraints:
+                c.accept_move(realIndexes=real, relativeIndexes=rel)
+            self.realCoordinates[rel] = moved_real
+            self.boxCoordinates[rel] = moved_box
+        self._last_accepted = not reject
+
+    def _try_remove(self, real, rel):
+        """Engine.py:3231-3275: the constraints as if the atom were gone,
+        the accept rule, then the atom leaves the configuration."""
+        self.tried += 1
+        self.removed[0] += 1
+        for c in self.constraints:
+            c.compute_as_if_amputated(realIndex=real, relativeIndex=rel)
+        reject = self._accept_rule(self.compute_total_standard_error(self.constraints,
+                                                                     current="amputationStandardError"))
+        if reject:
+            for c in self.constraints:
+                c.reject_amputation(realIndex=real, relativeIndex=rel)
+        else:
+            self.accepted += 1
+            self.removed[1] += 1
+            for c in self.constraints + self.rigidConstraints:
+                c.accept_amputation(realIndex=real, relativeIndex=rel)
+            self._collect_atom(int(real[0]), int(rel[0]))
+        self._last_accepted = not reject
+
+    # ---------------------------------------------------- atoms collector
+    def is_collected(self, realIndex):
+        return realIndex in self._collected_set
+
+    def relative_indexes(self, realIndexes):
+        """Positions of atoms (original indexes) in the current arrays."""
+        return np.searchsorted(self._realIndex, np.asarray(realIndexes)).astype(INT_TYPE)
+
+    def _collect_atom(self, real, rel):
+        """Engine._on_collector_collect_atom (Engine.py:758-794): the atom
+        leaves every per-atom array; counts and number density follow."""
+        el, name = self.allElements[rel], self.allNames[rel]
+        if self.numberOfAtomsPerElement[el] - 1 <= 0:
+            raise ValueError("removing the last '%s' atom is not allowed" % el)
+        for c in self.constraints + self.rigidConstraints:
+            c._on_collector_collect_atom(real, rel)
+        self.realCoordinates = np.delete(self.realCoordinates, rel, axis=0)
+        self.boxCoordinates = np.delete(self.boxCoordinates, rel, axis=0)
+        self.moleculesIndex = np.delete(self.moleculesIndex, rel, axis=0)
+        self.elementsIndex = np.delete(self.elementsIndex, rel, axis=0)
+        self.namesIndex = np.delete(self.namesIndex, rel, axis=0)
+        self.allElements.pop(rel)
+        self.allNames.pop(rel)
+        self.numberOfAtomsPerElement[el] -= 1
+        self.numberOfAtomsPerName[name] -= 1
+        self.numberOfAtoms -= 1
+        self.numberDensity = FLOAT_TYPE(self.numberOfAtoms) / FLOAT_TYPE(self.volume)
+        self._realIndex = np.delete(self._realIndex, rel)
+        self._collected_set.add(real)
+        self.collectedAtoms.append(real)
+        self._drop_device()
 
     # ------------------------------------------------------------ device
     def _drop_device(self):
@@ -227,6 +326,10 @@ class Engine:
         if self.rigidConstraints:
             raise NotImplementedError("rigid (distance) constraints veto moves one at a time in Engine.run; "
                                       "the device engine evaluates the experimental pair constraints only")
+        if any(hasattr(g, "get_ready_for_move") or hasattr(g, "pick_from_list") for g in self.generators):
+            raise NotImplementedError("swap and remove moves change their atoms per move; they run in Engine.run")
+        if self.collectedAtoms:
+            raise NotImplementedError("a configuration with removed atoms runs in Engine.run")
         if self._dev is not None and self._dev_kmax >= kmax and self._dev_synced:
             return self._dev
         self._pull_device()

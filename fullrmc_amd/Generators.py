@@ -10,6 +10,10 @@ rotation angle, acceptance) and NumPy's legacy ``numpy.random`` (directions).
   RotationGenerator     Generators/Rotations.py:113-178 with
                         get_rotation_matrix, Core/Collection.py:437-456
   MoveGeneratorCombinator  Core/MoveGenerator.py:792-811 (shuffle=False)
+  SwapPositionsGenerator   Generators/Swaps.py:22-133 with SwapGenerator,
+                        Core/MoveGenerator.py:321-530
+  AtomsRemoveGenerator  Generators/Removes.py:22-86 with RemoveGenerator,
+                        Core/MoveGenerator.py:107-301
 """
 import random
 
@@ -119,3 +123,97 @@ class MoveGeneratorCombinator:
         for g in self.combination:
             coordinates = g.move(coordinates)
         return coordinates
+
+
+class SwapPositionsGenerator:
+    """Exchange the positions of the group's atoms with those of a sub-list
+    drawn from ``swapList`` (``random.randint`` over the sub-lists).  The
+    engine extends the moved group to the group plus the drawn sub-list
+    (get_ready_for_move, Core/MoveGenerator.py:502-529); the move puts the
+    partner's positions on the group's atoms and the group's on the partner's
+    (Generators/Swaps.py:115-133)."""
+
+    native_kind = None
+
+    def __init__(self, group=None, swapLength=1, swapList=None):
+        n = int(swapLength)
+        if n < 1:
+            raise ValueError("swapLength must be a positive integer")
+        self.swapLength = n
+        self.set_swap_list(swapList)
+
+    def set_swap_list(self, swapList):
+        """Core/MoveGenerator.py:390-436: sub-lists of swapLength distinct
+        non-negative atom indexes."""
+        if swapList is None:
+            self.swapList = ()
+            return
+        if not isinstance(swapList, (list, tuple)):
+            raise ValueError("swapList must be a list")
+        out = []
+        for sl in swapList:
+            if not isinstance(sl, (list, tuple)):
+                raise ValueError("swapList items must be a list")
+            sub = [INT_TYPE(x) for x in sl]
+            if any(x < 0 for x in sub) or len(set(int(x) for x in sub)) != len(sub) or len(sub) != self.swapLength:
+                raise ValueError("swapList items must hold swapLength distinct non-negative indexes")
+            out.append(np.array(sub, dtype=INT_TYPE))
+        self.swapList = tuple(out)
+
+    def get_ready_for_move(self, groupAtomsIndexes, engine=None):
+        """The atoms the swap moves: the group, then a random sub-list of
+        those with no removed atom (Core/MoveGenerator.py:460-529; the group
+        itself when none is left)."""
+        remaining = self.swapList
+        if engine is not None and engine.collectedAtoms:
+            remaining = [sl for sl in self.swapList if not any(engine.is_collected(int(i)) for i in sl)]
+        if len(remaining):
+            swap = remaining[random.randint(0, len(remaining) - 1)]
+        else:
+            swap = groupAtomsIndexes
+        return np.concatenate((groupAtomsIndexes, swap))
+
+    def move(self, coordinates):
+        L = self.swapLength
+        out = np.array(coordinates, dtype=FLOAT_TYPE)
+        out[:L, :] = coordinates[L:, :]
+        out[L:, :] = coordinates[:L, :]
+        return out
+
+
+class AtomsRemoveGenerator:
+    """Remove one atom from the configuration: an atom drawn from
+    ``atomsList`` (``np.random.choice``) or, without a list, any atom index
+    of the original configuration (``random.randint``); nothing is removed
+    once ``maximumCollected`` atoms are gone or when the drawn atom already
+    is (pick_from_list, Generators/Removes.py:59-86)."""
+
+    native_kind = None
+
+    def __init__(self, group=None, maximumCollected=None, allowFittingScaleFactor=False, atomsList=None):
+        if maximumCollected is not None:
+            maximumCollected = INT_TYPE(maximumCollected)
+            if not maximumCollected > 0:
+                raise ValueError("maximumCollected must be bigger than 0")
+        self.maximumCollected = maximumCollected
+        self.allowFittingScaleFactor = bool(allowFittingScaleFactor)
+        if atomsList is not None:
+            # set_atoms_list (Core/MoveGenerator.py:236-270): order kept, no repeats
+            atomsList = np.array([INT_TYPE(i) for i in atomsList], dtype=INT_TYPE)
+            if np.any(atomsList < 0) or len(set(atomsList.tolist())) != len(atomsList):
+                raise ValueError("atomsList items must be distinct indexes >= 0")
+        self.atomsList = atomsList
+
+    def pick_from_list(self, engine):
+        if self.atomsList is None:
+            index = np.array([random.randint(0, engine.originalNumberOfAtoms - 1)], dtype=INT_TYPE)
+        else:
+            index = np.random.choice(self.atomsList, 1)
+        if self.maximumCollected is not None and len(engine.collectedAtoms) >= self.maximumCollected:
+            index = np.array([], dtype=INT_TYPE)
+        if len(index) and engine.is_collected(int(index[0])):
+            index = np.array([], dtype=INT_TYPE)
+        return index
+
+    def move(self, coordinates):
+        raise TypeError("a remove generator removes atoms; it does not move them")

@@ -10,6 +10,11 @@ constraint chain (chain.py), the accept rule, and accept/reject updates.
   Translation move    Generators/Translations.py:171-187, Core/Collection.py:674-701
   Rotation move       Generators/Rotations.py:141-178, Core/Collection.py:437-456
   Combinator          Core/MoveGenerator.py:792-811 (no shuffle)
+  Swap                Generators/Swaps.py:115-133, Core/MoveGenerator.py:460-529
+  Remove + amputation Generators/Removes.py:59-86, Engine.py:3231-3275, 758-794,
+                      PairDistributionConstraints.py:1168-1233,
+                      StructureFactorConstraints.py:1098-1163,
+                      DistanceConstraints.py:774-819
   PDC / SFC update    PairDistributionConstraints.py:1044-1166,
                       StructureFactorConstraints.py:975-1096
   PCF / reduced SFC   PairCorrelationConstraints.py:305-392 (same update,
@@ -85,6 +90,47 @@ class Combined:
         return coords
 
 
+class Swap:
+    """Position exchange with a random sub-list of swapList."""
+
+    def __init__(self, swapList, swapLength=1):
+        self.L = swapLength
+        self.swapList = [np.array(x, dtype=np.int32) for x in swapList]
+
+    def ready(self, idx, sysm):
+        rem = [x for x in self.swapList if not any(int(i) in sysm.collected for i in x)]
+        if len(rem):
+            return np.concatenate((idx, rem[random.randint(0, len(rem) - 1)]))
+        return np.concatenate((idx, idx))
+
+    def move(self, coords):
+        out = np.array(coords, dtype=F32)
+        out[:self.L] = coords[self.L:]
+        out[self.L:] = coords[:self.L]
+        return out
+
+
+class Remove:
+    """Atom removal: from atomsList by np.random.choice, else any original
+    index by random.randint; nothing once maxCollected atoms are gone or
+    when the drawn atom already is."""
+
+    def __init__(self, atomsList=None, maxCollected=None):
+        self.atomsList = None if atomsList is None else np.array(atomsList, dtype=np.int32)
+        self.maxCollected = maxCollected
+
+    def pick(self, sysm):
+        if self.atomsList is None:
+            idx = np.array([random.randint(0, sysm.n0 - 1)], dtype=np.int32)
+        else:
+            idx = np.random.choice(self.atomsList, 1)
+        if self.maxCollected is not None and len(sysm.collected) >= self.maxCollected:
+            return idx[:0]
+        if len(idx) and int(idx[0]) in sysm.collected:
+            return idx[:0]
+        return idx
+
+
 class System:
     """Box, species and molecules: the engine attributes the constraints read."""
 
@@ -104,12 +150,31 @@ class System:
         self.volume = F32(abs(np.linalg.det(self.basis.astype(np.float64))))
         self.rho0 = F32(len(self.real)) / F32(self.volume)
         self.nel = len(self.elements)
+        self.n0 = len(self.real)
+        self.collected = set()
+        self.real_index = np.arange(self.n0, dtype=np.int32)
+
+    def rel(self, real):
+        return np.searchsorted(self.real_index, np.asarray(real)).astype(np.int32)
+
+    def collect(self, real, rel):
+        """Engine.py:758-794"""
+        el = self.allElements[rel]
+        self.real = np.delete(self.real, rel, axis=0)
+        self.box = np.delete(self.box, rel, axis=0)
+        self.mol = np.delete(self.mol, rel, axis=0)
+        self.elem = np.delete(self.elem, rel, axis=0)
+        self.allElements.pop(rel)
+        self.counts[el] -= 1
+        self.rho0 = F32(len(self.real)) / F32(self.volume)
+        self.real_index = np.delete(self.real_index, rel)
+        self.collected.add(int(real))
 
 
 class PDC:
     """PairDistributionConstraint on the oracle path."""
 
-    def __init__(self, sysm, experimentalData, weights, scaleFactor=1.0, dataWeights=None):
+    def __init__(self, sysm, experimentalData, weights, scaleFactor=1.0, dataWeights=None, elementWeights=None):
         self.s = sysm
         self.exp = np.asarray(experimentalData, dtype=F32)
         self.bin, self.min, self.max, self.centers, self.H, self.vols = chain.pdc_grid(self.exp)
@@ -117,11 +182,17 @@ class PDC:
         self.w = weights
         self.sf = scaleFactor
         self.dw = dataWeights
+        self.elw = elementWeights  # per-element weights: needed to re-weight after a removal
 
-    def model(self, intra, inter):
+    def _args(self, counts, w, rho0):
         s = self.s
-        return chain.total_gr(intra, inter, s.elements, s.counts, self.w, s.volume, s.rho0, self.vols,
-                              self.centers, self.sf)
+        return (s.counts if counts is None else counts, self.w if w is None else w,
+                s.rho0 if rho0 is None else rho0)
+
+    def model(self, intra, inter, counts=None, w=None, rho0=None):
+        s = self.s
+        counts, w, rho0 = self._args(counts, w, rho0)
+        return chain.total_gr(intra, inter, s.elements, counts, w, s.volume, rho0, self.vols, self.centers, self.sf)
 
     def hist_full(self, box, mol, elem):
         return refcpu.full_pairs_histograms_coords(box, self.s.basis, True, mol, elem, self.s.nel, self.min,
@@ -132,8 +203,23 @@ class PDC:
                                                        self.s.nel, self.min, self.max, self.bin, self.H,
                                                        allAtoms=True)
 
-    def chi2(self, intra, inter):
-        return chain.chi_square(self.y, self.model(intra, inter), self.dw)
+    def chi2(self, intra, inter, counts=None, w=None, rho0=None):
+        return chain.chi_square(self.y, self.model(intra, inter, counts, w, rho0), self.dw)
+
+    def amputate(self, idx):
+        """compute_as_if_amputated: the atom's pairs out, counts, weights and
+        number density of the system without it."""
+        s = self.s
+        self.before(idx)
+        self.amp_i, self.amp_e = self.intra - self.bi, self.inter - self.be
+        counts = dict(s.counts)
+        counts[s.allElements[int(idx[0])]] -= 1
+        self.amp_w = chain.normalized_weighting(counts, self.elw)
+        rho0 = F32((len(s.real) - 1) / s.volume)
+        self.amp_sd = self.chi2(self.amp_i, self.amp_e, counts, self.amp_w, rho0)
+
+    def accept_amputation(self):
+        self.intra, self.inter, self.w, self.sd = self.amp_i, self.amp_e, self.amp_w, self.amp_sd
 
     # reference state machine
     def compute_data(self):
@@ -166,8 +252,10 @@ class PDC:
 class SFC(PDC):
     """StructureFactorConstraint on the oracle path."""
 
-    def __init__(self, sysm, experimentalData, weights, rmin, rmax, dr, scaleFactor=1.0, dataWeights=None):
+    def __init__(self, sysm, experimentalData, weights, rmin, rmax, dr, scaleFactor=1.0, dataWeights=None,
+                 elementWeights=None):
         self.s = sysm
+        self.elw = elementWeights
         self.exp = np.asarray(experimentalData, dtype=F32)
         self.q = self.exp[:, 0].astype(F32)
         self.y = self.exp[:, 1].astype(F32)
@@ -177,28 +265,31 @@ class SFC(PDC):
         self.sf = scaleFactor
         self.dw = dataWeights
 
-    def model(self, intra, inter):
+    def model(self, intra, inter, counts=None, w=None, rho0=None):
         s = self.s
-        return chain.total_sq(intra, inter, s.elements, s.counts, self.w, s.volume, s.rho0, self.vols,
-                              self.centers, self.M, self.sf)
+        counts, w, rho0 = self._args(counts, w, rho0)
+        return chain.total_sq(intra, inter, s.elements, counts, w, s.volume, rho0, self.vols, self.centers, self.M,
+                              self.sf)
 
 
 class PCF(PDC):
     """PairCorrelationConstraint on the oracle path: PDC grid, g(r) model."""
 
-    def model(self, intra, inter):
+    def model(self, intra, inter, counts=None, w=None, rho0=None):
         s = self.s
-        return chain.total_gofr(intra, inter, s.elements, s.counts, self.w, s.volume, s.rho0, self.vols,
-                                self.centers, self.sf)
+        counts, w, rho0 = self._args(counts, w, rho0)
+        return chain.total_gofr(intra, inter, s.elements, counts, w, s.volume, rho0, self.vols, self.centers,
+                                self.sf)
 
 
 class RSFC(SFC):
     """ReducedStructureFactorConstraint on the oracle path."""
 
-    def model(self, intra, inter):
+    def model(self, intra, inter, counts=None, w=None, rho0=None):
         s = self.s
-        return chain.total_sq(intra, inter, s.elements, s.counts, self.w, s.volume, s.rho0, self.vols,
-                              self.centers, self.M, self.sf, reduced=True)
+        counts, w, rho0 = self._args(counts, w, rho0)
+        return chain.total_sq(intra, inter, s.elements, counts, w, s.volume, rho0, self.vols, self.centers, self.M,
+                              self.sf, reduced=True)
 
 
 class IMD:
@@ -270,12 +361,42 @@ class IMD:
     def accept(self):
         self.number, self.dsum, self.sd = self.new_n, self.new_d, self.after_sd
 
+    def accept_amputation(self, idx):
+        self.before(idx)
+        self.number, self.dsum = self.number - self.bn, self.dsum - self.bd
+        self.sd = self.value(self.number, self.dsum)
+
 
 class Engine:
+    def remove(self, real, idx):
+        """Engine.py:3231-3275"""
+        self.tried += 1
+        for c in self.constraints:
+            c.amputate(idx)
+        new = chain.total_error([c.amp_sd for c in self.constraints])
+        reject = False
+        if new > self.tse:
+            if random.random() > self.tolerance:
+                reject = True
+            else:
+                self.tolerated += 1
+                self.tse = new
+        else:
+            self.tse = new
+        if not reject:
+            self.accepted += 1
+            for c in self.constraints:
+                c.accept_amputation()
+            for c in self.rigid:
+                c.accept_amputation(idx)
+            self.s.collect(int(real[0]), int(idx[0]))
+        return not reject
+
     def __init__(self, sysm, groups, generator, constraints, tolerance=0.0, rigid=()):
+        """generator: one for every group, or a list with one per group."""
         self.s = sysm
         self.groups = [np.array(g, dtype=np.int32) for g in groups]
-        self.generator = generator
+        self.generators = list(generator) if isinstance(generator, (list, tuple)) else [generator] * len(groups)
         self.constraints = constraints
         self.rigid = list(rigid)
         self.tolerance = F32(F32(tolerance) / 100.)
@@ -291,9 +412,27 @@ class Engine:
         for _ in range(nsteps):
             self.generated += 1
             gi = np.int32(random.randint(0, len(self.groups) - 1))
-            idx = self.groups[gi]
+            gen = self.generators[gi]
+            if hasattr(gen, "pick"):
+                real = gen.pick(s)
+                ok = False
+                if len(real):
+                    ok = self.remove(real, s.rel(real))
+                if log is not None:
+                    log.append((int(gi), ok, float(self.tse)))
+                continue
+            real = self.groups[gi]
+            if s.collected:
+                real = real[[int(i) not in s.collected for i in real]]
+            if hasattr(gen, "ready") and len(real):
+                real = gen.ready(real, s) if len(real) == gen.L else real[:0]
+            if not len(real):
+                if log is not None:
+                    log.append((int(gi), False, float(self.tse)))
+                continue
+            idx = s.rel(real)
             before = np.array(s.real[idx], dtype=F32)
-            moved_real = self.generator.move(before)
+            moved_real = gen.move(before)
             moved_box = refcpu.transform_coordinates(s.recip, moved_real)
             reject = False
             for c in self.rigid:

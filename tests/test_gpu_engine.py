@@ -12,7 +12,8 @@ from fullrmc_amd.Constraints import (InterMolecularDistanceConstraint, PairCorre
                                      PairDistributionConstraint, ReducedStructureFactorConstraint,
                                      StructureFactorConstraint)
 from fullrmc_amd.Engine import Engine
-from fullrmc_amd.Generators import MoveGeneratorCombinator, RotationGenerator, TranslationGenerator
+from fullrmc_amd.Generators import (AtomsRemoveGenerator, MoveGeneratorCombinator, RotationGenerator,
+                                    SwapPositionsGenerator, TranslationGenerator)
 from oracle import chain, refcpu
 from oracle import engine as oeng
 
@@ -282,6 +283,74 @@ def test_intermolecular_distance_constraint_vetoes_in_dropin_loop(small_triclini
     assert np.float32(c.standardError).tobytes() == np.float32(oc.sd).tobytes()
     assert E.tried < E.generated  # some moves were vetoed before being tried
     assert oc.sd > 0
+    with pytest.raises(NotImplementedError):
+        E.run_device(10)
+
+
+def test_swap_moves_in_dropin_loop():
+    """SwapPositionsGenerator on the Fe atoms of a B2 alloy (partners drawn
+    from the Al atoms), translations on the Al atoms: swaps move two atoms
+    of different elements, so the pair-type slots change."""
+    s = systems.b2_alloy(cells=(5, 5, 5), a=3.01, sigma=0.08, seed=13)
+    exp = pdc_target(s, 0.08, 14, 0.01, 0.02, 300)
+    n = len(s["real"])
+    el = s["elements"]
+    al = [[j] for j in range(n) if el[j] == "Al"]
+    gp = [SwapPositionsGenerator(swapList=al) if el[i] == "Fe" else TranslationGenerator(0.1) for i in range(n)]
+    go = [oeng.Swap(al) if el[i] == "Fe" else oeng.Translation(0.1) for i in range(n)]
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], go, _pdc_cons_of({"exp": exp}), 200, 21,
+                                    tolerance=5.0)
+    random.seed(21)
+    np.random.seed(21)
+    E = device_engine(s, [PairDistributionConstraint(exp)], groups=[[i] for i in range(n)], gen=gp)
+    E.set_tolerance(5.0)
+    dlog = []
+    E.run(200, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    swaps = [a for g, a, _ in dlog if el[g] == "Fe"]
+    assert len(swaps) > 20 and any(swaps)
+    with pytest.raises(NotImplementedError):
+        E.run_device(10)
+
+
+def test_remove_moves_amputation_in_dropin_loop():
+    """AtomsRemoveGenerator groups beside translations on a B2 alloy with a
+    G(r), an S(Q) and a distance constraint: each removal is evaluated as if
+    amputated (atom counts, weights, number density without the atom) and,
+    accepted, takes the atom out of every array; later moves run on the
+    smaller system.  Same trajectory as the oracle."""
+    s = systems.b2_alloy(cells=(5, 5, 5), a=3.01, sigma=0.08, seed=17)
+    pdc_exp = pdc_target(s, 0.08, 18, 0.01, 0.02, 300)
+    q = np.linspace(0.5, 20, 80).astype(F32)
+    sfc_exp = sfc_target(s, 0.08, 18, q, 0.0, 6.0, 0.02)
+    n = len(s["real"])
+    el = s["elements"]
+    al = [j for j in range(n) if el[j] == "Al"]
+    groups = [[i] for i in range(n)] + [[0]] * 60
+    elw = {e: WEIGHTS[e] for e in sorted(set(el))}
+    gp = [TranslationGenerator(0.1)] * n + [AtomsRemoveGenerator(maximumCollected=12, atomsList=al)] * 60
+    go = [oeng.Translation(0.1)] * n + [oeng.Remove(al, 12)] * 60
+
+    def cons(osys, w):
+        return [oeng.PDC(osys, pdc_exp, w, elementWeights=elw),
+                oeng.SFC(osys, sfc_exp, w, 0.0, 6.0, 0.02, elementWeights=elw)]
+
+    oe, olog, pys, nps = oracle_run(s, groups, go, cons, 300, 23, tolerance=20.0,
+                                    rigid_fn=lambda o: [oeng.IMD(o, 2.3)])
+    random.seed(23)
+    np.random.seed(23)
+    E = device_engine(s, [PairDistributionConstraint(pdc_exp),
+                          StructureFactorConstraint(sfc_exp, rmin=0.0, rmax=6.0, dr=0.02),
+                          InterMolecularDistanceConstraint(2.3)], groups=groups, gen=gp)
+    E.set_tolerance(20.0)
+    dlog = []
+    E.run(300, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    c, oc = E.rigidConstraints[0], oe.rigid[0]
+    assert c.data["number"].tobytes() == oc.number.tobytes()
+    assert c.data["distanceSum"].tobytes() == oc.dsum.tobytes()
+    assert 0 < len(E.collectedAtoms) <= 12 and E.numberOfAtoms == n - len(E.collectedAtoms)
+    assert sorted(E.collectedAtoms) == sorted(oe.s.collected)
     with pytest.raises(NotImplementedError):
         E.run_device(10)
 
