@@ -102,6 +102,17 @@ struct Inline {
     float nw[kInline][3];
 };
 
+// An accepted single-atom move whose commit is applied by the next window's
+// k_delta (before its own deltas, which read the moved atom at its new
+// position) instead of a separate k_commit launch: the old and new box
+// coordinates of the atom and the proposal's delta block (base += dn).
+struct Pending {
+    int32_t valid;
+    int32_t atom, el;
+    float old[3], nw[3];
+    const int32_t *dn;
+};
+
 // Which proposal of a window k_commit commits: p >= 0 a given one (the
 // caller decided), p < 0 the first proposal that the reference's rule accepts
 // (Engine.py:3308-3317) given the current total cur, the tolerance and the
@@ -117,7 +128,7 @@ struct CommitCtl {
 constexpr int kDP = 4;  // proposals per k_delta block
 
 // launchers (engine_kernels.hip)
-int launch_delta(const View &V, const Inline &I, int32_t k, hipStream_t st);
+int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hipStream_t st);
 int launch_eval(const View &V, int32_t k, hipStream_t st);
 int launch_eval_b(const View &V, int32_t k, hipStream_t st);
 int launch_commit(const View &V, const Inline &I, const CommitCtl &C, hipStream_t st);
@@ -154,6 +165,9 @@ struct frmc_engine {
     float *tse_dev = nullptr;
     unsigned long long *h_commit = nullptr;  // [2] written by k_commit (slot = window parity)
     bool device_decided = false;     // the last window's commit was queued on the device
+    bool deferred = false;           // the last window's commit is left to the next k_delta
+    frmc::Pending pend{};            // accepted move not yet applied on the device
+    const int32_t *win_dn = nullptr; // delta buffer of the last launched window
     int32_t pending_check = -2;      // decision of the last device-decided window (-2: none)
     uint32_t pending_seq = 0;
     frmc::Inline last_inline;        // proposals of the last evaluated window (k = 0: staged)
@@ -185,6 +199,7 @@ View make_view(frmc_engine *e, int stage_idx);
 int launch_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
                           const float *u, float tol);
 int finish_device_decided(frmc_engine *e, int32_t k, float *tse_out);
+int flush_commit(frmc_engine *e);
 int settle_window(frmc_engine *e, int32_t p);
 int finish_windows(frmc_engine *e);
 void collect_times(frmc_engine *e);

@@ -4,10 +4,13 @@
 // engine_kernels.hip, the native move driver in driver.hip.
 //
 // A window: the host writes the proposals into one of two pinned staging
-// buffers, launches k_delta, k_eval_a and k_eval_b on the engine stream, and
-// polls the completion flags k_eval_b writes into pinned memory after the
-// results; a commit is launched behind them without waiting.  The staging
-// buffer a commit reads is not rewritten until the next window has completed.
+// buffers (copied to the device unless the proposals fit in the kernel
+// arguments), launches k_delta, k_eval_a and k_eval_b on the engine stream,
+// and polls the result words k_eval_b writes into pinned memory.  An accepted single-atom move of the native loop is committed by
+// the next window's k_delta (or by flush_commit before anything else touches
+// the state); other commits are launched behind the window without waiting.
+// The staging buffer a commit reads is not rewritten until the next window
+// has completed.
 #include <math.h>
 #include <stdlib.h>
 #include <time.h>
@@ -212,8 +215,9 @@ static int launch_window(frmc_engine *e, int s, int32_t k, const Inline *I, cons
     int rc;
     if (with_delta) {
         KTimer t(e, FRMC_K_DELTA);
-        rc = launch_delta(V, *I, k, e->st);
+        rc = launch_delta(V, *I, e->pend, k, e->st);
         if (rc) return rc;
+        e->pend.valid = 0;  // applied by this window's k_delta
     }
     {
         KTimer t(e, FRMC_K_EVAL);
@@ -276,12 +280,17 @@ static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids,
     }
     S.poff[k] = off;
     memcpy(S.pnew, new_box, sizeof(float) * 3 * (size_t)off);
-    // the kernels read the device copy; the copy is ordered before them on the stream
-    FRMC_CHECK_HIP(hipMemcpyAsync(S.dev, S.host, S.bytes, hipMemcpyHostToDevice, e->st));
     e->cur_stage = s;
     Inline &I = e->last_inline;
     I.k = 0;
-    if (e->gmax == 1 && k <= kInline) {
+    const bool inl = e->gmax == 1 && k <= kInline;
+    if (!inl) {
+        // the kernels read the device copy; the copy is ordered before them on
+        // the stream.  Single-atom windows carry everything in the kernel
+        // arguments and skip it (the host block still backs note_commit).
+        FRMC_CHECK_HIP(hipMemcpyAsync(S.dev, S.host, S.bytes, hipMemcpyHostToDevice, e->st));
+    }
+    if (inl) {
         I.k = k;
         for (int32_t p = 0; p < k; p++) {
             const int32_t id = e->h_gatoms[e->h_goff[group_ids[p]]];
@@ -294,7 +303,15 @@ static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids,
         }
     }
     int rc;
-    if (u && k <= kInline) {
+    e->win_dn = e->dnbuf[e->cur_dn];  // the buffer this window's deltas go to
+    if (u && e->gmax == 1) {
+        // single-atom moves of the native loop: the host's decision is
+        // committed by the next window's k_delta (or flush_commit)
+        rc = launch_window(e, s, k, &I);
+        e->device_decided = false;
+        e->deferred = true;
+    } else if (u && k <= kInline) {
+        e->deferred = false;
         CommitCtl C;
         C.k = k;
         C.p = -1;
@@ -305,10 +322,35 @@ static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids,
         rc = launch_window(e, s, k, &I, &C);
         e->device_decided = true;
     } else {
+        e->deferred = false;
         rc = launch_window(e, s, k, &I);
         e->device_decided = false;
     }
     return rc;
+}
+
+// Apply an accepted move still left for a next window's k_delta, as a
+// k_commit of its own (before anything else reads or replaces the state).
+int flush_commit(frmc_engine *e) {
+    if (!e->pend.valid) return FRMC_OK;
+    Inline I1;
+    I1.k = 1;
+    I1.atom[0] = e->pend.atom;
+    I1.el[0] = e->pend.el;
+    for (int d = 0; d < 3; d++) {
+        I1.old[0][d] = e->pend.old[d];
+        I1.nw[0][d] = e->pend.nw[d];
+    }
+    CommitCtl C;
+    C.k = 1;
+    C.p = 0;
+    C.cur = e->cur_tse;
+    C.tol = 0.0f;
+    C.dn = e->pend.dn;
+    View V = make_view(e, e->cur_stage);
+    e->pend.valid = 0;
+    KTimer t(e, FRMC_K_COMMIT);
+    return launch_commit(V, I1, C, e->st);
 }
 
 static int complete_eval(frmc_engine *e, int32_t k) {
@@ -356,8 +398,11 @@ int finish_device_decided(frmc_engine *e, int32_t k, float *tse_out) {
     return FRMC_OK;
 }
 
-// wait for the stream and verify the last device-made decision
+// commit what is pending, wait for the stream and verify the last
+// device-made decision
 int finish_windows(frmc_engine *e) {
+    int rc = flush_commit(e);
+    if (rc) return rc;
     FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
     return check_device_decision(e);
 }
@@ -365,6 +410,26 @@ int finish_windows(frmc_engine *e) {
 // The host's decision for the last window: p >= 0 committed (by the device
 // when it decided, else by a commit launched here), -1 none.
 int settle_window(frmc_engine *e, int32_t p) {
+    if (e->deferred) {
+        e->deferred = false;
+        if (p < 0) {
+            e->last_k = 0;
+            return FRMC_OK;
+        }
+        const Stage &S = e->stage[e->cur_stage];
+        const int32_t a0 = S.poff[p];
+        const int32_t id = e->h_gatoms[e->h_goff[S.pgroup[p]]];
+        e->pend.valid = 1;
+        e->pend.atom = id;
+        e->pend.el = e->h_elem[id];
+        for (int d = 0; d < 3; d++) {
+            e->pend.old[d] = S.pold[3 * (size_t)a0 + d];
+            e->pend.nw[d] = S.pnew[3 * (size_t)a0 + d];
+        }
+        e->pend.dn = e->win_dn + (int64_t)p * e->dnStride;
+        note_commit(e, p);
+        return FRMC_OK;
+    }
     if (e->device_decided) {
         e->pending_check = p;
         e->pending_seq = e->seq;
@@ -586,6 +651,7 @@ int frmc_engine_destroy(frmc_engine *e) {
 
 int frmc_engine_compute_data(frmc_engine *e, float *chi2_out, float *tse_out) {
     FRMC_REQUIRE(e, "NULL engine");
+    if (int rc0 = flush_commit(e)) return rc0;
     hipStream_t st = e->st;
     for (int ci = 0; ci < e->C; ci++) {
         DevC &dc = e->c[ci];
@@ -605,6 +671,7 @@ int frmc_engine_refresh(frmc_engine *e, float *chi2_out, float *tse_out) {
     // the current state evaluated as one zero-delta window (the delta buffer
     // of the next window is all zero)
     FRMC_REQUIRE(e, "NULL engine");
+    if (int rc0 = flush_commit(e)) return rc0;
     const int cur = e->cur_dn;
     int rc = run_window(e, e->cur_stage, 1, nullptr);
     if (rc) return rc;
@@ -623,6 +690,7 @@ int frmc_engine_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const 
     FRMC_REQUIRE(e, "NULL engine");
     FRMC_REQUIRE(k >= 1 && k <= e->kmax, "k=%d out of range [1,%d]", k, e->kmax);
     FRMC_REQUIRE(group_ids && new_box && tse_out, "NULL argument");
+    if (int rc0 = flush_commit(e)) return rc0;
     int rc = stage_and_eval(e, k, group_ids, new_box);
     if (rc) return rc;
     memcpy(tse_out, e->h_tse.data(), sizeof(float) * k);
@@ -659,6 +727,7 @@ int frmc_engine_sync(frmc_engine *e) {
 int frmc_engine_get_state(frmc_engine *e, float *box_out, int32_t constraint, int32_t *slot_out, int32_t *base_out,
                           float *chi2_out, float *tse_out) {
     FRMC_REQUIRE(e, "NULL engine");
+    if (int rc0 = flush_commit(e)) return rc0;
     hipStream_t st = e->st;
     if (box_out)
         FRMC_CHECK_HIP(hipMemcpyAsync(box_out, e->xyz, sizeof(float) * 3 * (size_t)e->n, hipMemcpyDeviceToHost, st));
@@ -682,6 +751,7 @@ int frmc_engine_get_state(frmc_engine *e, float *box_out, int32_t constraint, in
 
 int frmc_engine_set_counts(frmc_engine *e, int32_t constraint, const int32_t *slot_in) {
     FRMC_REQUIRE(e && slot_in, "NULL argument");
+    if (int rc0 = flush_commit(e)) return rc0;
     FRMC_REQUIRE(constraint >= 0 && constraint < e->C, "constraint %d out of range", constraint);
     DevC &dc = e->c[constraint];
     const size_t count = 2 * (size_t)e->nel * e->nel * dc.H + 1;

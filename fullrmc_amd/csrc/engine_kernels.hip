@@ -7,7 +7,9 @@
 //             constraint).  Grid (chunks of 256 atoms, groups of kDP
 //             proposals), one other atom per thread; int32 pair-type deltas
 //             go by atomics into the proposal's zeroed block of the window's
-//             delta buffer (a move touches a few thousand bins).
+//             delta buffer (a move touches a few thousand bins).  It first
+//             commits the previous window's accepted single-atom move when
+//             one is pending (the work of k_commit, folded in).
 //   k_eval_a  grid (work units, proposals), one wave each: base + delta ->
 //             G(r) in the reference's float32 operation order
 //             (PairDistributionConstraints.py:854-895); per PDC constraint the
@@ -110,10 +112,32 @@ __device__ __forceinline__ void delta_pair(const View &V, int32_t *out, float so
     }
 }
 
+// slot counts (intra/inter) of one pair at squared norm s, sign -1 or +1
+template <bool PBC, bool ORTHO>
+__device__ __forceinline__ void commit_pair(const View &V, float s, int32_t ea, int32_t eb, bool intra, int32_t sign) {
+    int32_t b = -1;
+    for (int32_t ci = 0; ci < V.C; ci++) {
+        const DevC &c = V.c[ci];
+        if (!c.sameGrid) b = in_range(s, c.g) ? bin_of(s, c.g) : -1;
+        if (b < 0) continue;
+        const int32_t size = V.nel * V.nel * c.H;
+        const int32_t f = flat_slot(b, ea, eb, V.nel, c.H);
+        if (f < 0) {
+            if (sign > 0) atomicAdd(&c.slot[2 * size], 1);  // past the array: counted, not stored
+            continue;
+        }
+        atomicAdd(&c.slot[(intra ? 0 : size) + f], sign);
+    }
+}
+
 // grid (ceil(n/256), ceil(k/kDP)): one other atom per thread, kDP proposals
 // per block; deltas go straight into the proposal's (zeroed) block of dn.
+// P.valid: the previous window's accepted single-atom move is committed here
+// first -- base += its delta block (grid-stride), its slot counts (blocks of
+// the first proposal row), its new coordinates (one thread) -- and every
+// thread reads the moved atom at its new position.
 template <bool PBC, bool ORTHO>
-__global__ void __launch_bounds__(256) k_delta(View V, Inline I, int32_t k) {
+__global__ void __launch_bounds__(256) k_delta(View V, Inline I, Pending P, int32_t k) {
     __shared__ GroupLds G[kDP];
     const int tid = threadIdx.x;
     const int32_t p0 = blockIdx.y * kDP;
@@ -126,8 +150,30 @@ __global__ void __launch_bounds__(256) k_delta(View V, Inline I, int32_t k) {
     const int32_t j = blockIdx.x * blockDim.x + tid;
     const bool jok = j < V.n;
     const int32_t jj = min(j, V.n - 1);  // clamped: loads stay in bounds
-    const float cx = V.xyz[3 * (int64_t)jj], cy = V.xyz[3 * (int64_t)jj + 1], cz = V.xyz[3 * (int64_t)jj + 2];
+    float cx = V.xyz[3 * (int64_t)jj], cy = V.xyz[3 * (int64_t)jj + 1], cz = V.xyz[3 * (int64_t)jj + 2];
     const int32_t ej = V.elem[jj];
+    if (P.valid) {
+        const int64_t gt = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + tid;
+        const int64_t gs = (int64_t)gridDim.x * gridDim.y * blockDim.x;
+        for (int32_t ci = 0; ci < V.C; ci++) {
+            const DevC &c = V.c[ci];
+            for (int64_t s = gt; s < (int64_t)V.T * c.H; s += gs) c.base[s] += P.dn[c.dnOff + s];
+        }
+        if (blockIdx.y == 0 && jok && j != P.atom) {
+            const float so = pair_s<PBC, ORTHO>(P.old[0], P.old[1], P.old[2], cx, cy, cz, V.B);
+            const float sn = pair_s<PBC, ORTHO>(P.nw[0], P.nw[1], P.nw[2], cx, cy, cz, V.B);
+            const bool intra = V.mol[P.atom] == V.mol[j];
+            commit_pair<PBC, ORTHO>(V, so, P.el, ej, intra, -1);
+            commit_pair<PBC, ORTHO>(V, sn, P.el, ej, intra, +1);
+        }
+        if (jj == P.atom) {
+            cx = P.nw[0];
+            cy = P.nw[1];
+            cz = P.nw[2];
+        }
+        if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0)
+            for (int d = 0; d < 3; d++) V.xyz[3 * (int64_t)P.atom + d] = P.nw[d];
+    }
     for (int32_t q = 0; q < np; q++) {
         const int32_t p = p0 + q;
         int32_t *out = V.dn + (int64_t)p * V.dnStride;
@@ -483,22 +529,6 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
 
 // ---------------------------------------------------------------- k_commit
 
-template <bool PBC, bool ORTHO>
-__device__ __forceinline__ void commit_pair(const View &V, float s, int32_t ea, int32_t eb, bool intra, int32_t sign) {
-    int32_t b = -1;
-    for (int32_t ci = 0; ci < V.C; ci++) {
-        const DevC &c = V.c[ci];
-        if (!c.sameGrid) b = in_range(s, c.g) ? bin_of(s, c.g) : -1;
-        if (b < 0) continue;
-        const int32_t size = V.nel * V.nel * c.H;
-        const int32_t f = flat_slot(b, ea, eb, V.nel, c.H);
-        if (f < 0) {
-            if (sign > 0) atomicAdd(&c.slot[2 * size], 1);  // past the array: counted, not stored
-            continue;
-        }
-        atomicAdd(&c.slot[(intra ? 0 : size) + f], sign);
-    }
-}
 
 // grid ceil(n/256): picks the proposal to commit (every block makes the same
 // decision from the window's totals), then slot counts of the accepted group
@@ -607,13 +637,13 @@ __global__ void k_fill(int32_t *__restrict__ p, int64_t n) {
 
 // ------------------------------------------------------------------ launch
 
-int launch_delta(const View &V, const Inline &I, int32_t k, hipStream_t st) {
+int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hipStream_t st) {
     const dim3 grid(blocks_for(V.n, 256), (unsigned)((k + kDP - 1) / kDP));
     if (V.isPBC) {
-        if (V.ortho) k_delta<true, true><<<grid, 256, 0, st>>>(V, I, k);
-        else k_delta<true, false><<<grid, 256, 0, st>>>(V, I, k);
+        if (V.ortho) k_delta<true, true><<<grid, 256, 0, st>>>(V, I, P, k);
+        else k_delta<true, false><<<grid, 256, 0, st>>>(V, I, P, k);
     } else {
-        k_delta<false, false><<<grid, 256, 0, st>>>(V, I, k);
+        k_delta<false, false><<<grid, 256, 0, st>>>(V, I, P, k);
     }
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;

@@ -393,7 +393,8 @@ int frmc_engines_run_translation(frmc_engine **engines, int32_t nengines, int64_
    the number of timed launches. */
 #define FRMC_K_DELTA 0   /* k_delta: moved group vs all atoms, k proposals */
 #define FRMC_K_EVAL 1    /* k_eval_a: G(r), PDC chi-square, S(Q) per proposal */
-#define FRMC_K_COMMIT 2  /* k_commit: histograms and coordinates of the accepted move */
+#define FRMC_K_COMMIT 2  /* k_commit: histograms and coordinates of an accepted move
+                            (launched only when not folded into the next k_delta) */
 #define FRMC_K_EVAL_B 3  /* k_eval_b: SFC chi-squares, totals, completion flags */
 #define FRMC_NKERNELS 4
 int frmc_engine_profile(frmc_engine *e, int every);
