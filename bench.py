@@ -517,7 +517,7 @@ def main():
                     help="also run the 200,000-atom SiO2 box (full histograms and k=128 moves)")
     ap.add_argument("--sio2-moves", type=int, default=3000)
     ap.add_argument("--replicas", type=int, default=256, help="independent replicas run together (0: skip)")
-    ap.add_argument("--replica-moves", type=int, default=200, help="generated moves per replica")
+    ap.add_argument("--replica-moves", type=int, default=600, help="generated moves per replica")
     ap.add_argument("--molecular-moves", type=int, default=1000,
                     help="moves of the 6,500-atom molecular triclinic box (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -13,6 +13,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
@@ -91,6 +93,8 @@ static inline void to_box(const float M[9], const float r[3], float b[3]) {
     b[1] = r[0] * M[1] + r[1] * M[4] + r[2] * M[7];
     b[2] = r[0] * M[2] + r[1] * M[5] + r[2] * M[8];
 }
+
+constexpr int32_t kReplicaThreads = 16;  // host threads driving independent replicas
 
 // Window size from the running acceptance rate: enough proposals that the
 // window holds an acceptance with ~95 % probability, within [kmin, kmax].
@@ -288,27 +292,49 @@ extern "C" int frmc_engines_run_translation(frmc_engine **engines, int32_t nengi
         int rc = run.init();
         if (rc) return rc;
     }
-    // every engine has one window in flight on its own stream; the host
-    // draws and launches all windows, then finishes them in the same order
-    bool active = true;
-    while (active) {
-        active = false;
-        for (auto &run : runs)
-            if (!run.done()) {
-                int rc = run.launch();
-                if (rc) return rc;
-                active = true;
+    // A few host threads share the engines (engine r to thread r % T): each
+    // keeps one window in flight per engine on the engine's own stream, draws
+    // and launches all its windows, then finishes them in the same order.  The
+    // per-launch host cost is what bounds many small replicas, so it is spread
+    // over threads; engines never share state, so the threads need no locks.
+    const int32_t hw = (int32_t)std::max(1u, std::thread::hardware_concurrency());
+    const int32_t T = std::max<int32_t>(1, std::min<int32_t>({nengines, kReplicaThreads, hw}));
+    for (int32_t r = 1; r < nengines; r++)
+        FRMC_REQUIRE(engines[r]->device == engines[0]->device, "replicas must share one device");
+    std::vector<int> rcs(T, FRMC_OK);
+    std::vector<std::string> errs(T);
+    auto work = [&](int32_t t) {
+        int rc = (int)hipSetDevice(engines[0]->device) == (int)hipSuccess ? FRMC_OK : FRMC_EHIP;
+        if (rc) set_error("hipSetDevice failed in a replica thread");
+        bool active = rc == FRMC_OK;
+        while (active) {
+            active = false;
+            for (int32_t r = t; r < nengines && !rc; r += T)
+                if (!runs[r].done()) {
+                    rc = runs[r].launch();
+                    active = true;
+                }
+            for (int32_t r = t; r < nengines && !rc; r += T) {
+                if (runs[r].kw <= 0) continue;
+                rc = runs[r].finish();
+                runs[r].kw = 0;
             }
-        for (auto &run : runs) {
-            if (run.kw <= 0) continue;
-            int rc = run.finish();
-            if (rc) return rc;
-            run.kw = 0;
+            if (rc) active = false;
         }
-    }
-    for (auto &run : runs) {
-        int rc = finish_windows(run.e);
-        if (rc) return rc;
-    }
+        for (int32_t r = t; r < nengines && !rc; r += T) rc = finish_windows(runs[r].e);
+        if (rc) {
+            rcs[t] = rc;
+            errs[t] = frmc_last_error();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int32_t t = 1; t < T; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    for (int32_t t = 0; t < T; t++)
+        if (rcs[t]) {
+            set_error("%s", errs[t].c_str());
+            return rcs[t];
+        }
     return FRMC_OK;
 }

@@ -146,6 +146,7 @@ struct frmc_engine {
     bool own_stream = false;
     int32_t n = 0, nel = 0, T = 0, C = 0, G = 0, isPBC = 0, ortho = 0, kmax = 0, gmax = 0, total_gatoms = 0;
     int32_t ncu = 256;
+    int device = 0;                  // the HIP device the engine's memory and stream live on
     frmc::Basis B;
     float *xyz = nullptr, *dbasis = nullptr;
     int32_t *mol = nullptr, *elem = nullptr, *goff = nullptr, *gatoms = nullptr;

@@ -487,6 +487,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
                    d->basis[5] == 0.f && d->basis[6] == 0.f && d->basis[7] == 0.f;
         int dev = 0;
         ECHK(hipGetDevice(&dev));
+        e->device = dev;
         ECHK(hipDeviceGetAttribute(&e->ncu, hipDeviceAttributeMultiprocessorCount, dev));
         if (d->stream) {
             e->st = (hipStream_t)d->stream;
