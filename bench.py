@@ -114,12 +114,14 @@ def make_engine(start, pdc, sfc, seed, dr=0.02):
     return E
 
 
-def gpu_full_histogram(start, elem, dr, H, reps=3, float64=False):
+def gpu_full_histogram(start, elem, dr, H, reps=3, float64=False, autotune=True):
     """One O(N^2) recompute on the PDC grid (r in [0, dr*H)): median kernel
     time by HIP events; float64=True times the binary64-recompute mode and
-    also returns its number of double re-checks."""
+    also returns its number of double re-checks.  autotune=False times the
+    fixed-rule schedule instead of the tuned one; the schedule used is
+    returned last."""
     from fullrmc_amd import _device as D
-    from fullrmc_amd import _lib, systems
+    from fullrmc_amd import _lib, systems, tuning
     L = _lib.lib()
     box = systems.box_coordinates(start["recip"], start["real"])
     n = len(box)
@@ -128,6 +130,7 @@ def gpu_full_histogram(start, elem, dr, H, reps=3, float64=False):
     d_m, d_e = D.upload(start["molecules"]), D.upload(elem)
     counts = D.empty((2 * nel * nel * H + 4,), np.int32)
     rechecked = ctypes.c_int64(0)
+    tuning.set_autotune(autotune)
     L.frmc_set_timing(1)
     ms = []
     for _ in range(reps + 1):
@@ -141,9 +144,11 @@ def gpu_full_histogram(start, elem, dr, H, reps=3, float64=False):
                                                 D.stream()))
         ms.append(L.frmc_last_kernel_ms())
     L.frmc_set_timing(0)
+    sched = tuning.last_full_schedule()
+    tuning.set_autotune(True)
     ms = sorted(ms[1:])[len(ms[1:]) // 2]
     pairs = n * (n - 1) / 2
-    return pairs / (ms * 1e-3), ms, int(rechecked.value)
+    return pairs / (ms * 1e-3), ms, int(rechecked.value), sched
 
 
 def cpu_engine(start, pdc, sfc, dr, seed=7, init_from=None):
@@ -400,16 +405,20 @@ def sio2_section(args):
           flush=True)
     out = {"workload": "SiO2-like 200000 atoms (66667 Si), cube 145 A, min distance 1.5 A; PDC r[0,20]@0.01 "
                        "(2000 bins) + SFC Q[0.3,30]x1000, single-atom translation 0.1 A, k<=128"}
-    gps, fms, _ = gpu_full_histogram(start, elem, 0.01, 2000)
-    gps64, fms64, re64 = gpu_full_histogram(start, elem, 0.01, 2000, float64=True)
+    gps, fms, _, sched = gpu_full_histogram(start, elem, 0.01, 2000)
+    gps0, fms0, _, sched0 = gpu_full_histogram(start, elem, 0.01, 2000, autotune=False)
+    gps64, fms64, re64, sched64 = gpu_full_histogram(start, elem, 0.01, 2000, float64=True)
     tf = gps * OPS_PER_PAIR / 1e12
     out["full_histogram"] = {"gpairs_per_s": round(gps / 1e9, 2), "kernel_ms": round(fms, 3),
-                             "pairs": n * (n - 1) // 2,
+                             "pairs": n * (n - 1) // 2, "schedule": sched,
+                             "fixed_rule": {"gpairs_per_s": round(gps0 / 1e9, 2), "kernel_ms": round(fms0, 3),
+                                            "schedule": sched0},
                              "roofline": {"bound": "mfma", "compute": "fp32 VALU", "achieved": round(tf, 3),
                                           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                                           "frac": round(tf / PEAK_FP32_TFLOPS, 4)},
                              "float64_recompute_mode": {"gpairs_per_s": round(gps64 / 1e9, 2),
-                                                        "kernel_ms": round(fms64, 3), "rechecked_pairs": re64}}
+                                                        "kernel_ms": round(fms64, 3), "rechecked_pairs": re64,
+                                                        "schedule": sched64}}
     E = make_engine(start, pdc, sfc, seed=3, dr=0.01)
     E.run_device(500, k=128, pull=False)
     torch.cuda.synchronize()
@@ -598,10 +607,13 @@ def main():
 
     if not args.no_extras:
         # full-histogram recompute (one O(N^2) pass on the PDC grid)
-        gps, fms, _ = gpu_full_histogram(start, elem, 0.02, 1000)
+        gps, fms, _, sched = gpu_full_histogram(start, elem, 0.02, 1000)
+        gps0, fms0, _, sched0 = gpu_full_histogram(start, elem, 0.02, 1000, autotune=False)
         tf = gps * OPS_PER_PAIR / 1e12
         result["full_histogram"] = {"gpairs_per_s": round(gps / 1e9, 3), "kernel_ms": round(fms, 3),
-                                    "pairs": n * (n - 1) // 2,
+                                    "pairs": n * (n - 1) // 2, "schedule": sched,
+                                    "fixed_rule": {"gpairs_per_s": round(gps0 / 1e9, 3), "kernel_ms": round(fms0, 3),
+                                                   "schedule": sched0},
                                     "roofline": {"bound": "mfma", "compute": "fp32 VALU", "achieved": round(tf, 3),
                                                  "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                                                  "frac": round(tf / PEAK_FP32_TFLOPS, 4)}}

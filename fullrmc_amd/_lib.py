@@ -62,6 +62,9 @@ _PROTOS = {
     "frmc_last_overflow": [],
     "frmc_set_timing": [c_int],
     "frmc_last_kernel_ms": [],
+    "frmc_set_autotune": [c_int],
+    "frmc_force_full_schedule": [c_i32, c_i32, c_i32],
+    "frmc_last_full_schedule": [c_i32p, c_i32p, ctypes.POINTER(c_i64), c_i32p],
     # pairs_distances
     "frmc_pairs_distances_to_indexcoords": [c_i32, c_vp, c_i32, c_vp, c_int, c_int, c_vp, c_vp],
     "frmc_pairs_distances_to_multi_indexcoords": [c_vp, c_i32, c_vp, c_i32, c_vp, c_int, c_int, c_vp, c_vp],
@@ -130,7 +133,7 @@ _PROTOS = {
 }
 KERNEL_NAMES = ["k_delta", "k_eval_a", "k_commit", "k_eval_b"]
 _RESTYPES = {"frmc_last_error": ctypes.c_char_p, "frmc_last_overflow": c_i64, "frmc_set_timing": None,
-             "frmc_last_kernel_ms": c_f32}
+             "frmc_last_kernel_ms": c_f32, "frmc_set_autotune": None}
 
 _lib = None
 
@@ -146,6 +149,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise FrmcError("%s is not built; run __graft_entry__.build() (hipcc, gfx950)" % LIB_PATH)
+        # torch first: its HIP runtime is then the one the library binds to,
+        # so device pointers and streams from torch are valid in both
+        from . import _device
+        _device.torch()
         handle = ctypes.CDLL(LIB_PATH)
         for name, args in _PROTOS.items():
             fn = getattr(handle, name)

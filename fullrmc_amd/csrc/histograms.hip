@@ -21,6 +21,11 @@
 #include <math.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
 #include "frmc_device.h"
 #include "frmc_host.h"
 
@@ -112,12 +117,12 @@ __global__ void __launch_bounds__(256) k_multi_coords(const int32_t *__restrict_
 
 // ------------------------------------------------------------- full pairs
 
-// Work item = (row block bi of R*kFT atoms, column tile c of kFT atoms) with
-// c >= R*bi: the upper triangle of tiles, diagonal tiles included (there only
-// j > i is counted).  Items are numbered row block by row block, so row block
-// bi starts at item bi*nct - R*bi*(bi-1)/2; a workgroup takes a contiguous
-// range of items and keeps its row atoms in registers while bi is unchanged.
-constexpr int kFT = 256;
+// Work item = (row block bi of R*NT atoms, column tile c of NT atoms), NT the
+// workgroup size, with c >= R*bi: the upper triangle of tiles, diagonal tiles
+// included (there only j > i is counted).  Items are numbered row block by
+// row block, so row block bi starts at item bi*nct - R*bi*(bi-1)/2; a
+// workgroup takes a contiguous range of items and keeps its row atoms in
+// registers while bi is unchanged.
 
 struct FullSched {
     int32_t R;          // row atoms per thread
@@ -192,22 +197,22 @@ __device__ __forceinline__ int64_t edge_bin(float s, float ax, float ay, float a
     return (int64_t)kf;
 }
 
-template <int R, bool PBC, bool ORTHO, bool LDS_HIST, bool EDGE>
-__global__ void __launch_bounds__(kFT) k_full_pairs(const float *__restrict__ coords, int32_t n, Basis B,
+template <int NT, int R, bool PBC, bool ORTHO, bool LDS_HIST, bool EDGE>
+__global__ void __launch_bounds__(NT) k_full_pairs(const float *__restrict__ coords, int32_t n, Basis B,
                                                     const int32_t *__restrict__ mol,
                                                     const int32_t *__restrict__ elem, HistArgs A, FullSched S,
                                                     Edge64 E, int32_t *__restrict__ partial,
                                                     int32_t *__restrict__ counts) {
     extern __shared__ int32_t lds_hist[];  // 2*size ints when LDS_HIST
-    __shared__ float4 tile_xyzm[kFT];
-    __shared__ int32_t tile_e[kFT];
+    __shared__ float4 tile_xyzm[NT];
+    __shared__ int32_t tile_e[NT];
     __shared__ int32_t lds_overflow;
     __shared__ unsigned long long lds_rechecked;
 
     const int tid = threadIdx.x;
     const int32_t size = (int32_t)A.size;  // LDS_HIST: 2*size ints fit in LDS
     if (LDS_HIST)
-        for (int32_t s = tid; s < 2 * size; s += kFT) lds_hist[s] = 0;
+        for (int32_t s = tid; s < 2 * size; s += NT) lds_hist[s] = 0;
     if (tid == 0) {
         lds_overflow = 0;
         lds_rechecked = 0;
@@ -233,7 +238,7 @@ __global__ void __launch_bounds__(kFT) k_full_pairs(const float *__restrict__ co
             cur_bi = bi;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                ri[r] = (bi * R + r) * kFT + tid;
+                ri[r] = (bi * R + r) * NT + tid;
                 const int32_t i = min(ri[r], n - 1);  // clamped; rows >= n never count
                 rx[r] = coords[3 * (int64_t)i];
                 ry[r] = coords[3 * (int64_t)i + 1];
@@ -244,14 +249,14 @@ __global__ void __launch_bounds__(kFT) k_full_pairs(const float *__restrict__ co
         }
         __syncthreads();  // previous tile consumed (and LDS bins cleared)
         {
-            const int32_t j = min(c * kFT + tid, n - 1);
+            const int32_t j = min(c * NT + tid, n - 1);
             tile_xyzm[tid] = make_float4(coords[3 * (int64_t)j], coords[3 * (int64_t)j + 1],
                                          coords[3 * (int64_t)j + 2], __int_as_float(mol[j]));
             tile_e[tid] = elem[j];
         }
         __syncthreads();
-        const int32_t jbase = c * kFT;
-        const int32_t jn = min(kFT, n - jbase);
+        const int32_t jbase = c * NT;
+        const int32_t jn = min(NT, n - jbase);
         const bool diag = c < R * (bi + 1);  // tile overlaps the row block
 #pragma unroll 2
         for (int32_t t = 0; t < jn; t++) {
@@ -292,7 +297,7 @@ __global__ void __launch_bounds__(kFT) k_full_pairs(const float *__restrict__ co
     __syncthreads();
     if (LDS_HIST) {
         int32_t *dst = partial + (int64_t)blockIdx.x * 2 * size;
-        for (int32_t s = tid; s < 2 * size; s += kFT) dst[s] = lds_hist[s];
+        for (int32_t s = tid; s < 2 * size; s += NT) dst[s] = lds_hist[s];
     }
     if (tid == 0 && lds_overflow) atomicAdd(&counts[2 * A.size], lds_overflow);
     // EDGE: counts[2*size+1 .. +2] hold the number of double re-checks (64-bit, low word first)
@@ -420,7 +425,7 @@ static int read_basis(const float *basis, int isPBC, hipStream_t st, Basis &B, b
     return FRMC_OK;
 }
 
-template <int R, bool EDGE>
+template <int NT, int R, bool EDGE>
 static int launch_full_r(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
                          const int32_t *mol, const int32_t *elem, const HistArgs &A, FullSched S, const Edge64 &E,
                          int64_t blocks, bool lds, int32_t *partial, int32_t *counts, hipStream_t st) {
@@ -428,10 +433,10 @@ static int launch_full_r(const float *coords, int32_t n, const Basis &B, int isP
 #define FRMC_FULL_LAUNCH(P, O, L)                                                                               \
     do {                                                                                                        \
         if (shm > 48 * 1024)                                                                                    \
-            (void)hipFuncSetAttribute((const void *)k_full_pairs<R, P, O, L, EDGE>,                             \
+            (void)hipFuncSetAttribute((const void *)k_full_pairs<NT, R, P, O, L, EDGE>,                         \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                    \
-        k_full_pairs<R, P, O, L, EDGE><<<(unsigned)blocks, kFT, shm, st>>>(coords, n, B, mol, elem, A, S, E,   \
-                                                                           partial, counts);                    \
+        k_full_pairs<NT, R, P, O, L, EDGE><<<(unsigned)blocks, NT, shm, st>>>(coords, n, B, mol, elem, A, S, E,\
+                                                                              partial, counts);                 \
     } while (0)
     if (isPBC) {
         if (ortho) { if (lds) FRMC_FULL_LAUNCH(true, true, true); else FRMC_FULL_LAUNCH(true, true, false); }
@@ -444,38 +449,66 @@ static int launch_full_r(const float *coords, int32_t n, const Basis &B, int isP
     return FRMC_OK;
 }
 
-static FullSched full_sched(int32_t n, int32_t R, int64_t blocks) {
+static FullSched full_sched(int32_t n, int32_t nt, int32_t R, int64_t blocks) {
     FullSched S;
     S.R = R;
-    S.nct = (n + kFT - 1) / kFT;
-    S.nrb = (n + R * kFT - 1) / (R * kFT);
+    S.nct = (n + nt - 1) / nt;
+    S.nrb = (n + R * nt - 1) / (R * nt);
     S.items = row_block_start(S.nrb, S.nct, R);
     if (blocks > S.items) blocks = S.items;
     S.per_block = (S.items + blocks - 1) / blocks;
     return S;
 }
 
-// All unordered pairs into counts ([2][nel][nel][H] + overflow, already
-// zeroed or holding counts to add to).  Workgroups count into LDS-private
-// bins and write them to per-workgroup partials, which k_sum_partials adds
-// into counts (no contention on the global bins); histograms too large for
-// LDS count straight into global memory with atomics.
-static int launch_full_t(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
-                         const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E,
-                         int32_t *counts, hipStream_t st) {
+// ------------------------------------------------ full-histogram schedules
+
+// A schedule of the full histogram: workgroup size (= column tile), row atoms
+// per thread and resident workgroups per CU.  Every schedule gives the same
+// counts (integer atomics), so the choice only changes the time.
+struct FullChoice {
+    int32_t nt = 256, R = 2, per_cu = 2;
+};
+
+constexpr int32_t kMaxPerCu = 16;  // workgroups per CU a schedule may ask for
+
+// resident workgroups per CU for workgroup size nt: LDS (160 KB per CU, the
+// private bins plus the staged tile) and 2,048 threads per CU, at most 8
+static int32_t max_per_cu(size_t lds_bytes, bool lds, int32_t nt) {
+    int32_t by_lds = lds ? (int32_t)((156 * 1024) / (lds_bytes + 20 * (size_t)nt + 2048)) : 8;
+    int32_t p = by_lds < 2048 / nt ? by_lds : 2048 / nt;
+    return p < 1 ? 1 : (p > 8 ? 8 : p);
+}
+
+// the fixed rule used when tuning is off: 256 threads, the residency limit,
+// two row atoms per thread when that still leaves two items per workgroup
+static FullChoice heuristic_choice(int32_t n, size_t lds_bytes, bool lds, int ncu) {
+    FullChoice C;
+    C.nt = 256;
+    C.per_cu = max_per_cu(lds_bytes, lds, C.nt);
+    C.R = full_sched(n, C.nt, 2, (int64_t)ncu * C.per_cu).items < 2 * (int64_t)ncu * C.per_cu ? 1 : 2;
+    return C;
+}
+
+static int g_autotune = 1;                 // frmc_set_autotune
+static FullChoice g_forced;                // frmc_force_full_schedule (R == 0: none)
+static bool g_forced_on = false;
+static thread_local FullChoice g_last;     // frmc_last_full_schedule (per thread)
+static thread_local int64_t g_last_blocks = 0;
+static thread_local int g_last_tuned = 0;
+static std::mutex g_tune_mu;
+// shape class -> tuned schedule.  Class: device, floor(log2 n), bins
+// (2*nel*nel*H), boundary kind (0 none, 1 triclinic, 2 orthorhombic), EDGE.
+static std::map<std::tuple<int, int, int64_t, int, int>, FullChoice> g_tuned;
+
+// One full-histogram pass with schedule C: counts += all pairs' bins.
+static int run_full(const FullChoice &C, const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
+                    const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E, int ncu,
+                    int32_t *counts, hipStream_t st, int64_t *nblocks) {
     const size_t lds_bytes = sizeof(int32_t) * 2 * A.size;
     const bool lds = lds_bytes <= kMaxLdsHistBytes;
-    int dev = 0, ncu = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    // 160 KB of LDS per CU: as many resident workgroups as the private bins allow
-    int per_cu = lds ? (int)((156 * 1024) / (lds_bytes + 6 * 1024)) : 8;
-    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
-    const int64_t max_blocks = (int64_t)ncu * per_cu;
-    // two row atoms per thread when that still leaves two items per workgroup
-    FullSched S = full_sched(n, 2, max_blocks);
-    if (S.items < 2 * max_blocks) S = full_sched(n, 1, max_blocks);
+    const FullSched S = full_sched(n, C.nt, C.R, (int64_t)ncu * C.per_cu);
     const int64_t blocks = (S.items + S.per_block - 1) / S.per_block;
+    if (nblocks) *nblocks = blocks;
     int32_t *partial = nullptr;
     if (lds) {
         hipError_t e = pool_malloc((void **)&partial, lds_bytes * blocks, st);
@@ -487,17 +520,25 @@ static int launch_full_t(const float *coords, int32_t n, const Basis &B, int isP
     }
     Edge64 none;
     memset(&none, 0, sizeof(none));
-    int rc;
-    if (E)
-        rc = S.R == 2 ? launch_full_r<2, true>(coords, n, B, isPBC, ortho, mol, elem, A, S, *E, blocks, lds, partial,
-                                               counts, st)
-                      : launch_full_r<1, true>(coords, n, B, isPBC, ortho, mol, elem, A, S, *E, blocks, lds, partial,
-                                               counts, st);
-    else
-        rc = S.R == 2 ? launch_full_r<2, false>(coords, n, B, isPBC, ortho, mol, elem, A, S, none, blocks, lds,
-                                                partial, counts, st)
-                      : launch_full_r<1, false>(coords, n, B, isPBC, ortho, mol, elem, A, S, none, blocks, lds,
-                                                partial, counts, st);
+    const Edge64 &EE = E ? *E : none;
+    int rc = FRMC_EINVAL;
+#define FRMC_FULL_CASE(NT_, R_)                                                                                \
+    if (C.nt == NT_ && C.R == R_)                                                                              \
+        rc = E ? launch_full_r<NT_, R_, true>(coords, n, B, isPBC, ortho, mol, elem, A, S, EE, blocks, lds,    \
+                                              partial, counts, st)                                             \
+               : launch_full_r<NT_, R_, false>(coords, n, B, isPBC, ortho, mol, elem, A, S, EE, blocks, lds,   \
+                                               partial, counts, st);
+    FRMC_FULL_CASE(256, 1)
+    FRMC_FULL_CASE(256, 2)
+    FRMC_FULL_CASE(256, 4)
+    FRMC_FULL_CASE(512, 1)
+    FRMC_FULL_CASE(512, 2)
+    FRMC_FULL_CASE(512, 4)
+    FRMC_FULL_CASE(1024, 1)
+    FRMC_FULL_CASE(1024, 2)
+    FRMC_FULL_CASE(1024, 4)
+#undef FRMC_FULL_CASE
+    if (rc == FRMC_EINVAL) set_error("no full-histogram kernel for %d threads and %d row atoms", C.nt, C.R);
     if (rc == FRMC_OK && lds) {
         const int32_t per_split = 16;
         const int64_t len = 2 * A.size;
@@ -513,10 +554,125 @@ static int launch_full_t(const float *coords, int32_t n, const Basis &B, int isP
     return rc;
 }
 
+// Times every candidate schedule on the caller's inputs (into scratch
+// counts, so the caller's counts are untouched): each candidate runs twice
+// between HIP events on the call stream and its faster run is kept.
+static int tune_full(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho, const int32_t *mol,
+                     const int32_t *elem, const HistArgs &A, const Edge64 *E, int ncu, hipStream_t st,
+                     FullChoice &best) {
+    const size_t lds_bytes = sizeof(int32_t) * 2 * A.size;
+    const bool lds = lds_bytes <= kMaxLdsHistBytes;
+    std::vector<FullChoice> cand;
+    const FullChoice H0 = heuristic_choice(n, lds_bytes, lds, ncu);
+    cand.push_back(H0);
+    // workgroup sizes x row atoms x (1, 2, 4) times the resident workgroups
+    // per CU: more workgroups than fit at once shorten the tail of the grid
+    const int32_t nts[3] = {256, 512, 1024}, Rs[3] = {1, 2, 4};
+    for (int32_t nt : nts) {
+        const int32_t pmax = max_per_cu(lds_bytes, lds, nt);
+        for (int32_t R : Rs)
+            for (int32_t pc = pmax; pc <= 4 * pmax && pc <= kMaxPerCu; pc *= 2) {
+                FullChoice C;
+                C.nt = nt;
+                C.R = R;
+                C.per_cu = pc;
+                // a schedule with fewer items than CUs leaves the chip idle
+                if (full_sched(n, nt, R, 1).items < ncu) continue;
+                bool dup = false;
+                for (const FullChoice &D : cand) dup |= D.nt == nt && D.R == R && D.per_cu == pc;
+                if (!dup) cand.push_back(C);
+            }
+    }
+    const int64_t len = 2 * A.size + 4;
+    int32_t *scratch = nullptr;
+    hipError_t e = pool_malloc((void **)&scratch, sizeof(int32_t) * len, st);
+    if (e != hipSuccess) {
+        set_error("hipMallocAsync of %lld tuning counts failed: %s", (long long)len, hipGetErrorString(e));
+        return FRMC_ENOMEM;
+    }
+    const size_t nc = cand.size();
+    std::vector<hipEvent_t> ev(4 * nc, nullptr);
+    int rc = FRMC_OK;
+    for (size_t q = 0; q < ev.size() && rc == FRMC_OK; q++)
+        if (hipEventCreate(&ev[q]) != hipSuccess) {
+            set_error("hipEventCreate failed");
+            rc = FRMC_EHIP;
+        }
+    for (size_t c = 0; c < nc && rc == FRMC_OK; c++)
+        for (int rep = 0; rep < 2 && rc == FRMC_OK; rep++) {
+            k_fill_i32<<<64, 256, 0, st>>>(scratch, len, 0);
+            if (hipEventRecord(ev[4 * c + 2 * rep], st) != hipSuccess) rc = FRMC_EHIP;
+            if (rc == FRMC_OK)
+                rc = run_full(cand[c], coords, n, B, isPBC, ortho, mol, elem, A, E, ncu, scratch, st, nullptr);
+            if (rc == FRMC_OK && hipEventRecord(ev[4 * c + 2 * rep + 1], st) != hipSuccess) rc = FRMC_EHIP;
+            if (rc == FRMC_EHIP) set_error("hipEventRecord failed while tuning");
+        }
+    if (rc == FRMC_OK && hipStreamSynchronize(st) != hipSuccess) {
+        set_error("stream failed while tuning the full histogram");
+        rc = FRMC_EHIP;
+    }
+    float best_ms = 0.f;
+    best = H0;
+    for (size_t c = 0; c < nc && rc == FRMC_OK; c++) {
+        float m0 = 0.f, m1 = 0.f;
+        if (hipEventElapsedTime(&m0, ev[4 * c], ev[4 * c + 1]) != hipSuccess ||
+            hipEventElapsedTime(&m1, ev[4 * c + 2], ev[4 * c + 3]) != hipSuccess) {
+            set_error("hipEventElapsedTime failed while tuning");
+            rc = FRMC_EHIP;
+            break;
+        }
+        const float m = m0 < m1 ? m0 : m1;
+        if (c == 0 || m < best_ms) {
+            best_ms = m;
+            best = cand[c];
+        }
+    }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    (void)hipFreeAsync(scratch, st);
+    return rc;
+}
+
+// The full-histogram schedule: the forced one if set, else the tuned one for
+// the shape class (tuned on the first call of the class), else the fixed rule.
+static int choose_full(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
+                       const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E, int dev,
+                       int ncu, hipStream_t st, FullChoice &C, int &tuned) {
+    const size_t lds_bytes = sizeof(int32_t) * 2 * A.size;
+    const bool lds = lds_bytes <= kMaxLdsHistBytes;
+    tuned = 0;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (g_forced_on) {
+        C = g_forced;
+        return FRMC_OK;
+    }
+    if (!g_autotune) {
+        C = heuristic_choice(n, lds_bytes, lds, ncu);
+        return FRMC_OK;
+    }
+    int lg = 0;
+    while (lg < 30 && ((int64_t)1 << (lg + 1)) <= n) lg++;
+    const auto key = std::make_tuple(dev, lg, A.size, isPBC ? (ortho ? 2 : 1) : 0, E ? 1 : 0);
+    auto it = g_tuned.find(key);
+    if (it == g_tuned.end()) {
+        if (int rc = tune_full(coords, n, B, isPBC, ortho, mol, elem, A, E, ncu, st, C)) return rc;
+        g_tuned[key] = C;
+    } else {
+        C = it->second;
+    }
+    tuned = 1;
+    return FRMC_OK;
+}
+
 static int launch_full_timed(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
                              const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E,
                              int32_t *counts, hipStream_t st);
 
+// All unordered pairs into counts ([2][nel][nel][H] + overflow, already
+// zeroed or holding counts to add to).  Workgroups count into LDS-private
+// bins and write them to per-workgroup partials, which k_sum_partials adds
+// into counts (no contention on the global bins); histograms too large for
+// LDS count straight into global memory with atomics.
 int launch_full(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho, const int32_t *mol,
                 const int32_t *elem, const HistArgs &A, int32_t *counts, hipStream_t st) {
     return launch_full_timed(coords, n, B, isPBC, ortho, mol, elem, A, nullptr, counts, st);
@@ -526,6 +682,12 @@ static int launch_full_timed(const float *coords, int32_t n, const Basis &B, int
                              const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E,
                              int32_t *counts, hipStream_t st) {
     if (n < 2) return FRMC_OK;
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    FullChoice C;
+    int tuned = 0;
+    if (int rc = choose_full(coords, n, B, isPBC, ortho, mol, elem, A, E, dev, ncu, st, C, tuned)) return rc;
     hipEvent_t a = nullptr, b = nullptr;
     const bool timing = timing_enabled();
     if (timing) {
@@ -533,7 +695,8 @@ static int launch_full_timed(const float *coords, int32_t n, const Basis &B, int
         FRMC_CHECK_HIP(hipEventCreate(&b));
         FRMC_CHECK_HIP(hipEventRecord(a, st));
     }
-    int rc = launch_full_t(coords, n, B, isPBC, ortho, mol, elem, A, E, counts, st);
+    int64_t blocks = 0;
+    int rc = run_full(C, coords, n, B, isPBC, ortho, mol, elem, A, E, ncu, counts, st, &blocks);
     if (timing) {
         float ms = 0.f;
         hipError_t e = hipEventRecord(b, st);
@@ -547,6 +710,9 @@ static int launch_full_timed(const float *coords, int32_t n, const Basis &B, int
         }
         set_last_kernel_ms(ms);
     }
+    g_last = C;
+    g_last_blocks = blocks;
+    g_last_tuned = tuned;
     return rc;
 }
 
@@ -770,6 +936,39 @@ int frmc_full_pairs_counts_f64(const float *coords, int32_t n, const float *basi
         FRMC_CHECK_HIP(hipMemcpyAsync(rechecked, counts + 2 * A.size + 2, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         FRMC_CHECK_HIP(hipStreamSynchronize(st));
     }
+    return FRMC_OK;
+}
+
+void frmc_set_autotune(int enable) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_autotune = enable != 0;
+    if (enable < 0) g_tuned.clear();
+}
+
+int frmc_force_full_schedule(int32_t threads, int32_t rowAtoms, int32_t workgroupsPerCU) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (rowAtoms == 0) {
+        g_forced_on = false;
+        return FRMC_OK;
+    }
+    FRMC_REQUIRE(threads == 256 || threads == 512 || threads == 1024, "threads must be 256, 512 or 1024 (got %d)",
+                 threads);
+    FRMC_REQUIRE(rowAtoms == 1 || rowAtoms == 2 || rowAtoms == 4, "rowAtoms must be 1, 2 or 4 (got %d)", rowAtoms);
+    FRMC_REQUIRE(workgroupsPerCU >= 1 && workgroupsPerCU <= kMaxPerCu, "workgroupsPerCU must be in 1..%d (got %d)",
+                 kMaxPerCu, workgroupsPerCU);
+    g_forced.nt = threads;
+    g_forced.R = rowAtoms;
+    g_forced.per_cu = workgroupsPerCU;
+    g_forced_on = true;
+    return FRMC_OK;
+}
+
+int frmc_last_full_schedule(int32_t *threads, int32_t *rowAtoms, int64_t *workgroups, int32_t *tuned) {
+    FRMC_REQUIRE(threads && rowAtoms && workgroups && tuned, "NULL pointer argument");
+    *threads = g_last.nt;
+    *rowAtoms = g_last.R;
+    *workgroups = g_last_blocks;
+    *tuned = g_last_tuned;
     return FRMC_OK;
 }
 

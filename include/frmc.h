@@ -47,6 +47,24 @@ int64_t frmc_last_overflow(void);
 void frmc_set_timing(int enable);
 float frmc_last_kernel_ms(void);
 
+/* Full-histogram schedule autotuner.  No counterpart in the reference (its
+   pairs_histograms.pyx:86-176 has one fixed OpenMP loop); every schedule
+   gives the same counts.  A schedule is (threads per workgroup = column
+   tile: 256, 512 or 1024; row atoms per thread: 1, 2 or 4; workgroups per
+   CU: 1..16, more than fit at once allowed).  With tuning on (the
+   default) the first all-pairs call of a shape class (device, floor(log2 n),
+   bins, boundary kind, float64-recompute mode) times each candidate twice
+   with HIP events on the call's stream, outside the frmc_set_timing window,
+   and later calls of the class reuse the fastest.
+   frmc_set_autotune: 1 on, 0 off (fixed rule), -1 off and forget the
+   tuned classes.  frmc_force_full_schedule pins one schedule for every call
+   (rowAtoms == 0 releases it).  frmc_last_full_schedule reports the
+   schedule of this thread's last all-pairs call (tuned = 1 when it came
+   from the tuner). */
+void frmc_set_autotune(int enable);
+int frmc_force_full_schedule(int32_t threads, int32_t rowAtoms, int32_t workgroupsPerCU);
+int frmc_last_full_schedule(int32_t *threads, int32_t *rowAtoms, int64_t *workgroups, int32_t *tuned);
+
 /* ------------------------------------------------------------------------
  * pairs_distances  (Extensions/pairs_distances.pyx)
  * ---------------------------------------------------------------------- */
