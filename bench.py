@@ -303,12 +303,22 @@ def replicas_section(args, boxes):
     moves = sum(st["generated"] for st in stats)
     sizes = [len(start["real"]) for start, _ in reps]
     out = {"workload": "%d independent single-element boxes, N log-uniform in [1000, 20000] (mean %.0f), density "
-                       "0.05/A^3, PDC r[0,15]@0.05, translation 0.1 A, k<=32, own random streams per replica"
+                       "0.05/A^3, PDC r[0,15]@0.05, translation 0.1 A, k<=32, own random streams per replica; "
+                       "all replicas' windows of a step in one launch per kernel"
                        % (len(reps), float(np.mean(sizes))),
            "moves_per_s": round(moves / el, 1), "moves": moves, "seconds": round(el, 3),
            "accepted_fraction": round(sum(st["accepted"] for st in stats) / moves, 4),
            "windows": sum(st["windows"] for st in stats)}
     print("bench: replicas done (%.1f moves/s)" % out["moves_per_s"], file=sys.stderr, flush=True)
+    # the same replicas with one stream per engine (a window's kernels launched per replica)
+    m2 = max(50, args.replica_moves // 4)
+    t1 = time.perf_counter()
+    stats2 = run_replicas(engines, m2, k=32, batched=False)
+    torch.cuda.synchronize()
+    el2 = time.perf_counter() - t1
+    moves2 = sum(st["generated"] for st in stats2)
+    out["per_engine_streams"] = {"moves_per_s": round(moves2 / el2, 1), "moves": moves2, "seconds": round(el2, 3)}
+    print("bench: replicas on per-engine streams (%.1f moves/s)" % (moves2 / el2), file=sys.stderr, flush=True)
     if not args.no_cpu:
         # the oracle engine on four replicas spread over the size range, equal time each
         order = np.argsort(sizes)
@@ -526,7 +536,7 @@ def main():
                     help="also run the 200,000-atom SiO2 box (full histograms and k=128 moves)")
     ap.add_argument("--sio2-moves", type=int, default=3000)
     ap.add_argument("--replicas", type=int, default=256, help="independent replicas run together (0: skip)")
-    ap.add_argument("--replica-moves", type=int, default=600, help="generated moves per replica")
+    ap.add_argument("--replica-moves", type=int, default=3000, help="generated moves per replica")
     ap.add_argument("--molecular-moves", type=int, default=1000,
                     help="moves of the 6,500-atom molecular triclinic box (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -587,7 +587,7 @@ def _mt_to_numpy(m):
     return ("MT19937", np.array([m.mt[q] for q in range(624)], dtype=np.uint32), int(m.pos), 0, 0.0)
 
 
-def run_replicas(engines, numberOfSteps=100, k=32, seeds=None):
+def run_replicas(engines, numberOfSteps=100, k=32, seeds=None, batched=True):
     """Advance independent engines (replicas) by numberOfSteps generated
     moves each, with every replica's windows evaluated concurrently on the
     GPU.  Each replica runs exactly as ``random.seed(s); numpy.random.seed(s);
@@ -595,7 +595,9 @@ def run_replicas(engines, numberOfSteps=100, k=32, seeds=None):
     pair of random streams: ``seeds`` (one per engine) starts them, and
     without seeds each engine continues the streams of its previous replica
     run.  Every engine needs translation moves on all groups (the native
-    loop) and the same generator amplitude and tolerance.  Returns one stats
+    loop) and the same generator amplitude and tolerance.  ``batched``: with
+    single-atom groups, all replicas' windows of a step go out in one launch
+    per kernel; False gives each engine its own stream.  Returns one stats
     dict per engine."""
     engines = list(engines)
     if not engines:
@@ -624,8 +626,12 @@ def run_replicas(engines, numberOfSteps=100, k=32, seeds=None):
             raise ValueError("engine %d has no replica random streams: pass seeds" % r)
         pys[r] = _mt_from_python(E._replica_rng[0])
         nps[r] = _mt_from_numpy(E._replica_rng[1])
-    _lib.check(_lib.lib().frmc_engines_run_translation(handles, R, int(numberOfSteps), int(k), amp[0], amp[1], tol,
-                                                       pys, nps, stats))
+    _lib.lib().frmc_set_replica_batching(1 if batched else 0)
+    try:
+        _lib.check(_lib.lib().frmc_engines_run_translation(handles, R, int(numberOfSteps), int(k), amp[0], amp[1],
+                                                           tol, pys, nps, stats))
+    finally:
+        _lib.lib().frmc_set_replica_batching(1)
     out = []
     for r, E in enumerate(engines):
         E._replica_rng = (_mt_to_python(pys[r], E._replica_rng[0]), _mt_to_numpy(nps[r]))

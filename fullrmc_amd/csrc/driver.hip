@@ -13,6 +13,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <atomic>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -151,9 +153,21 @@ struct TranslationRun {
 
     bool done() const { return step >= nsteps; }
 
+    // proposals in the next window
+    int32_t next_k() const {
+        const int32_t kk = e->adaptive ? window_size(gen0, acc0, k) : k;
+        return (int32_t)std::min<int64_t>(kk, nsteps - step);
+    }
+
     int launch() {
-        kw = e->adaptive ? window_size(gen0, acc0, k) : k;
-        kw = (int32_t)std::min<int64_t>(kw, nsteps - step);
+        kw = next_k();
+        draw();
+        return launch_device_decided(e, kw, gids.data(), nbox.data(), uf.data(), tolerance);
+    }
+
+    // draw the kw proposals of the next window from both streams (saved first
+    // so that finish() can rewind them)
+    void draw() {
         py0 = *py_state;
         np0 = *np_state;
         Stream py{py_state, 0}, np_{np_state, 0};
@@ -185,7 +199,6 @@ struct TranslationRun {
         }
         aoff[kw] = off;
         for (int32_t p = 0; p < kw; p++) uf[p] = (float)uacc[p];
-        return launch_device_decided(e, kw, gids.data(), nbox.data(), uf.data(), tolerance);
     }
 
     int finish() {
@@ -241,9 +254,185 @@ struct TranslationRun {
     }
 };
 
+static int g_batch_replicas = 1;  // frmc_set_replica_batching
+
+// Fixed worker threads for the host halves of a batched step: run(f) calls
+// f(t) on every thread t in [0, T) (t = 0 on the caller) and returns when all
+// have finished.  Workers spin between steps (a step is tens of microseconds).
+class StepPool {
+  public:
+    explicit StepPool(int32_t T) : T_(T) {
+        for (int32_t t = 1; t < T_; t++) th_.emplace_back([this, t] { loop(t); });
+    }
+    ~StepPool() {
+        stop_.store(true);
+        gen_.fetch_add(1);
+        for (auto &x : th_) x.join();
+    }
+    void run(const std::function<void(int32_t)> &f) {
+        job_ = &f;
+        left_.store(T_ - 1);
+        gen_.fetch_add(1);
+        f(0);
+        while (left_.load() > 0) std::this_thread::yield();
+    }
+
+  private:
+    void loop(int32_t t) {
+        uint64_t seen = 0;
+        for (;;) {
+            uint64_t g;
+            while ((g = gen_.load()) == seen) std::this_thread::yield();
+            seen = g;
+            if (stop_.load()) return;
+            (*job_)(t);
+            left_.fetch_sub(1);
+        }
+    }
+    int32_t T_;
+    std::vector<std::thread> th_;
+    std::atomic<uint64_t> gen_{0};
+    std::atomic<int32_t> left_{0};
+    std::atomic<bool> stop_{false};
+    const std::function<void(int32_t)> *job_ = nullptr;
+};
+
+static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// All replicas' windows in one launch per kernel per step: the host draws
+// every active replica's proposals (threads over replicas), uploads windows,
+// proposals and block prefixes in one copy, launches k_delta_batch,
+// k_eval_a_batch and k_eval_b_batch on the first engine's stream, then walks
+// each replica's totals with the reference's rule exactly as the single
+// engine loop does.  Each replica keeps its own state, streams and deferred
+// commit, so every trajectory is the one its engine alone would produce.
+static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
+    const int32_t R = (int32_t)runs.size();
+    hipStream_t st = runs[0].e->st;
+    // the engines' own streams may still hold work on their buffers
+    for (auto &r : runs) {
+        if (int rc = flush_commit(r.e)) return rc;
+        FRMC_CHECK_HIP(hipStreamSynchronize(r.e->st));
+    }
+    std::vector<View> hv(R);
+    size_t shm_a = 0, shm_b = 0;
+    int32_t kmax = 1;
+    for (int32_t r = 0; r < R; r++) {
+        hv[r] = make_view(runs[r].e, 0);
+        shm_a = std::max(shm_a, eval_lds_bytes(hv[r]));
+        shm_b = std::max(shm_b, eval_b_lds_bytes(hv[r]));
+        kmax = std::max(kmax, runs[r].k);
+    }
+    const size_t pre_b = align16(sizeof(int32_t) * (R + 1)), win_b = align16(sizeof(BatchWin) * R);
+    const size_t head = 3 * pre_b + win_b;  // prefixes, windows, then the proposal table
+    const size_t bytes = head + sizeof(Prop) * (size_t)R * kmax;
+    View *d_views = nullptr;
+    void *d_blk = nullptr, *h_blk = nullptr;
+    struct Free {
+        hipStream_t st;
+        View *&a;
+        void *&b, *&c;
+        ~Free() {
+            (void)hipStreamSynchronize(st);  // nothing in flight reads the blocks
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+            if (c) (void)hipHostFree(c);
+        }
+    } guard{st, d_views, d_blk, h_blk};
+    FRMC_CHECK_HIP(hipMalloc((void **)&d_views, sizeof(View) * R));
+    FRMC_CHECK_HIP(hipMalloc(&d_blk, bytes));
+    FRMC_CHECK_HIP(hipHostMalloc(&h_blk, bytes, hipHostMallocDefault));
+    FRMC_CHECK_HIP(hipMemcpyAsync(d_views, hv.data(), sizeof(View) * R, hipMemcpyHostToDevice, st));
+    char *hb = (char *)h_blk, *db = (char *)d_blk;
+    int32_t *hpre[3] = {(int32_t *)hb, (int32_t *)(hb + pre_b), (int32_t *)(hb + 2 * pre_b)};
+    BatchWin *hw = (BatchWin *)(hb + 3 * pre_b);
+    Prop *hp = (Prop *)(hb + head);
+    BatchArgs A;
+    A.views = d_views;
+    A.pre_d = (const int32_t *)db;
+    A.pre_a = (const int32_t *)(db + pre_b);
+    A.pre_b = (const int32_t *)(db + 2 * pre_b);
+    A.win = (const BatchWin *)(db + 3 * pre_b);
+    A.props = (const Prop *)(db + head);
+
+    std::vector<int32_t> act(R), kws(R), poff(R);
+    std::vector<int> rcs(T, FRMC_OK);
+    std::vector<std::string> errs(T);
+    int32_t nact = 0;
+    auto fail = [&](int32_t t, int rc) {
+        if (rc && !rcs[t]) {
+            rcs[t] = rc;
+            errs[t] = frmc_last_error();
+        }
+    };
+    const std::function<void(int32_t)> prepare = [&](int32_t t) {
+        for (int32_t i = t; i < nact && !rcs[t]; i += T) {
+            TranslationRun &run = runs[act[i]];
+            run.kw = kws[i];
+            run.draw();
+            BatchWin &W = hw[i];
+            W.rep = act[i];
+            W.prop = poff[i];
+            fail(t, batch_prepare(run.e, run.kw, run.gids.data(), run.nbox.data(), W, hp + poff[i]));
+        }
+    };
+    const std::function<void(int32_t)> finish = [&](int32_t t) {
+        for (int32_t i = t; i < nact && !rcs[t]; i += T) {
+            fail(t, runs[act[i]].finish());
+            runs[act[i]].kw = 0;
+        }
+    };
+    StepPool pool(T);
+    for (;;) {
+        nact = 0;
+        int32_t off = 0;
+        for (int32_t r = 0; r < R; r++)
+            if (!runs[r].done()) {
+                act[nact] = r;
+                kws[nact] = runs[r].next_k();
+                poff[nact] = off;
+                off += kws[nact];
+                nact++;
+            }
+        if (nact == 0) break;
+        pool.run(prepare);
+        for (int32_t t = 0; t < T; t++)
+            if (rcs[t]) {
+                set_error("%s", errs[t].c_str());
+                return rcs[t];
+            }
+        hpre[0][0] = hpre[1][0] = hpre[2][0] = 0;
+        int64_t tot[3] = {0, 0, 0};
+        for (int32_t i = 0; i < nact; i++) {
+            const View &V = hv[act[i]];
+            const int32_t k = kws[i];
+            tot[0] += (int64_t)((V.n + 255) / 256) * ((k + kDP - 1) / kDP);
+            tot[1] += (int64_t)V.nunits * k;
+            tot[2] += k;
+            for (int q = 0; q < 3; q++) hpre[q][i + 1] = (int32_t)tot[q];
+        }
+        FRMC_REQUIRE(tot[0] < ((int64_t)1 << 31) && tot[1] < ((int64_t)1 << 31), "batched grid too large");
+        A.nact = nact;
+        FRMC_CHECK_HIP(hipMemcpyAsync(d_blk, h_blk, head + sizeof(Prop) * (size_t)off, hipMemcpyHostToDevice, st));
+        if (int rc = launch_batch(A, (int32_t)tot[0], (int32_t)tot[1], (int32_t)tot[2], shm_a, shm_b, st)) return rc;
+        pool.run(finish);
+        for (int32_t t = 0; t < T; t++)
+            if (rcs[t]) {
+                set_error("%s", errs[t].c_str());
+                return rcs[t];
+            }
+    }
+    FRMC_CHECK_HIP(hipStreamSynchronize(st));
+    for (auto &r : runs)
+        if (int rc = finish_windows(r.e)) return rc;
+    return FRMC_OK;
+}
+
 }  // namespace frmc
 
 using namespace frmc;
+
+extern "C" void frmc_set_replica_batching(int enable) { g_batch_replicas = enable != 0; }
 
 extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float minAmp, float maxAmp,
                                            float tolerance, frmc_mt19937 *py_state, frmc_mt19937 *np_state,
@@ -301,6 +490,9 @@ extern "C" int frmc_engines_run_translation(frmc_engine **engines, int32_t nengi
     const int32_t T = std::max<int32_t>(1, std::min<int32_t>({nengines, kReplicaThreads, hw}));
     for (int32_t r = 1; r < nengines; r++)
         FRMC_REQUIRE(engines[r]->device == engines[0]->device, "replicas must share one device");
+    bool batchable = g_batch_replicas != 0;
+    for (int32_t r = 0; r < nengines; r++) batchable &= engines[r]->gmax == 1;
+    if (batchable) return run_batched(runs, T);
     std::vector<int> rcs(T, FRMC_OK);
     std::vector<std::string> errs(T);
     auto work = [&](int32_t t) {

@@ -127,6 +127,30 @@ struct CommitCtl {
 
 constexpr int kDP = 4;  // proposals per k_delta block
 
+// Windows of many engines (replicas) launched together, one launch per kernel
+// (single-atom moves).  The views are uploaded once per run; each step
+// uploads the active replicas' windows, their proposals and the block
+// prefixes of the three kernels in one copy.
+struct Prop {  // one single-atom proposal: atom, element, old and new box coordinates
+    int32_t atom, el;
+    float old[3], nw[3];
+};
+struct BatchWin {
+    int32_t rep;             // replica (index into the views)
+    int32_t k, prop;         // proposals, first of them in the proposal table
+    uint32_t seq;            // the window's number (published with the results)
+    int32_t *dn, *dn_zero;   // this window's delta buffer, the buffer it zeroes
+    int64_t zero_len;
+    Pending P;               // the replica's accepted move still to commit
+};
+struct BatchArgs {
+    const View *views;
+    const BatchWin *win;     // [nact]
+    const Prop *props;
+    const int32_t *pre_d, *pre_a, *pre_b;  // [nact+1] first block of each window per kernel
+    int32_t nact;
+};
+
 // launchers (engine_kernels.hip)
 int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hipStream_t st);
 int launch_eval(const View &V, int32_t k, hipStream_t st);
@@ -136,8 +160,11 @@ int launch_symmetrise(int32_t *base, const int32_t *slot, int32_t nel, int32_t H
 int launch_fill(int32_t *p, int64_t n, hipStream_t st);
 int launch_full(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho, const int32_t *mol,
                 const int32_t *elem, const HistArgs &A, int32_t *counts, hipStream_t st);
+int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t shm_a, size_t shm_b,
+                 hipStream_t st);
 void pairwise_plan(int32_t n, std::vector<int32_t> &leaves, std::vector<int32_t> &prog);
 size_t eval_lds_bytes(const View &V);
+size_t eval_b_lds_bytes(const View &V);
 
 }  // namespace frmc
 
@@ -201,6 +228,8 @@ int launch_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, c
                           const float *u, float tol);
 int finish_device_decided(frmc_engine *e, int32_t k, float *tse_out);
 int flush_commit(frmc_engine *e);
+int batch_prepare(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box, BatchWin &W,
+                  Prop *props);
 int settle_window(frmc_engine *e, int32_t p);
 int finish_windows(frmc_engine *e);
 void collect_times(frmc_engine *e);

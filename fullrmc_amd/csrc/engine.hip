@@ -261,9 +261,11 @@ static int run_window(frmc_engine *e, int s, int32_t k, const Inline *I, const C
 // launch their evaluation.  With u (acceptance draws) the device also
 // commits the proposal the reference's rule accepts, queued behind the
 // evaluation.  complete_eval waits for the totals.
-static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
-                            const float *u = nullptr, float tol = 0.0f) {
-    const int s = 1 - e->cur_stage;
+// the host half of staging: group ids, old (host mirror) and new box
+// coordinates into the host block of the staging buffer not read by an
+// in-flight commit, which becomes the current one
+static int stage_host(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box, int &s) {
+    s = 1 - e->cur_stage;
     Stage &S = e->stage[s];
     int32_t off = 0;
     for (int32_t p = 0; p < k; p++) {
@@ -281,6 +283,14 @@ static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids,
     S.poff[k] = off;
     memcpy(S.pnew, new_box, sizeof(float) * 3 * (size_t)off);
     e->cur_stage = s;
+    return FRMC_OK;
+}
+
+static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
+                            const float *u = nullptr, float tol = 0.0f) {
+    int s = 0;
+    if (int rc0 = stage_host(e, k, group_ids, new_box, s)) return rc0;
+    Stage &S = e->stage[s];
     Inline &I = e->last_inline;
     I.k = 0;
     const bool inl = e->gmax == 1 && k <= kInline;
@@ -327,6 +337,44 @@ static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids,
         e->device_decided = false;
     }
     return rc;
+}
+
+// A single-atom window of the native loop for a batched launch: the same
+// staging, window bookkeeping and deferred commit as stage_and_launch, with
+// the proposals written to props and the window to W instead of launched.
+int batch_prepare(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box, BatchWin &W,
+                  Prop *props) {
+    FRMC_REQUIRE(e->gmax == 1 && k >= 1 && k <= e->kmax, "batched windows need single-atom groups and 1 <= k <= %d",
+                 e->kmax);
+    int s = 0;
+    if (int rc = stage_host(e, k, group_ids, new_box, s)) return rc;
+    for (int32_t p = 0; p < k; p++) {
+        const int32_t id = e->h_gatoms[e->h_goff[group_ids[p]]];
+        Prop &q = props[p];
+        q.atom = id;
+        q.el = e->h_elem[id];
+        for (int d = 0; d < 3; d++) {
+            q.old[d] = e->h_box[3 * (size_t)id + d];
+            q.nw[d] = new_box[3 * (size_t)p + d];
+        }
+    }
+    e->last_inline.k = 0;
+    e->win_dn = e->dnbuf[e->cur_dn];
+    // launch_window's bookkeeping
+    e->seq++;
+    W.k = k;
+    W.seq = e->seq;
+    W.dn = e->dnbuf[e->cur_dn];
+    W.dn_zero = e->dnbuf[1 - e->cur_dn];
+    W.zero_len = (int64_t)e->dn_used[1 - e->cur_dn] * e->dnStride;
+    W.P = e->pend;
+    e->pend.valid = 0;
+    e->dn_used[1 - e->cur_dn] = 0;
+    e->dn_used[e->cur_dn] = k;
+    e->cur_dn = 1 - e->cur_dn;
+    e->device_decided = false;
+    e->deferred = true;
+    return FRMC_OK;
 }
 
 // Apply an accepted move still left for a next window's k_delta, as a

@@ -136,30 +136,33 @@ __device__ __forceinline__ void commit_pair(const View &V, float s, int32_t ea, 
 // first -- base += its delta block (grid-stride), its slot counts (blocks of
 // the first proposal row), its new coordinates (one thread) -- and every
 // thread reads the moved atom at its new position.
-template <bool PBC, bool ORTHO>
-__global__ void __launch_bounds__(256) k_delta(View V, Inline I, Pending P, int32_t k) {
-    __shared__ GroupLds G[kDP];
+// Block (bx, by) of a (gx, gy) grid; dnw is the window's delta buffer; S
+// gives the proposals of a single-atom window (S.inl()), else they come from
+// the staging buffers through G.
+template <bool PBC, bool ORTHO, class Src>
+__device__ __forceinline__ void delta_body(const View &V, const Src &S, const Pending &P, int32_t *dnw, int32_t k,
+                                           int32_t bx, int32_t by, int32_t gx, int32_t gy, GroupLds *G) {
     const int tid = threadIdx.x;
-    const int32_t p0 = blockIdx.y * kDP;
+    const int32_t p0 = by * kDP;
     const int32_t np = min(kDP, k - p0);
-    const bool inl = I.k > 0;
+    const bool inl = S.inl();
     if (!inl) {
         for (int32_t q = 0; q < np; q++) load_group(V, p0 + q, G[q]);
         __syncthreads();
     }
-    const int32_t j = blockIdx.x * blockDim.x + tid;
+    const int32_t j = bx * blockDim.x + tid;
     const bool jok = j < V.n;
     const int32_t jj = min(j, V.n - 1);  // clamped: loads stay in bounds
     float cx = V.xyz[3 * (int64_t)jj], cy = V.xyz[3 * (int64_t)jj + 1], cz = V.xyz[3 * (int64_t)jj + 2];
     const int32_t ej = V.elem[jj];
     if (P.valid) {
-        const int64_t gt = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + tid;
-        const int64_t gs = (int64_t)gridDim.x * gridDim.y * blockDim.x;
+        const int64_t gt = ((int64_t)by * gx + bx) * blockDim.x + tid;
+        const int64_t gs = (int64_t)gx * gy * blockDim.x;
         for (int32_t ci = 0; ci < V.C; ci++) {
             const DevC &c = V.c[ci];
             for (int64_t s = gt; s < (int64_t)V.T * c.H; s += gs) c.base[s] += P.dn[c.dnOff + s];
         }
-        if (blockIdx.y == 0 && jok && j != P.atom) {
+        if (by == 0 && jok && j != P.atom) {
             const float so = pair_s<PBC, ORTHO>(P.old[0], P.old[1], P.old[2], cx, cy, cz, V.B);
             const float sn = pair_s<PBC, ORTHO>(P.nw[0], P.nw[1], P.nw[2], cx, cy, cz, V.B);
             const bool intra = V.mol[P.atom] == V.mol[j];
@@ -171,20 +174,20 @@ __global__ void __launch_bounds__(256) k_delta(View V, Inline I, Pending P, int3
             cy = P.nw[1];
             cz = P.nw[2];
         }
-        if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0)
+        if (bx == 0 && by == 0 && tid == 0)
             for (int d = 0; d < 3; d++) V.xyz[3 * (int64_t)P.atom + d] = P.nw[d];
     }
     for (int32_t q = 0; q < np; q++) {
         const int32_t p = p0 + q;
-        int32_t *out = V.dn + (int64_t)p * V.dnStride;
+        int32_t *out = dnw + (int64_t)p * V.dnStride;
         if (inl) {
-            // single-atom proposal: everything about the moved atom comes in the
-            // kernel arguments
-            const int32_t id = I.atom[p];
-            const int32_t ea = I.el[p];
+            // single-atom proposal: everything about the moved atom comes with
+            // the proposal (kernel arguments or the batch's proposal table)
+            const int32_t id = S.atom(p);
+            const int32_t ea = S.el(p);
             // M counts (moved atom a, other j) with diff = c[a] - c[j], before and after
-            const float so = pair_s<PBC, ORTHO>(I.old[p][0], I.old[p][1], I.old[p][2], cx, cy, cz, V.B);
-            const float sn = pair_s<PBC, ORTHO>(I.nw[p][0], I.nw[p][1], I.nw[p][2], cx, cy, cz, V.B);
+            const float so = pair_s<PBC, ORTHO>(S.old(p, 0), S.old(p, 1), S.old(p, 2), cx, cy, cz, V.B);
+            const float sn = pair_s<PBC, ORTHO>(S.nw(p, 0), S.nw(p, 1), S.nw(p, 2), cx, cy, cz, V.B);
             if (jok && j != id) delta_pair<PBC, ORTHO>(V, out, so, sn, ea, ej);
             continue;
         }
@@ -201,7 +204,7 @@ __global__ void __launch_bounds__(256) k_delta(View V, Inline I, Pending P, int3
         }
         // group-internal pairs: M - F leaves each unordered pair once, in slot
         // [e(later position), e(earlier position)]
-        if (blockIdx.x == 0) {
+        if (bx == 0) {
             for (int32_t pq = tid; pq < g * (g - 1) / 2; pq += blockDim.x) {
                 int32_t a, b;
                 decode_pair(pq, a, b);
@@ -213,6 +216,34 @@ __global__ void __launch_bounds__(256) k_delta(View V, Inline I, Pending P, int3
             }
         }
     }
+}
+
+// proposals of a single-atom window in the kernel arguments (k > 0)
+struct InlineSrc {
+    const Inline &I;
+    __device__ bool inl() const { return I.k > 0; }
+    __device__ int32_t atom(int32_t p) const { return I.atom[p]; }
+    __device__ int32_t el(int32_t p) const { return I.el[p]; }
+    __device__ float old(int32_t p, int d) const { return I.old[p][d]; }
+    __device__ float nw(int32_t p, int d) const { return I.nw[p][d]; }
+};
+
+// proposals of a replica in the batch's proposal table
+struct PropSrc {
+    const Prop *pp;
+    __device__ bool inl() const { return true; }
+    __device__ int32_t atom(int32_t p) const { return pp[p].atom; }
+    __device__ int32_t el(int32_t p) const { return pp[p].el; }
+    __device__ float old(int32_t p, int d) const { return pp[p].old[d]; }
+    __device__ float nw(int32_t p, int d) const { return pp[p].nw[d]; }
+};
+
+// grid (ceil(n/256), ceil(k/kDP))
+template <bool PBC, bool ORTHO>
+__global__ void __launch_bounds__(256) k_delta(View V, Inline I, Pending P, int32_t k) {
+    __shared__ GroupLds G[kDP];
+    const InlineSrc S{I};
+    delta_body<PBC, ORTHO>(V, S, P, V.dn, k, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, G);
 }
 
 // ------------------------------------------------------ pairwise summation
@@ -297,22 +328,21 @@ __device__ float block_pairwise(const float *a, const DevC &c, float *scratch, i
 // The S(Q) sum is a chain of H dependent adds per Q: the kernel is bound by
 // that latency, so each wave carries one chain per lane and the grid spreads
 // the (Q chunk, proposal) pairs over the CUs.
-__global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
-    extern __shared__ float sm[];
+// work unit u of proposal p; dnw is the window's delta buffer, sm the
+// block's dynamic LDS (eval_lds_bytes)
+__device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, int32_t p, int32_t u, float *sm) {
     const int32_t Hp = (V.Hmax + 3) & ~3;        // keeps float4 reads aligned
     float *Gs = sm;                              // [Hp] G(r) or PDC squared deviations
     float *scratch = Gs + Hp;                    // [maxLeaves + kStack]
     int32_t *lprog = (int32_t *)(scratch + V.maxLeaves + kStack);  // [2*maxLeaves]
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
-    const int32_t p = blockIdx.y;
-    const int32_t u = blockIdx.x;
     int32_t ci = 0;
     while (ci + 1 < V.C && u >= V.c[ci + 1].uOff) ci++;
     // constraint fields in registers: keeps every load below unconditional
     const DevC c = V.c[ci];
     const int32_t H = c.H, M = c.M, T = V.T;
     const bool rsp = c.rspace;
-    const int32_t *dn = V.dn + (int64_t)p * V.dnStride + c.dnOff;
+    const int32_t *dn = dnw + (int64_t)p * V.dnStride + c.dnOff;
     const int32_t *base = c.base;
     // S(Q) for q = qc*kQC + lane (SFC units): the first rows of the matrix
     // column and the experimental value load now, under the G(r) phase
@@ -478,6 +508,12 @@ __global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
     }
 }
 
+// grid (nunits, k)
+__global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
+    extern __shared__ float sm[];
+    eval_a_body(V, V.dn, blockIdx.y, blockIdx.x, sm);
+}
+
 // --------------------------------------------------------------- k_eval_b
 
 // A result for the host: the value and the window's sequence number in one
@@ -488,15 +524,15 @@ __device__ __forceinline__ void publish(unsigned long long *w, float v, uint32_t
     __hip_atomic_store(w, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-__global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
-    extern __shared__ float sm[];  // Mmax + maxLeaves + kStack, then 2*maxLeaves ints
+// proposal p of a k-block grid: the SFC chi-squares and the total, published
+// with the window's number seq; the blocks also zero zero_len ints of dn_zero
+__device__ __forceinline__ void eval_b_body(const View &V, int32_t *dn_zero, int64_t zero_len, uint32_t seq,
+                                            int32_t p, int32_t nb, float *sm) {
     float *scratch = sm + V.Mmax;
     int32_t *lprog = (int32_t *)(scratch + V.maxLeaves + kStack);
-    const int32_t p = blockIdx.x;
     // zero a slice of the other delta buffer for the next window
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < V.zero_len;
-         s += (int64_t)gridDim.x * blockDim.x)
-        V.dn_zero[s] = 0;
+    for (int64_t s = (int64_t)p * blockDim.x + threadIdx.x; s < zero_len; s += (int64_t)nb * blockDim.x)
+        dn_zero[s] = 0;
     float tse = 0.0f;
     for (int32_t ci = 0; ci < V.C; ci++) {
         const DevC &c = V.c[ci];
@@ -518,13 +554,69 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
         // than 8 terms are summed sequentially from 0
         if (threadIdx.x == 0) {
             tse = tse + chi;
-            publish(V.out_res + (int64_t)p * (V.C + 1) + 1 + ci, chi, V.seq);
+            publish(V.out_res + (int64_t)p * (V.C + 1) + 1 + ci, chi, seq);
         }
     }
     if (threadIdx.x == 0) {
         V.tse_dev[p] = tse;
-        publish(V.out_res + (int64_t)p * (V.C + 1), tse, V.seq);
+        publish(V.out_res + (int64_t)p * (V.C + 1), tse, seq);
     }
+}
+
+// grid k (Mmax + maxLeaves + kStack floats, then 2*maxLeaves ints of LDS)
+__global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
+    extern __shared__ float sm[];
+    eval_b_body(V, V.dn_zero, V.zero_len, V.seq, blockIdx.x, gridDim.x, sm);
+}
+
+// ----------------------------------------------------------- batched windows
+
+// Windows of many engines (replicas) in one launch per kernel: the blocks of
+// each active replica's window are numbered consecutively, pre[i] being the
+// first block of active replica i (pre[nact] = the grid).  A block finds its
+// replica by binary search and runs the single-engine kernel's body on that
+// replica's view, window and proposals.
+__device__ __forceinline__ int32_t find_slot(const int32_t *pre, int32_t nact, int32_t b) {
+    int32_t lo = 0, hi = nact;  // pre[lo] <= b < pre[hi]
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_delta_batch(BatchArgs A) {
+    const int32_t i = find_slot(A.pre_d, A.nact, blockIdx.x);
+    const BatchWin &W = A.win[i];
+    const View &V = A.views[W.rep];
+    const int32_t gx = (V.n + 255) / 256, gy = (W.k + kDP - 1) / kDP;
+    const int32_t local = blockIdx.x - A.pre_d[i];
+    const int32_t bx = local % gx, by = local / gx;
+    const PropSrc S{A.props + W.prop};
+    if (V.isPBC) {
+        if (V.ortho) delta_body<true, true>(V, S, W.P, W.dn, W.k, bx, by, gx, gy, nullptr);
+        else delta_body<true, false>(V, S, W.P, W.dn, W.k, bx, by, gx, gy, nullptr);
+    } else {
+        delta_body<false, false>(V, S, W.P, W.dn, W.k, bx, by, gx, gy, nullptr);
+    }
+}
+
+__global__ void __launch_bounds__(64 * kEW) k_eval_a_batch(BatchArgs A) {
+    extern __shared__ float sm[];
+    const int32_t i = find_slot(A.pre_a, A.nact, blockIdx.x);
+    const BatchWin &W = A.win[i];
+    const View &V = A.views[W.rep];
+    const int32_t local = blockIdx.x - A.pre_a[i];
+    eval_a_body(V, W.dn, local / V.nunits, local % V.nunits, sm);
+}
+
+__global__ void __launch_bounds__(256) k_eval_b_batch(BatchArgs A) {
+    extern __shared__ float sm[];
+    const int32_t i = find_slot(A.pre_b, A.nact, blockIdx.x);
+    const BatchWin &W = A.win[i];
+    const View &V = A.views[W.rep];
+    eval_b_body(V, W.dn_zero, W.zero_len, W.seq, blockIdx.x - A.pre_b[i], W.k, sm);
 }
 
 // ---------------------------------------------------------------- k_commit
@@ -663,12 +755,33 @@ int launch_eval(const View &V, int32_t k, hipStream_t st) {
 }
 
 int launch_eval_b(const View &V, int32_t k, hipStream_t st) {
-    const size_t shm_b = sizeof(float) * (V.Mmax + V.maxLeaves + kStack + (size_t)2 * V.maxLeaves);
+    const size_t shm_b = eval_b_lds_bytes(V);
     if (shm_b > 48 * 1024)
         (void)hipFuncSetAttribute((const void *)k_eval_b, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_b);
     k_eval_b<<<(unsigned)k, 256, shm_b, st>>>(V, k);
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;
+}
+
+int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t shm_a, size_t shm_b,
+                 hipStream_t st) {
+    k_delta_batch<<<(unsigned)nd, 256, 0, st>>>(A);
+    FRMC_CHECK_LAUNCH();
+    if (shm_a > 48 * 1024)
+        (void)hipFuncSetAttribute((const void *)k_eval_a_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm_a);
+    k_eval_a_batch<<<(unsigned)na, 64 * kEW, shm_a, st>>>(A);
+    FRMC_CHECK_LAUNCH();
+    if (shm_b > 48 * 1024)
+        (void)hipFuncSetAttribute((const void *)k_eval_b_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm_b);
+    k_eval_b_batch<<<(unsigned)nb, 256, shm_b, st>>>(A);
+    FRMC_CHECK_LAUNCH();
+    return FRMC_OK;
+}
+
+size_t eval_b_lds_bytes(const View &V) {
+    return sizeof(float) * (V.Mmax + V.maxLeaves + kStack + (size_t)2 * V.maxLeaves);
 }
 
 int launch_commit(const View &V, const Inline &I, const CommitCtl &C, hipStream_t st) {

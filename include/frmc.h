@@ -396,14 +396,19 @@ int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float
 /* Independent replicas (no reference counterpart: many separate runs of
    Engine.run, Engine.py:3377-3516, each with its own engine and its own pair
    of random streams): every engine advances by nsteps generated moves
-   exactly as frmc_engine_run_translation would advance it alone, while the
-   windows of all engines are in flight together on their own streams (the
-   host draws and launches every engine's window, then finishes them in
-   turn).  py_states, np_states and stats are arrays of nengines entries
-   (stats accumulate). */
+   exactly as frmc_engine_run_translation would advance it alone.
+   py_states, np_states and stats are arrays of nengines entries (stats
+   accumulate).  When every engine has single-atom groups (and batching is
+   on), all active replicas' windows go out in one launch per kernel per
+   step, on the first engine's stream; otherwise each engine's windows run
+   on its own stream, several in flight at once.  Engines of one call share
+   one device. */
 int frmc_engines_run_translation(frmc_engine **engines, int32_t nengines, int64_t nsteps, int32_t k, float minAmp,
                                  float maxAmp, float tolerance, frmc_mt19937 *py_states, frmc_mt19937 *np_states,
                                  frmc_run_stats *stats);
+/* 1 (default): frmc_engines_run_translation batches single-atom replicas'
+   windows into one launch per kernel; 0: one stream per engine. */
+void frmc_set_replica_batching(int enable);
 /* Per-kernel timing with HIP events recorded on the engine stream around
    the launches of every `every`-th window (two event records per timed
    launch).  frmc_engine_profile(e, every >= 1) resets and enables, (e, 0)

@@ -398,9 +398,11 @@ def test_config1_size_properties():
     assert np.float32(np.sum(chis)).tobytes() == np.float32(E.totalStandardError).tobytes()
 
 
-def test_replicas_run_concurrently_and_match_independent_runs():
-    """Three replicas of different sizes advanced together (windows of all
-    engines in flight on their own streams) end in exactly the state of three
+@pytest.mark.parametrize("batched", [True, False])
+def test_replicas_run_concurrently_and_match_independent_runs(batched):
+    """Three replicas of different sizes advanced together (batched: all
+    windows of a step in one launch per kernel; else every engine's windows
+    in flight on its own stream) end in exactly the state of three
     independent single-engine runs with the same seeds, and replica 0 in the
     oracle's state."""
     from fullrmc_amd.Engine import run_replicas
@@ -418,7 +420,7 @@ def test_replicas_run_concurrently_and_match_independent_runs():
         return E
 
     reps = [engine(s) for s in boxes]
-    stats = run_replicas(reps, nsteps, k=16, seeds=seeds)
+    stats = run_replicas(reps, nsteps, k=16, seeds=seeds, batched=batched)
     assert all(st["generated"] == nsteps for st in stats)
     for E, s, seed in zip(reps, boxes, seeds):
         E.sync_from_device()
@@ -444,6 +446,52 @@ def test_replicas_run_concurrently_and_match_independent_runs():
 
 def _pdc_cons_of(s):
     return lambda osys, w: [oeng.PDC(osys, s["exp"], w)]
+
+
+def test_batched_replicas_of_mixed_cells_and_constraints(small_triclinic):
+    """One batched run over replicas that differ in everything a window
+    kernel reads: atom count, cubic or triclinic cell, one or two elements,
+    G(r) alone or with S(Q) (two grids), window sizes.  Each ends
+    bit-identical to its own single-engine run with the same seed."""
+    from fullrmc_amd.Engine import run_replicas
+    q = np.linspace(0.5, 12, 90).astype(F32)
+    specs = []
+    for i, n in enumerate((260, 700, 90, 1500)):
+        L = (n / 0.05) ** (1 / 3)
+        s = systems.random_box(n, L, 2.0, seed=70 + i, species=("Si", "O") if i % 2 else ("Ar",),
+                               fractions=(1 / 3, 2 / 3) if i % 2 else (1.0,))
+        s["exp"] = pdc_target(s, 0.1, 80 + i, 0.025, 0.05, 140)
+        specs.append((s, i == 1))
+    t = dict(small_triclinic)
+    t["exp"] = pdc_target(t, 0.1, 90, 0.025, 0.05, 150)
+    specs.append((t, True))
+    seeds, nsteps = [21, 22, 23, 24, 25], 200
+
+    def engine(s, with_sq):
+        cons = [PairDistributionConstraint(s["exp"])]
+        if with_sq:
+            cons.append(StructureFactorConstraint(np.stack([q, np.ones_like(q)], 1).astype(F32), rmin=0.0, rmax=7.0,
+                                                  dr=0.05))
+        E = device_engine(s, cons)
+        E.set_move_generator(TranslationGenerator(0.2))
+        return E
+
+    reps = [engine(s, w) for s, w in specs]
+    stats = run_replicas(reps, nsteps, k=24, seeds=seeds)
+    assert all(st["generated"] == nsteps for st in stats)
+    assert sum(st["accepted"] for st in stats) > 0
+    for E, (s, w), seed in zip(reps, specs, seeds):
+        E.sync_from_device()
+        random.seed(seed)
+        np.random.seed(seed)
+        S = engine(s, w)
+        S.run_device(nsteps, k=24)
+        assert E.realCoordinates.tobytes() == S.realCoordinates.tobytes()
+        assert np.float32(E.totalStandardError).tobytes() == np.float32(S.totalStandardError).tobytes()
+        assert E.accepted == S.accepted
+        for c, cs in zip(E.constraints, S.constraints):
+            assert c.data["intra"].tobytes() == cs.data["intra"].tobytes()
+            assert c.data["inter"].tobytes() == cs.data["inter"].tobytes()
 
 
 def test_molecular_triclinic_full_size():
