@@ -144,6 +144,17 @@ class Engine:
         """Use one move generator for every group."""
         self.generators = [generator] * len(self.groups)
 
+    def _distinct_generators(self):
+        """The distinct move generators of the groups (one C-level pass when
+        every group shares one)."""
+        gens = self.generators
+        if gens and gens.count(gens[0]) == len(gens):
+            return [gens[0]]
+        seen = {}
+        for g in gens:
+            seen.setdefault(id(g), g)
+        return list(seen.values())
+
     def set_tolerance(self, tolerance):
         """Percentage [0,100] of worse moves allowed (Engine.py:1826-1842)."""
         t = FLOAT_TYPE(tolerance)
@@ -326,7 +337,7 @@ class Engine:
         if self.rigidConstraints:
             raise NotImplementedError("rigid (distance) constraints veto moves one at a time in Engine.run; "
                                       "the device engine evaluates the experimental pair constraints only")
-        if any(hasattr(g, "get_ready_for_move") or hasattr(g, "pick_from_list") for g in self.generators):
+        if any(hasattr(g, "get_ready_for_move") or hasattr(g, "pick_from_list") for g in self._distinct_generators()):
             raise NotImplementedError("swap and remove moves change their atoms per move; they run in Engine.run")
         if self.collectedAtoms:
             raise NotImplementedError("a configuration with removed atoms runs in Engine.run")
@@ -436,9 +447,9 @@ class Engine:
         refreshed by the next :meth:`run`, :meth:`sync_from_device` or a
         pulling call).  Returns a dict of counters and timings."""
         h = self._device_engine(max(k, 1))
-        gen0 = self.generators[0]
-        nat = (native and all(g is gen0 for g in self.generators)
-               and getattr(gen0, "native_kind", None) == "translation")
+        distinct = self._distinct_generators()
+        gen0 = distinct[0]
+        nat = native and len(distinct) == 1 and getattr(gen0, "native_kind", None) == "translation"
         t0 = time.perf_counter()
         if nat:
             stats = self._run_native(h, int(numberOfSteps), int(k), gen0, log)
@@ -559,11 +570,15 @@ class Engine:
 
 # ---------------------------------------------------------------- replicas
 
+def _mt_words(m):
+    """The 624 state words of an frmc_mt19937, as a uint32 view."""
+    return np.ctypeslib.as_array(m.mt)
+
+
 def _mt_from_python(state):
     """frmc_mt19937 from a random.getstate() tuple."""
     m = _lib.MT19937()
-    for q in range(624):
-        m.mt[q] = state[1][q]
+    _mt_words(m)[:] = np.array(state[1][:624], dtype=np.uint32)
     m.pos = state[1][624]
     return m
 
@@ -573,18 +588,17 @@ def _mt_from_numpy(state):
     if state[3]:
         raise ValueError("numpy.random holds a cached Gaussian; the native generator cannot carry it")
     m = _lib.MT19937()
-    for q in range(624):
-        m.mt[q] = int(state[1][q])
+    _mt_words(m)[:] = np.asarray(state[1], dtype=np.uint32)
     m.pos = int(state[2])
     return m
 
 
 def _mt_to_python(m, like):
-    return (like[0], tuple(int(m.mt[q]) for q in range(624)) + (int(m.pos),), like[2])
+    return (like[0], tuple(_mt_words(m).tolist()) + (int(m.pos),), like[2])
 
 
 def _mt_to_numpy(m):
-    return ("MT19937", np.array([m.mt[q] for q in range(624)], dtype=np.uint32), int(m.pos), 0, 0.0)
+    return ("MT19937", _mt_words(m).copy(), int(m.pos), 0, 0.0)
 
 
 def run_replicas(engines, numberOfSteps=100, k=32, seeds=None, batched=True):
@@ -608,8 +622,9 @@ def run_replicas(engines, numberOfSteps=100, k=32, seeds=None, batched=True):
     amp = (float(gen.amplitude[0]), float(gen.amplitude[1]))
     tol = float(engines[0].tolerance)
     for E in engines:
-        g0 = E.generators[0]
-        if not (all(g is g0 for g in E.generators) and getattr(g0, "native_kind", None) == "translation"):
+        distinct = E._distinct_generators()
+        g0 = distinct[0]
+        if not (len(distinct) == 1 and getattr(g0, "native_kind", None) == "translation"):
             raise ValueError("replicas need translation moves on every group")
         if (float(g0.amplitude[0]), float(g0.amplitude[1])) != amp or float(E.tolerance) != tol:
             raise ValueError("replicas need the same move amplitude and tolerance")

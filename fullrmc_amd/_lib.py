@@ -64,6 +64,8 @@ _PROTOS = {
     "frmc_last_kernel_ms": [],
     "frmc_set_autotune": [c_int],
     "frmc_set_replica_batching": [c_int],
+    "frmc_set_replica_threads": [c_int],
+    "frmc_replica_batch_times": [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64)],
     "frmc_force_full_schedule": [c_i32, c_i32, c_i32],
     "frmc_last_full_schedule": [c_i32p, c_i32p, ctypes.POINTER(c_i64), c_i32p],
     # pairs_distances
@@ -135,7 +137,7 @@ _PROTOS = {
 KERNEL_NAMES = ["k_delta", "k_eval_a", "k_commit", "k_eval_b"]
 _RESTYPES = {"frmc_last_error": ctypes.c_char_p, "frmc_last_overflow": c_i64, "frmc_set_timing": None,
              "frmc_last_kernel_ms": c_f32, "frmc_set_autotune": None,
-             "frmc_set_replica_batching": None}
+             "frmc_set_replica_batching": None, "frmc_set_replica_threads": None}
 
 _lib = None
 

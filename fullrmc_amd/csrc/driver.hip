@@ -11,6 +11,8 @@
 // rejected), so the k proposals of a window are the ones the sequential loop
 // generates, and the trajectory is the sequential one.
 #include <math.h>
+#include <sched.h>
+#include <time.h>
 
 #include <algorithm>
 #include <atomic>
@@ -255,6 +257,28 @@ struct TranslationRun {
 };
 
 static int g_batch_replicas = 1;  // frmc_set_replica_batching
+static int g_replica_threads = 0;  // frmc_set_replica_threads (0: automatic)
+
+// host threads for replicas: the CPUs this process may run on, less two for
+// the caller's other threads and the HIP runtime, at most kReplicaThreads
+static int32_t replica_threads(int32_t nengines) {
+    int32_t cpus = (int32_t)std::max(1u, std::thread::hardware_concurrency());
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) cpus = std::max(1, CPU_COUNT(&set));
+    int32_t T = g_replica_threads > 0 ? g_replica_threads : std::min<int32_t>(kReplicaThreads, std::max(1, cpus - 2));
+    return std::max<int32_t>(1, std::min<int32_t>(T, nengines));
+}
+// host time of the last batched run, by phase (frmc_replica_batch_times):
+// drawing and staging, upload and launch, waiting and walking, set-up,
+// final commits; steps
+static double g_batch_ms[5] = {0, 0, 0, 0, 0};
+static int64_t g_batch_steps = 0;
+
+static double now_ms() {
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + 1e-6 * ts.tv_nsec;
+}
 
 // Fixed worker threads for the host halves of a batched step: run(f) calls
 // f(t) on every thread t in [0, T) (t = 0 on the caller) and returns when all
@@ -299,14 +323,34 @@ class StepPool {
 
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// All replicas' windows in one launch per kernel per step: the host draws
-// every active replica's proposals (threads over replicas), uploads windows,
-// proposals and block prefixes in one copy, launches k_delta_batch,
-// k_eval_a_batch and k_eval_b_batch on the first engine's stream, then walks
-// each replica's totals with the reference's rule exactly as the single
-// engine loop does.  Each replica keeps its own state, streams and deferred
-// commit, so every trajectory is the one its engine alone would produce.
+constexpr int32_t kLanes = 2;  // replica groups whose steps alternate on the GPU
+
+// One lane of a batched run: a fixed set of replicas and its own host and
+// device step blocks (windows, block maps, proposals), so that one lane's
+// host work overlaps the other lane's kernels.
+struct Lane {
+    std::vector<int32_t> reps;       // replicas of the lane
+    std::vector<int32_t> act, kws, poff;
+    int32_t nact = 0;
+    bool inflight = false;
+    size_t win_b = 0, map_b = 0, bytes = 0;
+    void *h = nullptr, *d = nullptr;
+    BatchArgs A{};
+    int32_t nd = 0, na = 0, nb = 0;
+};
+
+// All replicas' windows in one launch per kernel per step and lane: the host
+// draws every active replica's proposals (threads over replicas), uploads
+// windows, proposals and block maps in one copy, launches k_delta_batch,
+// k_eval_a_batch and (when some window needs it) k_eval_b_batch on the first
+// engine's stream, then walks each replica's totals with the reference's
+// rule exactly as the single-engine loop does.  The replicas are split into
+// two lanes whose steps alternate, so one lane's host work runs while the
+// other's kernels do.  Each replica keeps its own state, streams and
+// deferred commit, so every trajectory is the one its engine alone would
+// produce.
 static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
+    const double t_start = now_ms();
     const int32_t R = (int32_t)runs.size();
     hipStream_t st = runs[0].e->st;
     // the engines' own streams may still hold work on their buffers
@@ -315,116 +359,186 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
         FRMC_CHECK_HIP(hipStreamSynchronize(r.e->st));
     }
     std::vector<View> hv(R);
-    size_t shm_a = 0, shm_b = 0;
+    size_t shm_a = 0, shm_b = 0, shm_d = 0;
     int32_t kmax = 1;
+    int64_t maxd = 0, maxa = 0;
     for (int32_t r = 0; r < R; r++) {
         hv[r] = make_view(runs[r].e, 0);
+        if (hv[r].dnStride <= kBatchLdsInts) shm_d = std::max(shm_d, sizeof(int32_t) * hv[r].dnStride);
         shm_a = std::max(shm_a, eval_lds_bytes(hv[r]));
         shm_b = std::max(shm_b, eval_b_lds_bytes(hv[r]));
         kmax = std::max(kmax, runs[r].k);
+        maxd += (int64_t)((hv[r].n + kChunk - 1) / kChunk) * runs[r].k;
+        maxa += (int64_t)hv[r].nunits * runs[r].k;
     }
-    const size_t pre_b = align16(sizeof(int32_t) * (R + 1)), win_b = align16(sizeof(BatchWin) * R);
-    const size_t head = 3 * pre_b + win_b;  // prefixes, windows, then the proposal table
-    const size_t bytes = head + sizeof(Prop) * (size_t)R * kmax;
+    FRMC_REQUIRE(maxd < ((int64_t)1 << 30) && maxa < ((int64_t)1 << 30), "batched grid too large");
+    std::vector<Lane> lanes(std::min<int32_t>(kLanes, R));
+    const int32_t L = (int32_t)lanes.size();
+    // replicas dealt to the lanes in order of size, so the lanes carry equal work
+    {
+        std::vector<int32_t> order(R);
+        for (int32_t r = 0; r < R; r++) order[r] = r;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return hv[x].n > hv[y].n; });
+        for (int32_t q = 0; q < R; q++) lanes[q % L].reps.push_back(order[q]);
+    }
     View *d_views = nullptr;
-    void *d_blk = nullptr, *h_blk = nullptr;
     struct Free {
         hipStream_t st;
-        View *&a;
-        void *&b, *&c;
+        View *&v;
+        std::vector<Lane> &ls;
         ~Free() {
             (void)hipStreamSynchronize(st);  // nothing in flight reads the blocks
-            if (a) (void)hipFree(a);
-            if (b) (void)hipFree(b);
-            if (c) (void)hipHostFree(c);
+            if (v) (void)hipFree(v);
+            for (auto &l : ls) {
+                if (l.d) (void)hipFree(l.d);
+                if (l.h) (void)hipHostFree(l.h);
+            }
         }
-    } guard{st, d_views, d_blk, h_blk};
+    } guard{st, d_views, lanes};
     FRMC_CHECK_HIP(hipMalloc((void **)&d_views, sizeof(View) * R));
-    FRMC_CHECK_HIP(hipMalloc(&d_blk, bytes));
-    FRMC_CHECK_HIP(hipHostMalloc(&h_blk, bytes, hipHostMallocDefault));
     FRMC_CHECK_HIP(hipMemcpyAsync(d_views, hv.data(), sizeof(View) * R, hipMemcpyHostToDevice, st));
-    char *hb = (char *)h_blk, *db = (char *)d_blk;
-    int32_t *hpre[3] = {(int32_t *)hb, (int32_t *)(hb + pre_b), (int32_t *)(hb + 2 * pre_b)};
-    BatchWin *hw = (BatchWin *)(hb + 3 * pre_b);
-    Prop *hp = (Prop *)(hb + head);
-    BatchArgs A;
-    A.views = d_views;
-    A.pre_d = (const int32_t *)db;
-    A.pre_a = (const int32_t *)(db + pre_b);
-    A.pre_b = (const int32_t *)(db + 2 * pre_b);
-    A.win = (const BatchWin *)(db + 3 * pre_b);
-    A.props = (const Prop *)(db + head);
-
-    std::vector<int32_t> act(R), kws(R), poff(R);
+    // step block of a lane: windows, the three block maps, then the proposals
+    for (auto &l : lanes) {
+        const int32_t n = (int32_t)l.reps.size();
+        int64_t md = 0, ma = 0;
+        for (int32_t r : l.reps) {
+            md += (int64_t)((hv[r].n + kChunk - 1) / kChunk) * runs[r].k;
+            ma += (int64_t)hv[r].nunits * runs[r].k;
+        }
+        l.win_b = align16(sizeof(BatchWin) * n);
+        l.map_b = align16(sizeof(int2) * (md + 2 * ma));
+        l.bytes = l.win_b + l.map_b + sizeof(Prop) * (size_t)n * kmax;
+        l.act.resize(n);
+        l.kws.resize(n);
+        l.poff.resize(n);
+        FRMC_CHECK_HIP(hipMalloc(&l.d, l.bytes));
+        FRMC_CHECK_HIP(hipHostMalloc(&l.h, l.bytes, hipHostMallocDefault));
+        l.A.views = d_views;
+        l.A.win = (const BatchWin *)l.d;
+    }
     std::vector<int> rcs(T, FRMC_OK);
     std::vector<std::string> errs(T);
-    int32_t nact = 0;
     auto fail = [&](int32_t t, int rc) {
         if (rc && !rcs[t]) {
             rcs[t] = rc;
             errs[t] = frmc_last_error();
         }
     };
+    auto first_error = [&]() {
+        for (int32_t t = 0; t < T; t++)
+            if (rcs[t]) {
+                set_error("%s", errs[t].c_str());
+                return rcs[t];
+            }
+        return (int)FRMC_OK;
+    };
+    Lane *cur = nullptr;
     const std::function<void(int32_t)> prepare = [&](int32_t t) {
-        for (int32_t i = t; i < nact && !rcs[t]; i += T) {
-            TranslationRun &run = runs[act[i]];
-            run.kw = kws[i];
+        Lane &l = *cur;
+        BatchWin *hw = (BatchWin *)l.h;
+        Prop *hp = (Prop *)((char *)l.h + l.win_b + l.map_b);
+        for (int32_t i = t; i < l.nact && !rcs[t]; i += T) {
+            TranslationRun &run = runs[l.act[i]];
+            run.kw = l.kws[i];
             run.draw();
             BatchWin &W = hw[i];
-            W.rep = act[i];
-            W.prop = poff[i];
-            fail(t, batch_prepare(run.e, run.kw, run.gids.data(), run.nbox.data(), W, hp + poff[i]));
+            W.rep = l.act[i];
+            W.prop = l.poff[i];
+            fail(t, batch_prepare(run.e, run.kw, run.gids.data(), run.nbox.data(), W, hp + l.poff[i]));
         }
     };
     const std::function<void(int32_t)> finish = [&](int32_t t) {
-        for (int32_t i = t; i < nact && !rcs[t]; i += T) {
-            fail(t, runs[act[i]].finish());
-            runs[act[i]].kw = 0;
+        Lane &l = *cur;
+        for (int32_t i = t; i < l.nact && !rcs[t]; i += T) {
+            fail(t, runs[l.act[i]].finish());
+            runs[l.act[i]].kw = 0;
         }
     };
     StepPool pool(T);
-    for (;;) {
-        nact = 0;
+    for (double &x : g_batch_ms) x = 0.0;
+    g_batch_steps = 0;
+    // draw, stage, upload and launch the next window of every active replica of lane l
+    auto launch_lane = [&](Lane &l) -> int {
+        const double t0 = now_ms();
+        l.nact = 0;
         int32_t off = 0;
-        for (int32_t r = 0; r < R; r++)
+        for (int32_t r : l.reps)
             if (!runs[r].done()) {
-                act[nact] = r;
-                kws[nact] = runs[r].next_k();
-                poff[nact] = off;
-                off += kws[nact];
-                nact++;
+                l.act[l.nact] = r;
+                l.kws[l.nact] = runs[r].next_k();
+                l.poff[l.nact] = off;
+                off += l.kws[l.nact];
+                l.nact++;
             }
-        if (nact == 0) break;
+        l.inflight = l.nact > 0;
+        if (!l.inflight) return FRMC_OK;
+        g_batch_steps++;
+        cur = &l;
         pool.run(prepare);
-        for (int32_t t = 0; t < T; t++)
-            if (rcs[t]) {
-                set_error("%s", errs[t].c_str());
-                return rcs[t];
-            }
-        hpre[0][0] = hpre[1][0] = hpre[2][0] = 0;
-        int64_t tot[3] = {0, 0, 0};
-        for (int32_t i = 0; i < nact; i++) {
-            const View &V = hv[act[i]];
-            const int32_t k = kws[i];
-            tot[0] += (int64_t)((V.n + 255) / 256) * ((k + kDP - 1) / kDP);
-            tot[1] += (int64_t)V.nunits * k;
-            tot[2] += k;
-            for (int q = 0; q < 3; q++) hpre[q][i + 1] = (int32_t)tot[q];
+        if (int rc = first_error()) return rc;
+        const double t1 = now_ms();
+        // block maps: (window, block within the window) per block of each kernel
+        int2 *hm = (int2 *)((char *)l.h + l.win_b);
+        int32_t nd = 0, na = 0, nb = 0;
+        for (int32_t i = 0; i < l.nact; i++) {
+            const View &V = hv[l.act[i]];
+            const int32_t k = l.kws[i], bd = ((V.n + kChunk - 1) / kChunk) * k, ba = V.nunits * k;
+            for (int32_t q = 0; q < bd; q++) hm[nd++] = make_int2(i, q);
         }
-        FRMC_REQUIRE(tot[0] < ((int64_t)1 << 31) && tot[1] < ((int64_t)1 << 31), "batched grid too large");
-        A.nact = nact;
-        FRMC_CHECK_HIP(hipMemcpyAsync(d_blk, h_blk, head + sizeof(Prop) * (size_t)off, hipMemcpyHostToDevice, st));
-        if (int rc = launch_batch(A, (int32_t)tot[0], (int32_t)tot[1], (int32_t)tot[2], shm_a, shm_b, st)) return rc;
+        for (int32_t i = 0; i < l.nact; i++) {
+            const View &V = hv[l.act[i]];
+            const int32_t ba = V.nunits * l.kws[i];
+            for (int32_t q = 0; q < ba; q++) hm[nd + na++] = make_int2(i, q);
+        }
+        for (int32_t i = 0; i < l.nact; i++) {
+            const View &V = hv[l.act[i]];
+            if (V.C == 1 && V.c[0].rspace) continue;  // k_eval_a_batch publishes the total
+            for (int32_t q = 0; q < l.kws[i]; q++) hm[nd + na + nb++] = make_int2(i, q);
+        }
+        l.A.map_d = (const int2 *)((char *)l.d + l.win_b);
+        l.A.map_a = l.A.map_d + nd;
+        l.A.map_b = l.A.map_a + na;
+        l.A.props = (const Prop *)((char *)l.d + l.win_b + l.map_b);
+        // windows and maps, then the proposals, in one copy when they are adjacent
+        FRMC_CHECK_HIP(hipMemcpyAsync(l.d, l.h, l.win_b + sizeof(int2) * (nd + na + nb), hipMemcpyHostToDevice, st));
+        FRMC_CHECK_HIP(hipMemcpyAsync((char *)l.d + l.win_b + l.map_b, (char *)l.h + l.win_b + l.map_b,
+                                      sizeof(Prop) * (size_t)off, hipMemcpyHostToDevice, st));
+        if (int rc = launch_batch(l.A, nd, na, nb, shm_d, shm_a, shm_b, st)) return rc;
+        const double t2 = now_ms();
+        g_batch_ms[0] += t1 - t0;
+        g_batch_ms[1] += t2 - t1;
+        return FRMC_OK;
+    };
+    auto finish_lane = [&](Lane &l) -> int {
+        if (!l.inflight) return FRMC_OK;
+        const double t0 = now_ms();
+        cur = &l;
         pool.run(finish);
-        for (int32_t t = 0; t < T; t++)
-            if (rcs[t]) {
-                set_error("%s", errs[t].c_str());
-                return rcs[t];
-            }
+        l.inflight = false;
+        g_batch_ms[2] += now_ms() - t0;
+        return first_error();
+    };
+    g_batch_ms[3] = now_ms() - t_start;
+    for (auto &l : lanes)
+        if (int rc = launch_lane(l)) return rc;
+    for (;;) {
+        bool any = false;
+        for (auto &l : lanes) {
+            if (!l.inflight) continue;
+            any = true;
+            if (int rc = finish_lane(l)) return rc;
+            if (int rc = launch_lane(l)) return rc;
+        }
+        if (!any) break;
     }
+    // the accepted moves of the last windows, on the batch stream
+    const double t_end = now_ms();
+    for (auto &r : runs)
+        if (int rc = flush_commit(r.e, st)) return rc;
     FRMC_CHECK_HIP(hipStreamSynchronize(st));
     for (auto &r : runs)
         if (int rc = finish_windows(r.e)) return rc;
+    g_batch_ms[4] = now_ms() - t_end;
     return FRMC_OK;
 }
 
@@ -433,6 +547,15 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
 using namespace frmc;
 
 extern "C" void frmc_set_replica_batching(int enable) { g_batch_replicas = enable != 0; }
+
+extern "C" void frmc_set_replica_threads(int nthreads) { g_replica_threads = nthreads > 0 ? nthreads : 0; }
+
+extern "C" int frmc_replica_batch_times(double *ms_out, int64_t *steps_out) {
+    FRMC_REQUIRE(ms_out && steps_out, "NULL pointer argument");
+    for (int q = 0; q < 5; q++) ms_out[q] = g_batch_ms[q];
+    *steps_out = g_batch_steps;
+    return FRMC_OK;
+}
 
 extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float minAmp, float maxAmp,
                                            float tolerance, frmc_mt19937 *py_state, frmc_mt19937 *np_state,
@@ -486,8 +609,7 @@ extern "C" int frmc_engines_run_translation(frmc_engine **engines, int32_t nengi
     // and launches all its windows, then finishes them in the same order.  The
     // per-launch host cost is what bounds many small replicas, so it is spread
     // over threads; engines never share state, so the threads need no locks.
-    const int32_t hw = (int32_t)std::max(1u, std::thread::hardware_concurrency());
-    const int32_t T = std::max<int32_t>(1, std::min<int32_t>({nengines, kReplicaThreads, hw}));
+    const int32_t T = replica_threads(nengines);
     for (int32_t r = 1; r < nengines; r++)
         FRMC_REQUIRE(engines[r]->device == engines[0]->device, "replicas must share one device");
     bool batchable = g_batch_replicas != 0;

@@ -129,8 +129,10 @@ constexpr int kDP = 4;  // proposals per k_delta block
 
 // Windows of many engines (replicas) launched together, one launch per kernel
 // (single-atom moves).  The views are uploaded once per run; each step
-// uploads the active replicas' windows, their proposals and the block
-// prefixes of the three kernels in one copy.
+// uploads the active replicas' windows, their proposals and the block maps
+// of the kernels in one copy.
+constexpr int32_t kChunk = 4096;         // other atoms per workgroup of k_delta_batch
+constexpr int32_t kBatchLdsInts = 8192;  // largest delta block counted in LDS there (32 KB)
 struct Prop {  // one single-atom proposal: atom, element, old and new box coordinates
     int32_t atom, el;
     float old[3], nw[3];
@@ -145,10 +147,9 @@ struct BatchWin {
 };
 struct BatchArgs {
     const View *views;
-    const BatchWin *win;     // [nact]
+    const BatchWin *win;     // [active windows]
     const Prop *props;
-    const int32_t *pre_d, *pre_a, *pre_b;  // [nact+1] first block of each window per kernel
-    int32_t nact;
+    const int2 *map_d, *map_a, *map_b;  // per block of each kernel: (window, block within it)
 };
 
 // launchers (engine_kernels.hip)
@@ -160,7 +161,7 @@ int launch_symmetrise(int32_t *base, const int32_t *slot, int32_t nel, int32_t H
 int launch_fill(int32_t *p, int64_t n, hipStream_t st);
 int launch_full(const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho, const int32_t *mol,
                 const int32_t *elem, const HistArgs &A, int32_t *counts, hipStream_t st);
-int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t shm_a, size_t shm_b,
+int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t shm_d, size_t shm_a, size_t shm_b,
                  hipStream_t st);
 void pairwise_plan(int32_t n, std::vector<int32_t> &leaves, std::vector<int32_t> &prog);
 size_t eval_lds_bytes(const View &V);
@@ -227,7 +228,7 @@ View make_view(frmc_engine *e, int stage_idx);
 int launch_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
                           const float *u, float tol);
 int finish_device_decided(frmc_engine *e, int32_t k, float *tse_out);
-int flush_commit(frmc_engine *e);
+int flush_commit(frmc_engine *e, hipStream_t on = nullptr);
 int batch_prepare(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box, BatchWin &W,
                   Prop *props);
 int settle_window(frmc_engine *e, int32_t p);

@@ -378,8 +378,10 @@ int batch_prepare(frmc_engine *e, int32_t k, const int32_t *group_ids, const flo
 }
 
 // Apply an accepted move still left for a next window's k_delta, as a
-// k_commit of its own (before anything else reads or replaces the state).
-int flush_commit(frmc_engine *e) {
+// k_commit of its own (before anything else reads or replaces the state), on
+// the engine's stream or on stream `on` (the caller then orders the engine's
+// stream after it).
+int flush_commit(frmc_engine *e, hipStream_t on) {
     if (!e->pend.valid) return FRMC_OK;
     Inline I1;
     I1.k = 1;
@@ -397,6 +399,7 @@ int flush_commit(frmc_engine *e) {
     C.dn = e->pend.dn;
     View V = make_view(e, e->cur_stage);
     e->pend.valid = 0;
+    if (on && on != e->st) return launch_commit(V, I1, C, on);
     KTimer t(e, FRMC_K_COMMIT);
     return launch_commit(V, I1, C, e->st);
 }

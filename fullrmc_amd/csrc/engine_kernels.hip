@@ -328,9 +328,29 @@ __device__ float block_pairwise(const float *a, const DevC &c, float *scratch, i
 // The S(Q) sum is a chain of H dependent adds per Q: the kernel is bound by
 // that latency, so each wave carries one chain per lane and the grid spreads
 // the (Q chunk, proposal) pairs over the CUs.
+// A result for the host: the value and the window's sequence number in one
+// 64-bit store to host memory, so the host needs no ordering between stores
+// (it waits until every word carries the current window's number).
+__device__ __forceinline__ void publish(unsigned long long *w, float v, uint32_t seq) {
+    const unsigned long long word = ((unsigned long long)seq << 32) | (unsigned long long)__float_as_uint(v);
+    __hip_atomic_store(w, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// What k_eval_b does for a window whose engine has a single r-space
+// constraint, done by k_eval_a instead (batched windows): publish the
+// chi-square and the total, and zero the other delta buffer.
+struct Fuse {
+    int32_t *dn_zero;
+    int64_t zero_len;
+    uint32_t seq;
+    int32_t nb;  // blocks of the window
+};
+
 // work unit u of proposal p; dnw is the window's delta buffer, sm the
-// block's dynamic LDS (eval_lds_bytes)
-__device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, int32_t p, int32_t u, float *sm) {
+// block's dynamic LDS (eval_lds_bytes); F: also k_eval_b's work (one r-space
+// constraint)
+__device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, int32_t p, int32_t u, float *sm,
+                                            const Fuse *F) {
     const int32_t Hp = (V.Hmax + 3) & ~3;        // keeps float4 reads aligned
     float *Gs = sm;                              // [Hp] G(r) or PDC squared deviations
     float *scratch = Gs + Hp;                    // [maxLeaves + kStack]
@@ -446,6 +466,17 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
     if (rsp) {
         const float chi = block_pairwise(Gs, c, scratch, lprog);
         if (tid == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
+        if (F) {
+            // np.sum([chi]) from 0, as k_eval_b sums it
+            if (tid == 0) {
+                const float tse = 0.0f + chi;
+                publish(V.out_res + (int64_t)p * 2 + 1, chi, F->seq);
+                V.tse_dev[p] = tse;
+                publish(V.out_res + (int64_t)p * 2, tse, F->seq);
+            }
+            for (int64_t s = (int64_t)p * blockDim.x + tid; s < F->zero_len; s += (int64_t)F->nb * blockDim.x)
+                F->dn_zero[s] = 0;
+        }
         return;
     }
     // The S(Q) sum starts from +0, as NumPy's add reduction does (a sum of
@@ -511,18 +542,10 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
 // grid (nunits, k)
 __global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
     extern __shared__ float sm[];
-    eval_a_body(V, V.dn, blockIdx.y, blockIdx.x, sm);
+    eval_a_body(V, V.dn, blockIdx.y, blockIdx.x, sm, nullptr);
 }
 
 // --------------------------------------------------------------- k_eval_b
-
-// A result for the host: the value and the window's sequence number in one
-// 64-bit store to host memory, so the host needs no ordering between stores
-// (it waits until every word carries the current window's number).
-__device__ __forceinline__ void publish(unsigned long long *w, float v, uint32_t seq) {
-    const unsigned long long word = ((unsigned long long)seq << 32) | (unsigned long long)__float_as_uint(v);
-    __hip_atomic_store(w, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
 
 // proposal p of a k-block grid: the SFC chi-squares and the total, published
 // with the window's number seq; the blocks also zero zero_len ints of dn_zero
@@ -572,51 +595,199 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
 // ----------------------------------------------------------- batched windows
 
 // Windows of many engines (replicas) in one launch per kernel: the blocks of
-// each active replica's window are numbered consecutively, pre[i] being the
-// first block of active replica i (pre[nact] = the grid).  A block finds its
-// replica by binary search and runs the single-engine kernel's body on that
-// replica's view, window and proposals.
-__device__ __forceinline__ int32_t find_slot(const int32_t *pre, int32_t nact, int32_t b) {
-    int32_t lo = 0, hi = nact;  // pre[lo] <= b < pre[hi]
-    while (hi - lo > 1) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= b) lo = mid;
-        else hi = mid;
+// each active replica's window are numbered consecutively, and a per-kernel
+// map gives each block its window (index into the step's windows) and its
+// block index within that window.  A block runs the single-engine kernel's
+// body on that replica's view, window and proposals.
+
+// The fields of a replica's view that the batched delta loop reads, copied
+// to registers once per workgroup: the view itself sits in global memory,
+// where every atomic of the loop could (as far as the compiler knows) change
+// it, so reading it in the loop would reload it after each atomic.
+struct DeltaCtx {
+    int32_t n, nel, C;
+    Basis B;
+    Grid g[kMaxC];
+    int32_t H[kMaxC], dnOff[kMaxC], same[kMaxC];
+    int32_t *slot[kMaxC];
+    const float *xyz;
+    const int32_t *mol, *elem;
+};
+
+__device__ __forceinline__ void load_ctx(const View &V, DeltaCtx &X) {
+    X.n = V.n;
+    X.nel = V.nel;
+    X.C = V.C;
+    X.B = V.B;
+    X.xyz = V.xyz;
+    X.mol = V.mol;
+    X.elem = V.elem;
+#pragma unroll
+    for (int ci = 0; ci < kMaxC; ci++) {
+        const int cc = ci < X.C ? ci : 0;
+        X.g[ci] = V.c[cc].g;
+        X.H[ci] = V.c[cc].H;
+        X.dnOff[ci] = V.c[cc].dnOff;
+        X.same[ci] = ci < X.C ? V.c[cc].sameGrid : 1;
+        X.slot[ci] = V.c[cc].slot;
     }
-    return lo;
+}
+
+// delta_pair and commit_pair on the register copy
+__device__ __forceinline__ void delta_pair_x(const DeltaCtx &X, int32_t *out, float so, float sn, int32_t ea,
+                                             int32_t eb) {
+#pragma unroll
+    for (int ci = 0; ci < kMaxC; ci++) {
+        if (X.same[ci]) continue;  // also every ci >= C
+        const int32_t bo = in_range(so, X.g[ci]) ? bin_of(so, X.g[ci]) : -1;
+        const int32_t bn = in_range(sn, X.g[ci]) ? bin_of(sn, X.g[ci]) : -1;
+        if (bo >= 0) {
+            const int32_t t = type_index(bo, ea, eb, X.nel, X.H[ci]);
+            if (t >= 0) atomicSub(&out[X.dnOff[ci] + t], 1);
+        }
+        if (bn >= 0) {
+            const int32_t t = type_index(bn, ea, eb, X.nel, X.H[ci]);
+            if (t >= 0) atomicAdd(&out[X.dnOff[ci] + t], 1);
+        }
+    }
+}
+
+__device__ __forceinline__ void commit_pair_x(const DeltaCtx &X, float s, int32_t ea, int32_t eb, bool intra,
+                                              int32_t sign) {
+    int32_t b = -1;
+#pragma unroll
+    for (int ci = 0; ci < kMaxC; ci++) {
+        if (ci >= X.C) break;
+        if (!X.same[ci]) b = in_range(s, X.g[ci]) ? bin_of(s, X.g[ci]) : -1;
+        if (b < 0) continue;
+        const int32_t size = X.nel * X.nel * X.H[ci];
+        const int32_t f = flat_slot(b, ea, eb, X.nel, X.H[ci]);
+        if (f < 0) {
+            if (sign > 0) atomicAdd(&X.slot[ci][2 * size], 1);  // past the array: counted, not stored
+            continue;
+        }
+        atomicAdd(&X.slot[ci][(intra ? 0 : size) + f], sign);
+    }
+}
+
+// One workgroup per (proposal p, chunk ch of kChunk other atoms) of a
+// replica's window: the proposal's pair-type deltas are counted in LDS
+// (LDSB, when the delta block fits) and then written out -- with plain stores
+// when one chunk covers all atoms (the workgroup owns the block), else added
+// bin by bin -- or, without LDSB, added straight to the (zeroed) block.  The
+// replica's pending commit is spread over its workgroups as in k_delta:
+// pair-type sums (grid-stride), slot counts (the p == 0 workgroups, each for
+// its chunk), the moved atom's coordinates (one thread).
+template <bool PBC, bool ORTHO, bool LDSB>
+__device__ __forceinline__ void delta_chunk(const View &V, const BatchWin &W, const Prop &q, int32_t p, int32_t ch,
+                                            int32_t nch, int32_t *bins) {
+    const int tid = threadIdx.x;
+    DeltaCtx X;
+    load_ctx(V, X);
+    const int32_t dnStride = V.dnStride, T = V.T, k = W.k;
+    int32_t *out = W.dn + (int64_t)p * dnStride;
+    const Pending P = W.P;
+    const int32_t qa = q.atom, qe = q.el;
+    const float ox = q.old[0], oy = q.old[1], oz = q.old[2];
+    const float nx = q.nw[0], ny = q.nw[1], nz = q.nw[2];
+    if (LDSB) {
+        for (int32_t s = tid; s < dnStride; s += blockDim.x) bins[s] = 0;
+        __syncthreads();
+    }
+    int32_t *acc = LDSB ? bins : out;
+    const bool pv = P.valid != 0;
+    if (pv) {
+        const int64_t gt = ((int64_t)p * nch + ch) * blockDim.x + tid;
+        const int64_t gs = (int64_t)k * nch * blockDim.x;
+        for (int32_t ci = 0; ci < X.C; ci++) {
+            int32_t *base = V.c[ci].base;
+            const int32_t len = T * V.c[ci].H, off = V.c[ci].dnOff;
+            for (int64_t s = gt; s < len; s += gs) base[s] += P.dn[off + s];
+        }
+        if (p == 0 && ch == 0 && tid == 0)
+            for (int d = 0; d < 3; d++) V.xyz[3 * (int64_t)P.atom + d] = P.nw[d];
+    }
+    const int32_t pmol = pv ? X.mol[P.atom] : 0;
+    const int32_t j1 = min(X.n, (ch + 1) * kChunk);
+#pragma unroll 4
+    for (int32_t j = ch * kChunk + tid; j < j1; j += blockDim.x) {
+        float cx = X.xyz[3 * (int64_t)j], cy = X.xyz[3 * (int64_t)j + 1], cz = X.xyz[3 * (int64_t)j + 2];
+        const int32_t ej = X.elem[j];
+        if (pv) {
+            if (p == 0 && j != P.atom) {
+                const float so = pair_s<PBC, ORTHO>(P.old[0], P.old[1], P.old[2], cx, cy, cz, X.B);
+                const float sn = pair_s<PBC, ORTHO>(P.nw[0], P.nw[1], P.nw[2], cx, cy, cz, X.B);
+                const bool intra = pmol == X.mol[j];
+                commit_pair_x(X, so, P.el, ej, intra, -1);
+                commit_pair_x(X, sn, P.el, ej, intra, +1);
+            }
+            if (j == P.atom) {
+                cx = P.nw[0];
+                cy = P.nw[1];
+                cz = P.nw[2];
+            }
+        }
+        if (j != qa) {
+            // M counts (moved atom a, other j) with diff = c[a] - c[j], before and after
+            const float so = pair_s<PBC, ORTHO>(ox, oy, oz, cx, cy, cz, X.B);
+            const float sn = pair_s<PBC, ORTHO>(nx, ny, nz, cx, cy, cz, X.B);
+            delta_pair_x(X, acc, so, sn, qe, ej);
+        }
+    }
+    if (LDSB) {
+        __syncthreads();
+        if (nch == 1) {
+            for (int32_t s = tid; s < dnStride; s += blockDim.x) out[s] = bins[s];
+        } else {
+            for (int32_t s = tid; s < dnStride; s += blockDim.x)
+                if (bins[s]) atomicAdd(&out[s], bins[s]);
+        }
+    }
+}
+
+template <bool PBC, bool ORTHO>
+__device__ __forceinline__ void delta_chunk_any(const View &V, const BatchWin &W, const Prop &q, int32_t p,
+                                                int32_t ch, int32_t nch, int32_t *bins) {
+    if (V.dnStride <= kBatchLdsInts) delta_chunk<PBC, ORTHO, true>(V, W, q, p, ch, nch, bins);
+    else delta_chunk<PBC, ORTHO, false>(V, W, q, p, ch, nch, bins);
 }
 
 __global__ void __launch_bounds__(256) k_delta_batch(BatchArgs A) {
-    const int32_t i = find_slot(A.pre_d, A.nact, blockIdx.x);
-    const BatchWin &W = A.win[i];
+    extern __shared__ int32_t bins[];  // min(max dnStride, kBatchLdsInts) ints
+    const int2 m = A.map_d[blockIdx.x];
+    const BatchWin &W = A.win[m.x];
     const View &V = A.views[W.rep];
-    const int32_t gx = (V.n + 255) / 256, gy = (W.k + kDP - 1) / kDP;
-    const int32_t local = blockIdx.x - A.pre_d[i];
-    const int32_t bx = local % gx, by = local / gx;
-    const PropSrc S{A.props + W.prop};
+    const int32_t nch = (V.n + kChunk - 1) / kChunk;
+    const int32_t local = m.y;
+    const int32_t p = local / nch, ch = local % nch;
+    const Prop &q = A.props[W.prop + p];
     if (V.isPBC) {
-        if (V.ortho) delta_body<true, true>(V, S, W.P, W.dn, W.k, bx, by, gx, gy, nullptr);
-        else delta_body<true, false>(V, S, W.P, W.dn, W.k, bx, by, gx, gy, nullptr);
+        if (V.ortho) delta_chunk_any<true, true>(V, W, q, p, ch, nch, bins);
+        else delta_chunk_any<true, false>(V, W, q, p, ch, nch, bins);
     } else {
-        delta_body<false, false>(V, S, W.P, W.dn, W.k, bx, by, gx, gy, nullptr);
+        delta_chunk_any<false, false>(V, W, q, p, ch, nch, bins);
     }
 }
 
 __global__ void __launch_bounds__(64 * kEW) k_eval_a_batch(BatchArgs A) {
     extern __shared__ float sm[];
-    const int32_t i = find_slot(A.pre_a, A.nact, blockIdx.x);
-    const BatchWin &W = A.win[i];
+    const int2 m = A.map_a[blockIdx.x];
+    const BatchWin &W = A.win[m.x];
     const View &V = A.views[W.rep];
-    const int32_t local = blockIdx.x - A.pre_a[i];
-    eval_a_body(V, W.dn, local / V.nunits, local % V.nunits, sm);
+    if (V.C == 1 && V.c[0].rspace) {
+        const Fuse F{W.dn_zero, W.zero_len, W.seq, W.k};
+        eval_a_body(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, &F);
+    } else {
+        eval_a_body(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, nullptr);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_eval_b_batch(BatchArgs A) {
     extern __shared__ float sm[];
-    const int32_t i = find_slot(A.pre_b, A.nact, blockIdx.x);
-    const BatchWin &W = A.win[i];
+    const int2 m = A.map_b[blockIdx.x];
+    const BatchWin &W = A.win[m.x];
     const View &V = A.views[W.rep];
-    eval_b_body(V, W.dn_zero, W.zero_len, W.seq, blockIdx.x - A.pre_b[i], W.k, sm);
+    eval_b_body(V, W.dn_zero, W.zero_len, W.seq, m.y, W.k, sm);
 }
 
 // ---------------------------------------------------------------- k_commit
@@ -763,15 +934,16 @@ int launch_eval_b(const View &V, int32_t k, hipStream_t st) {
     return FRMC_OK;
 }
 
-int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t shm_a, size_t shm_b,
+int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t shm_d, size_t shm_a, size_t shm_b,
                  hipStream_t st) {
-    k_delta_batch<<<(unsigned)nd, 256, 0, st>>>(A);
+    k_delta_batch<<<(unsigned)nd, 256, shm_d, st>>>(A);
     FRMC_CHECK_LAUNCH();
     if (shm_a > 48 * 1024)
         (void)hipFuncSetAttribute((const void *)k_eval_a_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shm_a);
     k_eval_a_batch<<<(unsigned)na, 64 * kEW, shm_a, st>>>(A);
     FRMC_CHECK_LAUNCH();
+    if (nb == 0) return FRMC_OK;  // every window's total came from k_eval_a_batch
     if (shm_b > 48 * 1024)
         (void)hipFuncSetAttribute((const void *)k_eval_b_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shm_b);

@@ -409,6 +409,14 @@ int frmc_engines_run_translation(frmc_engine **engines, int32_t nengines, int64_
 /* 1 (default): frmc_engines_run_translation batches single-atom replicas'
    windows into one launch per kernel; 0: one stream per engine. */
 void frmc_set_replica_batching(int enable);
+/* Host threads that draw and walk the replicas' windows (0, the default:
+   the CPUs available to the process less two, at most 16). */
+void frmc_set_replica_threads(int nthreads);
+/* Host time of the last batched replica run by phase, in ms (ms_out holds
+   5): [0] drawing and staging the proposals, [1] the uploads and launches,
+   [2] waiting for the totals and walking them, [3] set-up, [4] the final
+   commits; steps_out the lane steps. */
+int frmc_replica_batch_times(double *ms_out, int64_t *steps_out);
 /* Per-kernel timing with HIP events recorded on the engine stream around
    the launches of every `every`-th window (two event records per timed
    launch).  frmc_engine_profile(e, every >= 1) resets and enables, (e, 0)
