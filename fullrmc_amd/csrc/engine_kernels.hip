@@ -101,6 +101,7 @@ __device__ __forceinline__ void delta_pair(const View &V, int32_t *out, float so
         if (c.sameGrid) continue;
         const int32_t bo = in_range(so, c.g) ? bin_of(so, c.g) : -1;
         const int32_t bn = in_range(sn, c.g) ? bin_of(sn, c.g) : -1;
+        if (bo == bn) continue;  // -1 and +1 in the same bin (or both out of range) cancel
         if (bo >= 0) {
             const int32_t t = type_index(bo, ea, eb, V.nel, c.H);
             if (t >= 0) atomicSub(&out[c.dnOff + t], 1);
@@ -641,6 +642,7 @@ __device__ __forceinline__ void delta_pair_x(const DeltaCtx &X, int32_t *out, fl
         if (X.same[ci]) continue;  // also every ci >= C
         const int32_t bo = in_range(so, X.g[ci]) ? bin_of(so, X.g[ci]) : -1;
         const int32_t bn = in_range(sn, X.g[ci]) ? bin_of(sn, X.g[ci]) : -1;
+        if (bo == bn) continue;  // -1 and +1 in the same bin (or both out of range) cancel
         if (bo >= 0) {
             const int32_t t = type_index(bo, ea, eb, X.nel, X.H[ci]);
             if (t >= 0) atomicSub(&out[X.dnOff[ci] + t], 1);
