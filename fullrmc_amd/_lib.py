@@ -131,6 +131,7 @@ _PROTOS = {
     "frmc_engine_set_real": [c_vp, c_f32p],
     "frmc_engine_set_counts": [c_vp, c_i32, c_i32p],
     "frmc_engine_profile": [c_vp, c_int],
+    "frmc_engine_time_eval": [c_vp, c_i32, c_i32, c_f32p],
     "frmc_engine_kernel_times": [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64)],
     "frmc_engine_set_adaptive": [c_vp, c_int],
 }

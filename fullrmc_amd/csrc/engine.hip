@@ -848,6 +848,34 @@ int frmc_engine_profile(frmc_engine *e, int every) {
     return FRMC_OK;
 }
 
+int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out) {
+    FRMC_REQUIRE(e && ms_out, "NULL argument");
+    FRMC_REQUIRE(k >= 1 && k <= e->kmax, "k=%d out of range [1,%d]", k, e->kmax);
+    FRMC_REQUIRE(reps >= 1 && reps <= 100000, "reps=%d out of range [1,100000]", reps);
+    // k_eval_a writes only the window's S(Q) and chi-square scratch, which
+    // the next window recomputes: the engine state is untouched
+    const View V = make_view(e, e->cur_stage);
+    hipEvent_t a = nullptr, b = nullptr;
+    FRMC_CHECK_HIP(hipEventCreate(&a));
+    hipError_t err = hipEventCreate(&b);
+    int rc = FRMC_OK;
+    if (err == hipSuccess) err = hipEventRecord(a, e->st);
+    for (int32_t r = 0; r < reps && err == hipSuccess && rc == FRMC_OK; r++) rc = launch_eval(V, k, e->st);
+    if (err == hipSuccess && rc == FRMC_OK) err = hipEventRecord(b, e->st);
+    if (err == hipSuccess && rc == FRMC_OK) err = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (err == hipSuccess && rc == FRMC_OK) err = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (err != hipSuccess) {
+        set_error("k_eval_a timing failed: %s", hipGetErrorString(err));
+        return FRMC_EHIP;
+    }
+    *ms_out = ms / reps;
+    return FRMC_OK;
+}
+
 int frmc_engine_kernel_times(frmc_engine *e, double *ms_out, int64_t *launches_out) {
     FRMC_REQUIRE(e && ms_out && launches_out, "NULL argument");
     FRMC_CHECK_HIP(hipStreamSynchronize(e->st));

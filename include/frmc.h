@@ -430,6 +430,13 @@ int frmc_replica_batch_times(double *ms_out, int64_t *steps_out);
 #define FRMC_NKERNELS 4
 int frmc_engine_profile(frmc_engine *e, int every);
 int frmc_engine_kernel_times(frmc_engine *e, double *ms_out, int64_t *launches_out);
+/* k_eval_a of the engine's last window state launched reps times back to
+   back on the engine stream between two HIP events; *ms_out = the average
+   per launch (launch gaps amortised, unlike the per-launch event pairs of
+   frmc_engine_profile).  It writes only the window's scratch (S(Q) and
+   chi-square partials), so the engine state is unchanged.  No reference
+   counterpart (measurement only). */
+int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out);
 /* window size of frmc_engine_run_translation: adaptive (default; k is the
    maximum, the size follows the acceptance rate) or always k */
 int frmc_engine_set_adaptive(frmc_engine *e, int enable);
