@@ -364,11 +364,12 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
     int64_t maxd = 0, maxa = 0;
     for (int32_t r = 0; r < R; r++) {
         hv[r] = make_view(runs[r].e, 0);
-        if (hv[r].dnStride <= kBatchLdsInts) shm_d = std::max(shm_d, sizeof(int32_t) * hv[r].dnStride);
+        if (kBP * hv[r].dnStride <= kBatchLdsInts)
+            shm_d = std::max(shm_d, sizeof(int32_t) * kBP * hv[r].dnStride);
         shm_a = std::max(shm_a, eval_lds_bytes(hv[r]));
         shm_b = std::max(shm_b, eval_b_lds_bytes(hv[r]));
         kmax = std::max(kmax, runs[r].k);
-        maxd += (int64_t)((hv[r].n + kChunk - 1) / kChunk) * runs[r].k;
+        maxd += (int64_t)((hv[r].n + kChunk - 1) / kChunk) * ((runs[r].k + kBP - 1) / kBP);
         maxa += (int64_t)hv[r].nunits * runs[r].k;
     }
     FRMC_REQUIRE(maxd < ((int64_t)1 << 30) && maxa < ((int64_t)1 << 30), "batched grid too large");
@@ -397,17 +398,18 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
     } guard{st, d_views, lanes};
     FRMC_CHECK_HIP(hipMalloc((void **)&d_views, sizeof(View) * R));
     FRMC_CHECK_HIP(hipMemcpyAsync(d_views, hv.data(), sizeof(View) * R, hipMemcpyHostToDevice, st));
-    // step block of a lane: windows, the three block maps, then the proposals
+    // step block of a lane: windows, then the step's proposals, then the three
+    // block maps
     for (auto &l : lanes) {
         const int32_t n = (int32_t)l.reps.size();
         int64_t md = 0, ma = 0;
         for (int32_t r : l.reps) {
-            md += (int64_t)((hv[r].n + kChunk - 1) / kChunk) * runs[r].k;
+            md += (int64_t)((hv[r].n + kChunk - 1) / kChunk) * ((runs[r].k + kBP - 1) / kBP);
             ma += (int64_t)hv[r].nunits * runs[r].k;
         }
         l.win_b = align16(sizeof(BatchWin) * n);
         l.map_b = align16(sizeof(int2) * (md + 2 * ma));
-        l.bytes = l.win_b + l.map_b + sizeof(Prop) * (size_t)n * kmax;
+        l.bytes = l.win_b + align16(sizeof(Prop) * (size_t)n * kmax) + l.map_b;
         l.act.resize(n);
         l.kws.resize(n);
         l.poff.resize(n);
@@ -436,7 +438,7 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
     const std::function<void(int32_t)> prepare = [&](int32_t t) {
         Lane &l = *cur;
         BatchWin *hw = (BatchWin *)l.h;
-        Prop *hp = (Prop *)((char *)l.h + l.win_b + l.map_b);
+        Prop *hp = (Prop *)((char *)l.h + l.win_b);
         for (int32_t i = t; i < l.nact && !rcs[t]; i += T) {
             TranslationRun &run = runs[l.act[i]];
             run.kw = l.kws[i];
@@ -477,12 +479,14 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
         pool.run(prepare);
         if (int rc = first_error()) return rc;
         const double t1 = now_ms();
-        // block maps: (window, block within the window) per block of each kernel
-        int2 *hm = (int2 *)((char *)l.h + l.win_b);
+        // block maps: (window, block within the window) per block of each
+        // kernel, right after the step's proposals
+        const size_t moff = l.win_b + align16(sizeof(Prop) * (size_t)off);
+        int2 *hm = (int2 *)((char *)l.h + moff);
         int32_t nd = 0, na = 0, nb = 0;
         for (int32_t i = 0; i < l.nact; i++) {
             const View &V = hv[l.act[i]];
-            const int32_t k = l.kws[i], bd = ((V.n + kChunk - 1) / kChunk) * k, ba = V.nunits * k;
+            const int32_t k = l.kws[i], bd = ((V.n + kChunk - 1) / kChunk) * ((k + kBP - 1) / kBP);
             for (int32_t q = 0; q < bd; q++) hm[nd++] = make_int2(i, q);
         }
         for (int32_t i = 0; i < l.nact; i++) {
@@ -495,14 +499,12 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
             if (V.C == 1 && V.c[0].rspace) continue;  // k_eval_a_batch publishes the total
             for (int32_t q = 0; q < l.kws[i]; q++) hm[nd + na + nb++] = make_int2(i, q);
         }
-        l.A.map_d = (const int2 *)((char *)l.d + l.win_b);
+        l.A.props = (const Prop *)((char *)l.d + l.win_b);
+        l.A.map_d = (const int2 *)((char *)l.d + moff);
         l.A.map_a = l.A.map_d + nd;
         l.A.map_b = l.A.map_a + na;
-        l.A.props = (const Prop *)((char *)l.d + l.win_b + l.map_b);
-        // windows and maps, then the proposals, in one copy when they are adjacent
-        FRMC_CHECK_HIP(hipMemcpyAsync(l.d, l.h, l.win_b + sizeof(int2) * (nd + na + nb), hipMemcpyHostToDevice, st));
-        FRMC_CHECK_HIP(hipMemcpyAsync((char *)l.d + l.win_b + l.map_b, (char *)l.h + l.win_b + l.map_b,
-                                      sizeof(Prop) * (size_t)off, hipMemcpyHostToDevice, st));
+        // windows, proposals and maps: one copy of the used bytes
+        FRMC_CHECK_HIP(hipMemcpyAsync(l.d, l.h, moff + sizeof(int2) * (nd + na + nb), hipMemcpyHostToDevice, st));
         if (int rc = launch_batch(l.A, nd, na, nb, shm_d, shm_a, shm_b, st)) return rc;
         const double t2 = now_ms();
         g_batch_ms[0] += t1 - t0;

@@ -672,51 +672,65 @@ __device__ __forceinline__ void commit_pair_x(const DeltaCtx &X, float s, int32_
     }
 }
 
-// One workgroup per (proposal p, chunk ch of kChunk other atoms) of a
-// replica's window: the proposal's pair-type deltas are counted in LDS
-// (LDSB, when the delta block fits) and then written out -- with plain stores
-// when one chunk covers all atoms (the workgroup owns the block), else added
-// bin by bin -- or, without LDSB, added straight to the (zeroed) block.  The
+// One workgroup per (group pb of up to kBP proposals, chunk ch of kChunk
+// other atoms) of a replica's window: each other atom's coordinates are read
+// once for the group's proposals, whose pair-type deltas are counted in LDS
+// (LDSB, when kBP delta blocks fit) and then written out -- with plain stores
+// when one chunk covers all atoms (the workgroup owns the blocks), else added
+// bin by bin -- or, without LDSB, added straight to the (zeroed) blocks.  The
 // replica's pending commit is spread over its workgroups as in k_delta:
-// pair-type sums (grid-stride), slot counts (the p == 0 workgroups, each for
+// pair-type sums (grid-stride), slot counts (the pb == 0 workgroups, each for
 // its chunk), the moved atom's coordinates (one thread).
 template <bool PBC, bool ORTHO, bool LDSB>
-__device__ __forceinline__ void delta_chunk(const View &V, const BatchWin &W, const Prop &q, int32_t p, int32_t ch,
-                                            int32_t nch, int32_t *bins) {
+__device__ __forceinline__ void delta_chunk(const View &V, const BatchWin &W, const Prop *props, int32_t pb,
+                                            int32_t ch, int32_t nch, int32_t *bins) {
     const int tid = threadIdx.x;
     DeltaCtx X;
     load_ctx(V, X);
     const int32_t dnStride = V.dnStride, T = V.T, k = W.k;
-    int32_t *out = W.dn + (int64_t)p * dnStride;
+    const int32_t p0 = pb * kBP, np = min(kBP, k - p0);
+    int32_t *out = W.dn + (int64_t)p0 * dnStride;
     const Pending P = W.P;
-    const int32_t qa = q.atom, qe = q.el;
-    const float ox = q.old[0], oy = q.old[1], oz = q.old[2];
-    const float nx = q.nw[0], ny = q.nw[1], nz = q.nw[2];
+    int32_t qa[kBP], qe[kBP];
+    float ox[kBP], oy[kBP], oz[kBP], nx[kBP], ny[kBP], nz[kBP];
+#pragma unroll
+    for (int q = 0; q < kBP; q++) {
+        const Prop &pr = props[p0 + (q < np ? q : 0)];
+        qa[q] = q < np ? pr.atom : -1;
+        qe[q] = pr.el;
+        ox[q] = pr.old[0];
+        oy[q] = pr.old[1];
+        oz[q] = pr.old[2];
+        nx[q] = pr.nw[0];
+        ny[q] = pr.nw[1];
+        nz[q] = pr.nw[2];
+    }
     if (LDSB) {
-        for (int32_t s = tid; s < dnStride; s += blockDim.x) bins[s] = 0;
+        for (int32_t s = tid; s < np * dnStride; s += blockDim.x) bins[s] = 0;
         __syncthreads();
     }
     int32_t *acc = LDSB ? bins : out;
     const bool pv = P.valid != 0;
+    const int32_t nblk = ((k + kBP - 1) / kBP) * nch;
     if (pv) {
-        const int64_t gt = ((int64_t)p * nch + ch) * blockDim.x + tid;
-        const int64_t gs = (int64_t)k * nch * blockDim.x;
+        const int64_t gt = ((int64_t)pb * nch + ch) * blockDim.x + tid;
+        const int64_t gs = (int64_t)nblk * blockDim.x;
         for (int32_t ci = 0; ci < X.C; ci++) {
             int32_t *base = V.c[ci].base;
             const int32_t len = T * V.c[ci].H, off = V.c[ci].dnOff;
             for (int64_t s = gt; s < len; s += gs) base[s] += P.dn[off + s];
         }
-        if (p == 0 && ch == 0 && tid == 0)
+        if (pb == 0 && ch == 0 && tid == 0)
             for (int d = 0; d < 3; d++) V.xyz[3 * (int64_t)P.atom + d] = P.nw[d];
     }
     const int32_t pmol = pv ? X.mol[P.atom] : 0;
     const int32_t j1 = min(X.n, (ch + 1) * kChunk);
-#pragma unroll 4
+#pragma unroll 2
     for (int32_t j = ch * kChunk + tid; j < j1; j += blockDim.x) {
         float cx = X.xyz[3 * (int64_t)j], cy = X.xyz[3 * (int64_t)j + 1], cz = X.xyz[3 * (int64_t)j + 2];
         const int32_t ej = X.elem[j];
         if (pv) {
-            if (p == 0 && j != P.atom) {
+            if (pb == 0 && j != P.atom) {
                 const float so = pair_s<PBC, ORTHO>(P.old[0], P.old[1], P.old[2], cx, cy, cz, X.B);
                 const float sn = pair_s<PBC, ORTHO>(P.nw[0], P.nw[1], P.nw[2], cx, cy, cz, X.B);
                 const bool intra = pmol == X.mol[j];
@@ -729,45 +743,46 @@ __device__ __forceinline__ void delta_chunk(const View &V, const BatchWin &W, co
                 cz = P.nw[2];
             }
         }
-        if (j != qa) {
+#pragma unroll
+        for (int q = 0; q < kBP; q++) {
+            if (q >= np || j == qa[q]) continue;
             // M counts (moved atom a, other j) with diff = c[a] - c[j], before and after
-            const float so = pair_s<PBC, ORTHO>(ox, oy, oz, cx, cy, cz, X.B);
-            const float sn = pair_s<PBC, ORTHO>(nx, ny, nz, cx, cy, cz, X.B);
-            delta_pair_x(X, acc, so, sn, qe, ej);
+            const float so = pair_s<PBC, ORTHO>(ox[q], oy[q], oz[q], cx, cy, cz, X.B);
+            const float sn = pair_s<PBC, ORTHO>(nx[q], ny[q], nz[q], cx, cy, cz, X.B);
+            delta_pair_x(X, acc + q * dnStride, so, sn, qe[q], ej);
         }
     }
     if (LDSB) {
         __syncthreads();
         if (nch == 1) {
-            for (int32_t s = tid; s < dnStride; s += blockDim.x) out[s] = bins[s];
+            for (int32_t s = tid; s < np * dnStride; s += blockDim.x) out[s] = bins[s];
         } else {
-            for (int32_t s = tid; s < dnStride; s += blockDim.x)
+            for (int32_t s = tid; s < np * dnStride; s += blockDim.x)
                 if (bins[s]) atomicAdd(&out[s], bins[s]);
         }
     }
 }
 
 template <bool PBC, bool ORTHO>
-__device__ __forceinline__ void delta_chunk_any(const View &V, const BatchWin &W, const Prop &q, int32_t p,
+__device__ __forceinline__ void delta_chunk_any(const View &V, const BatchWin &W, const Prop *props, int32_t pb,
                                                 int32_t ch, int32_t nch, int32_t *bins) {
-    if (V.dnStride <= kBatchLdsInts) delta_chunk<PBC, ORTHO, true>(V, W, q, p, ch, nch, bins);
-    else delta_chunk<PBC, ORTHO, false>(V, W, q, p, ch, nch, bins);
+    if (kBP * V.dnStride <= kBatchLdsInts) delta_chunk<PBC, ORTHO, true>(V, W, props, pb, ch, nch, bins);
+    else delta_chunk<PBC, ORTHO, false>(V, W, props, pb, ch, nch, bins);
 }
 
 __global__ void __launch_bounds__(256) k_delta_batch(BatchArgs A) {
-    extern __shared__ int32_t bins[];  // min(max dnStride, kBatchLdsInts) ints
+    extern __shared__ int32_t bins[];  // kBP delta blocks (the largest that fits kBatchLdsInts)
     const int2 m = A.map_d[blockIdx.x];
     const BatchWin &W = A.win[m.x];
     const View &V = A.views[W.rep];
     const int32_t nch = (V.n + kChunk - 1) / kChunk;
-    const int32_t local = m.y;
-    const int32_t p = local / nch, ch = local % nch;
-    const Prop &q = A.props[W.prop + p];
+    const int32_t pb = m.y / nch, ch = m.y % nch;
+    const Prop *props = A.props + W.prop;
     if (V.isPBC) {
-        if (V.ortho) delta_chunk_any<true, true>(V, W, q, p, ch, nch, bins);
-        else delta_chunk_any<true, false>(V, W, q, p, ch, nch, bins);
+        if (V.ortho) delta_chunk_any<true, true>(V, W, props, pb, ch, nch, bins);
+        else delta_chunk_any<true, false>(V, W, props, pb, ch, nch, bins);
     } else {
-        delta_chunk_any<false, false>(V, W, q, p, ch, nch, bins);
+        delta_chunk_any<false, false>(V, W, props, pb, ch, nch, bins);
     }
 }
 
