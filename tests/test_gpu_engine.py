@@ -108,6 +108,35 @@ def _pdc_cons(s):
     return lambda osys, w: [oeng.PDC(osys, s["exp"], w)]
 
 
+@pytest.fixture(scope="module")
+def config0_run():
+    """Plain RMC, one element, 2,000 atoms in a 30 A cube (uniform, rejection
+    below 2.0 A), target G(r) from a 0.1 A Gaussian-jittered copy on
+    r in [0,15) at 0.05 A, translations of 0.1 A, 10,000 steps: the oracle's
+    trajectory."""
+    s = systems.random_box(2000, 30.0, 2.0, seed=100)
+    s["exp"] = pdc_target(s, 0.1, 101, 0.025, 0.05, 300)
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(2000)], oeng.Translation(0.1), _pdc_cons(s), 10000, 7)
+    return s, oe, olog, pys, nps
+
+
+@pytest.mark.parametrize("k", [1, 64])
+def test_config0_full_run_matches_oracle(config0_run, k):
+    """The whole 10,000-step run, one proposal per window and speculative
+    windows of up to 64: the same group, decision and total at every step, and
+    the same final state, as the oracle."""
+    s, oe, olog, pys, nps = config0_run
+    random.seed(7)
+    np.random.seed(7)
+    E = device_engine(s, [PairDistributionConstraint(s["exp"])])
+    E.set_move_generator(TranslationGenerator(0.1))
+    dlog = []
+    st = E.run_device(10000, k=k, log=dlog)
+    assert st["generated"] == 10000
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    assert 1000 < sum(a for _, a, _ in dlog) < 9000
+
+
 @pytest.mark.parametrize("k", [1, 7, 64])
 def test_native_translation_matches_oracle(single_box, k):
     s = single_box
@@ -446,6 +475,42 @@ def test_replicas_run_concurrently_and_match_independent_runs(batched):
 
 def _pdc_cons_of(s):
     return lambda osys, w: [oeng.PDC(osys, s["exp"], w)]
+
+
+def test_config3_shaped_replicas_match_independent_runs():
+    """48 replicas shaped like the 256 of the replica workload (N log-uniform
+    in [1000, 20000], density 0.05 per A^3, 2.0 A minimum distance, targets
+    from 0.1 A jittered copies on r in [0,15) at 0.05 A, k = 32), batched:
+    each ends bit-identical to its own single-engine run with the same seed."""
+    from fullrmc_amd.Engine import run_replicas
+    sizes = systems.replica_sizes(48, 1000, 20000, 77)
+    boxes = []
+    for r, n in enumerate(sizes):
+        L = float((n / 0.05) ** (1.0 / 3.0))
+        b = systems.random_box(int(n), L, 2.0, 300 + r)
+        b["exp"] = pdc_target(b, 0.1, 400 + r, 0.025, 0.05, 300)
+        boxes.append(b)
+
+    def engine(b):
+        E = device_engine(b, [PairDistributionConstraint(b["exp"])])
+        E.set_move_generator(TranslationGenerator(0.1))
+        return E
+
+    reps = [engine(b) for b in boxes]
+    seeds, nsteps = [500 + r for r in range(len(reps))], 150
+    stats = run_replicas(reps, nsteps, k=32, seeds=seeds)
+    assert all(st["generated"] == nsteps for st in stats)
+    for E, b, seed in zip(reps, boxes, seeds):
+        E.sync_from_device()
+        random.seed(seed)
+        np.random.seed(seed)
+        S = engine(b)
+        S.run_device(nsteps, k=32)
+        assert E.realCoordinates.tobytes() == S.realCoordinates.tobytes()
+        assert np.float32(E.totalStandardError).tobytes() == np.float32(S.totalStandardError).tobytes()
+        assert E.accepted == S.accepted
+        c, cs = E.constraints[0], S.constraints[0]
+        assert c.data["inter"].tobytes() == cs.data["inter"].tobytes()
 
 
 def test_batched_replicas_of_mixed_cells_and_constraints(small_triclinic):
