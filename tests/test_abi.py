@@ -53,3 +53,27 @@ def test_oracle_symbols():
     L = refcpu.lib()
     for name in ("ref_distances_to_indexcoords", "ref_multiple_pairs_histograms_coords", "ref_mt_next32"):
         assert hasattr(L, name)
+
+
+def test_schedule_and_replica_controls_validate_without_gpu():
+    """The tuning and replica setters touch no device: a forced schedule is
+    range-checked on the host, and releasing it always succeeds."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libfrmc_hip.so not built")
+    from fullrmc_amd import tuning
+    for bad in [(128, 2, 2), (256, 3, 2), (512, 2, 0), (256, 1, 17), (2048, 1, 1)]:
+        with pytest.raises(ValueError):
+            tuning.force_full_schedule(*bad)
+    for good in [(256, 1, 1), (512, 4, 16), (1024, 2, 3)]:
+        tuning.force_full_schedule(*good)
+    tuning.force_full_schedule()
+    tuning.set_autotune(True, forget=True)
+    L = _lib.lib()
+    L.frmc_set_replica_threads(3)
+    L.frmc_set_replica_threads(0)
+    L.frmc_set_replica_batching(0)
+    L.frmc_set_replica_batching(1)
+    ms = (ctypes.c_double * 5)()
+    steps = ctypes.c_int64(-1)
+    _lib.check(L.frmc_replica_batch_times(ms, ctypes.byref(steps)))
+    assert steps.value >= 0

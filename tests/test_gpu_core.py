@@ -190,7 +190,7 @@ def test_argument_errors_match_reference_types(small_random):
         gph.multiple_pairs_histograms_coords(np.array([10 ** 6], np.int32), *args)
 
 
-def test_config1_full_histogram_13500_atoms():
+def test_full_histogram_b2_13500_atoms():
     """B2 alloy, 13,500 atoms, PDC grid r in [0,20] @0.02: full histogram
     bit-exact against the oracle (8 threads)."""
     s = systems.b2_alloy(seed=1)

@@ -109,7 +109,7 @@ def _pdc_cons(s):
 
 
 @pytest.fixture(scope="module")
-def config0_run():
+def plain_rmc_2000_run():
     """Plain RMC, one element, 2,000 atoms in a 30 A cube (uniform, rejection
     below 2.0 A), target G(r) from a 0.1 A Gaussian-jittered copy on
     r in [0,15) at 0.05 A, translations of 0.1 A, 10,000 steps: the oracle's
@@ -121,11 +121,11 @@ def config0_run():
 
 
 @pytest.mark.parametrize("k", [1, 64])
-def test_config0_full_run_matches_oracle(config0_run, k):
+def test_plain_rmc_2000_atoms_full_run_matches_oracle(plain_rmc_2000_run, k):
     """The whole 10,000-step run, one proposal per window and speculative
     windows of up to 64: the same group, decision and total at every step, and
     the same final state, as the oracle."""
-    s, oe, olog, pys, nps = config0_run
+    s, oe, olog, pys, nps = plain_rmc_2000_run
     random.seed(7)
     np.random.seed(7)
     E = device_engine(s, [PairDistributionConstraint(s["exp"])])
@@ -402,7 +402,7 @@ def test_molecular_triclinic_translation_plus_rotation(small_triclinic):
     assert_same_state(E, oe, dlog, olog, (pys, nps))
 
 
-def test_config1_size_properties():
+def test_b2_13500_size_properties():
     """B2 alloy at full size (13,500 atoms, PDC + SFC): after a device run the
     engine's histograms equal a fresh full recompute in the symmetrised sums
     and its total equals the chain evaluated on that recompute."""
@@ -477,7 +477,7 @@ def _pdc_cons_of(s):
     return lambda osys, w: [oeng.PDC(osys, s["exp"], w)]
 
 
-def test_config3_shaped_replicas_match_independent_runs():
+def test_48_replicas_of_1000_to_20000_atoms_match_independent_runs():
     """48 replicas shaped like the 256 of the replica workload (N log-uniform
     in [1000, 20000], density 0.05 per A^3, 2.0 A minimum distance, targets
     from 0.1 A jittered copies on r in [0,15) at 0.05 A, k = 32), batched:
