@@ -131,7 +131,10 @@ constexpr int kDP = 4;  // proposals per k_delta block
 // (single-atom moves).  The views are uploaded once per run; each step
 // uploads the active replicas' windows, their proposals and the block maps
 // of the kernels in one copy.
-constexpr int32_t kChunk = 4096;         // other atoms per workgroup of k_delta_batch
+// other atoms per workgroup of k_delta_batch: 2,048 measured fastest on 256
+// replicas (3.65M moves/s; 512: 3.10M, 1,024: 3.62M, 4,096: 3.44M,
+// 8,192: 2.87M)
+constexpr int32_t kChunk = 2048;
 // proposals per workgroup of k_delta_batch: 1 measured fastest on 256
 // replicas (3.21M moves/s; 2: 3.18M, 4: 2.64M -- fewer, longer workgroups
 // leave the latency-bound kernel short of waves)
