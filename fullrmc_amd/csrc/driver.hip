@@ -330,7 +330,7 @@ constexpr int32_t kLanes = 2;  // replica groups whose steps alternate on the GP
 // host work overlaps the other lane's kernels.
 struct Lane {
     std::vector<int32_t> reps;       // replicas of the lane
-    std::vector<int32_t> act, kws, poff;
+    std::vector<int32_t> act, kws;  // replica and window size of each step slot
     int32_t nact = 0;
     bool inflight = false;
     size_t win_b = 0, map_b = 0, bytes = 0;
@@ -339,12 +339,13 @@ struct Lane {
     int32_t nd = 0, na = 0, nb = 0;
 };
 
-// All replicas' windows in one launch per kernel per step and lane: the host
-// draws every active replica's proposals (threads over replicas), uploads
-// windows, proposals and block maps in one copy, launches k_delta_batch,
-// k_eval_a_batch and (when some window needs it) k_eval_b_batch on the first
-// engine's stream, then walks each replica's totals with the reference's
-// rule exactly as the single-engine loop does.  The replicas are split into
+// All replicas' windows in one launch per kernel per step and lane.  In one
+// parallel pass over a lane's replicas the host walks each replica's totals
+// of the window in flight with the reference's rule, exactly as the
+// single-engine loop does, and draws and stages its next window; then it
+// uploads windows, proposals and block maps in one copy and launches
+// k_delta_batch, k_eval_a_batch and (when some window needs it)
+// k_eval_b_batch on the first engine's stream.  The replicas are split into
 // two lanes whose steps alternate, so one lane's host work runs while the
 // other's kernels do.  Each replica keeps its own state, streams and
 // deferred commit, so every trajectory is the one its engine alone would
@@ -412,7 +413,6 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
         l.bytes = l.win_b + align16(sizeof(Prop) * (size_t)n * kmax) + l.map_b;
         l.act.resize(n);
         l.kws.resize(n);
-        l.poff.resize(n);
         FRMC_CHECK_HIP(hipMalloc(&l.d, l.bytes));
         FRMC_CHECK_HIP(hipHostMalloc(&l.h, l.bytes, hipHostMallocDefault));
         l.A.views = d_views;
@@ -434,50 +434,55 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
             }
         return (int)FRMC_OK;
     };
+    // One parallel pass per lane step: each thread takes its replicas of the
+    // lane in turn, walks the totals of the window in flight (if any), then
+    // draws and stages the replica's next window at the next free slot of
+    // the step's window and proposal tables (atomic counters).
     Lane *cur = nullptr;
-    const std::function<void(int32_t)> prepare = [&](int32_t t) {
+    std::atomic<int32_t> nslot{0}, noff{0};
+    const std::function<void(int32_t)> step = [&](int32_t t) {
         Lane &l = *cur;
         BatchWin *hw = (BatchWin *)l.h;
         Prop *hp = (Prop *)((char *)l.h + l.win_b);
-        for (int32_t i = t; i < l.nact && !rcs[t]; i += T) {
-            TranslationRun &run = runs[l.act[i]];
-            run.kw = l.kws[i];
+        const int32_t nr = (int32_t)l.reps.size();
+        for (int32_t i = t; i < nr && !rcs[t]; i += T) {
+            const int32_t r = l.reps[i];
+            TranslationRun &run = runs[r];
+            if (run.kw > 0) {
+                fail(t, run.finish());
+                run.kw = 0;
+                if (rcs[t]) break;
+            }
+            if (run.done()) continue;
+            const int32_t kw = run.next_k();
+            const int32_t slot = nslot.fetch_add(1), off = noff.fetch_add(kw);
+            run.kw = kw;
             run.draw();
-            BatchWin &W = hw[i];
-            W.rep = l.act[i];
-            W.prop = l.poff[i];
-            fail(t, batch_prepare(run.e, run.kw, run.gids.data(), run.nbox.data(), W, hp + l.poff[i]));
-        }
-    };
-    const std::function<void(int32_t)> finish = [&](int32_t t) {
-        Lane &l = *cur;
-        for (int32_t i = t; i < l.nact && !rcs[t]; i += T) {
-            fail(t, runs[l.act[i]].finish());
-            runs[l.act[i]].kw = 0;
+            BatchWin &W = hw[slot];
+            W.rep = r;
+            W.prop = off;
+            l.act[slot] = r;
+            l.kws[slot] = kw;
+            fail(t, batch_prepare(run.e, kw, run.gids.data(), run.nbox.data(), W, hp + off));
         }
     };
     StepPool pool(T);
     for (double &x : g_batch_ms) x = 0.0;
     g_batch_steps = 0;
-    // draw, stage, upload and launch the next window of every active replica of lane l
-    auto launch_lane = [&](Lane &l) -> int {
+    // walk lane l's window in flight, draw and stage the next one, upload and
+    // launch it
+    auto step_lane = [&](Lane &l) -> int {
         const double t0 = now_ms();
-        l.nact = 0;
-        int32_t off = 0;
-        for (int32_t r : l.reps)
-            if (!runs[r].done()) {
-                l.act[l.nact] = r;
-                l.kws[l.nact] = runs[r].next_k();
-                l.poff[l.nact] = off;
-                off += l.kws[l.nact];
-                l.nact++;
-            }
+        cur = &l;
+        nslot.store(0);
+        noff.store(0);
+        pool.run(step);
+        if (int rc = first_error()) return rc;
+        l.nact = nslot.load();
+        const int32_t off = noff.load();
         l.inflight = l.nact > 0;
         if (!l.inflight) return FRMC_OK;
         g_batch_steps++;
-        cur = &l;
-        pool.run(prepare);
-        if (int rc = first_error()) return rc;
         const double t1 = now_ms();
         // block maps: (window, block within the window) per block of each
         // kernel, right after the step's proposals
@@ -506,30 +511,18 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
         // windows, proposals and maps: one copy of the used bytes
         FRMC_CHECK_HIP(hipMemcpyAsync(l.d, l.h, moff + sizeof(int2) * (nd + na + nb), hipMemcpyHostToDevice, st));
         if (int rc = launch_batch(l.A, nd, na, nb, shm_d, shm_a, shm_b, st)) return rc;
-        const double t2 = now_ms();
         g_batch_ms[0] += t1 - t0;
-        g_batch_ms[1] += t2 - t1;
+        g_batch_ms[1] += now_ms() - t1;
         return FRMC_OK;
     };
-    auto finish_lane = [&](Lane &l) -> int {
-        if (!l.inflight) return FRMC_OK;
-        const double t0 = now_ms();
-        cur = &l;
-        pool.run(finish);
-        l.inflight = false;
-        g_batch_ms[2] += now_ms() - t0;
-        return first_error();
-    };
     g_batch_ms[3] = now_ms() - t_start;
-    for (auto &l : lanes)
-        if (int rc = launch_lane(l)) return rc;
+    for (auto &l : lanes) l.inflight = true;  // the first pass only draws
     for (;;) {
         bool any = false;
         for (auto &l : lanes) {
             if (!l.inflight) continue;
             any = true;
-            if (int rc = finish_lane(l)) return rc;
-            if (int rc = launch_lane(l)) return rc;
+            if (int rc = step_lane(l)) return rc;
         }
         if (!any) break;
     }

@@ -413,9 +413,10 @@ void frmc_set_replica_batching(int enable);
    the CPUs available to the process less two, at most 16). */
 void frmc_set_replica_threads(int nthreads);
 /* Host time of the last batched replica run by phase, in ms (ms_out holds
-   5): [0] drawing and staging the proposals, [1] the uploads and launches,
-   [2] waiting for the totals and walking them, [3] set-up, [4] the final
-   commits; steps_out the lane steps. */
+   5): [0] waiting for each window's totals, walking them and drawing and
+   staging the next window (one parallel pass per lane step), [1] the
+   uploads and launches, [2] unused (0), [3] set-up, [4] the final commits;
+   steps_out the lane steps. */
 int frmc_replica_batch_times(double *ms_out, int64_t *steps_out);
 /* Per-kernel timing with HIP events recorded on the engine stream around
    the launches of every `every`-th window (two event records per timed
