@@ -56,6 +56,8 @@ float frmc_last_kernel_ms(void);
    bins, boundary kind, float64-recompute mode) times each candidate twice
    with HIP events on the call's stream, outside the frmc_set_timing window,
    and later calls of the class reuse the fastest.
+   The tuning call synchronises its stream, so the first call of a class
+   cannot be captured in a HIP graph (tune first, or force a schedule).
    frmc_set_autotune: 1 on, 0 off (fixed rule), -1 off and forget the
    tuned classes.  frmc_force_full_schedule pins one schedule for every call
    (rowAtoms == 0 releases it).  frmc_last_full_schedule reports the
@@ -402,7 +404,8 @@ int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32_t k, float
    on), all active replicas' windows go out in one launch per kernel per
    step, on the first engine's stream; otherwise each engine's windows run
    on its own stream, several in flight at once.  Engines of one call share
-   one device. */
+   one device.  After an error return the engines' states are unspecified
+   (a staged move may be half applied): destroy and recreate them. */
 int frmc_engines_run_translation(frmc_engine **engines, int32_t nengines, int64_t nsteps, int32_t k, float minAmp,
                                  float maxAmp, float tolerance, frmc_mt19937 *py_states, frmc_mt19937 *np_states,
                                  frmc_run_stats *stats);
