@@ -152,6 +152,9 @@ class System:
         self.nel = len(self.elements)
         self.n0 = len(self.real)
         self.collected = set()
+        # threads of the per-move histogram loops (the reference's ncores;
+        # private bins per thread, so counts do not depend on it)
+        self.ncores = 1
         self.real_index = np.arange(self.n0, dtype=np.int32)
 
     def rel(self, real):
@@ -201,7 +204,7 @@ class PDC:
     def hist_multi(self, idx, box):
         return refcpu.multiple_pairs_histograms_coords(idx, box, self.s.basis, True, self.s.mol, self.s.elem,
                                                        self.s.nel, self.min, self.max, self.bin, self.H,
-                                                       allAtoms=True)
+                                                       allAtoms=True, ncores=self.s.ncores)
 
     def chi2(self, intra, inter, counts=None, w=None, rho0=None):
         return chain.chi_square(self.y, self.model(intra, inter, counts, w, rho0), self.dw)
