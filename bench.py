@@ -620,6 +620,11 @@ def main():
     if roof is not None:
         roof["back_to_back_us"] = round(1e3 * b2b.value, 2)
         roof["back_to_back_k"] = kwin
+    # the batched delta kernel against the same FP32 peak (70 ops per
+    # proposal and other atom, old and new position)
+    idel = _lib.KERNEL_NAMES.index("k_delta")
+    roof_delta = kernel_roofline("k_delta", ms[idel] * 1e-3 / max(1, launches[idel]), n,
+                                 evaluated / max(1, windows))
 
     result = {"metric": "bit-exact-trajectory RMC moves/s + full-histogram Gpairs/s, GPU vs host CPU",
               "value": round(value, 1), "unit": "moves/s", "n_gpus": 1, "steps": args.steps,
@@ -630,6 +635,7 @@ def main():
                                      "single-atom translation 0.1 A, k=%d speculative windows" % args.k,
                          "atoms": n, "moves_per_step": args.moves, "k": args.k},
               "roofline": roof,
+              "roofline_k_delta": roof_delta,
               "accepted_fraction": round(accepted / moves, 4),
               "windows": windows, "proposals_evaluated": evaluated, "kernels": kernels}
 
@@ -652,7 +658,11 @@ def main():
         t1 = time.perf_counter()
         E1.run_device(1000, k=1, pull=False)
         torch.cuda.synchronize()
-        result["plain_one_move_per_window_moves_per_s"] = round(1000 / (time.perf_counter() - t1), 1)
+        plain = 1000 / (time.perf_counter() - t1)
+        result["plain_one_move_per_window_moves_per_s"] = round(plain, 1)
+        # algorithmic bytes of one move's delta: 20 B per other atom (xyz,
+        # molecule, element)
+        result["plain_one_move_per_window_gb_per_s"] = round(20.0 * n * plain / 1e9, 3)
         # the reference's loop structure calling the drop-in extensions per move
         E2 = make_engine(start, pdc, sfc, seed=1)
         E2.run(2)
