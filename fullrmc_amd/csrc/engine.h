@@ -29,7 +29,8 @@ namespace frmc {
 constexpr int kMaxC = 4;   // constraints per engine
 constexpr int kMaxG = 64;  // atoms per group
 constexpr int kQC = 64;    // Q points per k_eval_a wave (one per lane)
-constexpr int kEW = 1;     // waves per k_eval_a block: a work unit is kEW*kQC Q points
+constexpr int kEW = 1;     // S(Q) waves per k_eval_a block: a work unit is kEW*kQC Q points
+constexpr int kEvalThreads = 256;  // k_eval_a block: all threads build G(r), wave 0 runs the S(Q) sum
 
 struct DevC {
     int32_t kind;   // FRMC_PDC, FRMC_SFC, FRMC_PCF or FRMC_RSFC

@@ -309,18 +309,18 @@ __device__ float block_pairwise(const float *a, const DevC &c, float *scratch, i
 
 // --------------------------------------------------------------- k_eval_a
 
-// One block of kEW waves per (work unit, proposal): grid (nunits, k).  A work
-// unit is one r-space constraint (PDC, PCF), or kEW*64 Q points of one S(Q)
-// constraint (SFC, RSFC).  The block rebuilds the proposal's G(r) from
-// base + delta in the reference's float32 operation order
-// (PairDistributionConstraints.py:854-895) into LDS, its waves sharing the
-// bins, then
+// One block of kEvalThreads threads per (work unit, proposal): grid (nunits,
+// k).  A work unit is one r-space constraint (PDC, PCF), or 64 Q points of
+// one S(Q) constraint (SFC, RSFC).  All threads rebuild the proposal's G(r)
+// from base + delta in the reference's float32 operation order
+// (PairDistributionConstraints.py:854-895) into LDS (a few bins each, so
+// their loads go out in few rounds), then
 //   PDC: squared deviations of G(r) and their NumPy pairwise sum (:810-838)
 //        -> chi2d;
 //   PCF: the same for g(r), the G(r) sum over the shell volumes, with the
 //        scale factor applied through G(r) (PairCorrelationConstraints.py:
 //        126-169);
-//   SFC: S(Q) of its Q points, one per lane, as the sequential float32 sum
+//   SFC: S(Q) of its Q points, one per lane of wave 0, as the sequential float32 sum
 //        over r of G[r]*M[r,q] (StructureFactorConstraints.py:772-778), the
 //        matrix column streamed from HBM/L2 (transposed, a batch of rows
 //        ahead in registers) and G broadcast from LDS -> squared deviations;
@@ -367,7 +367,8 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
     const int32_t *base = c.base;
     // S(Q) for q = qc*kQC + lane (SFC units): the first rows of the matrix
     // column and the experimental value load now, under the G(r) phase
-    const int32_t q = ((u - c.uOff) * kEW + w) * kQC + lane;
+    static_assert(kEW == 1, "one S(Q) wave per block");
+    const int32_t q = (u - c.uOff) * kQC + lane;
     const bool qok = !rsp && q < M;
     const float *mcol = c.sqm + (int64_t)min(q, M - 1) * c.hpb;  // M[r][q] at mcol[r], 16-byte aligned
     constexpr int B = 32;                        // rows per register batch (hpb is a multiple of B)
@@ -382,7 +383,7 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
         }
     };
     float ex_q = 0.0f, dw_q = 1.0f;
-    if (!rsp) {
+    if (!rsp && w == 0) {
         load_m(ma, 0);
         ex_q = c.expv[min(q, M - 1)];
         if (c.hasW) dw_q = c.dataW[min(q, M - 1)];
@@ -391,7 +392,7 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
         // U bins per lane per pass; every load is unconditional (clamped index,
         // valid pointer): a load under a divergent branch is followed by its
         // own wait, which serialises the batch
-        constexpr int NT = 64 * kEW;
+        constexpr int NT = kEvalThreads;
         constexpr int U = (1024 + NT - 1) / NT;  // bins per thread per pass: 1024 per pass
         const float *expv = rsp ? c.expv : c.shellV;                 // S(Q): unused (expv has M entries)
         const float *dataW = (rsp && c.hasW) ? c.dataW : c.shellV;   // unused unless hasW
@@ -480,6 +481,7 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
         }
         return;
     }
+    if (w != 0) return;  // the S(Q) sum: one Q per lane of wave 0
     // The S(Q) sum starts from +0, as NumPy's add reduction does (a sum of
     // -0 products is +0).
     float acc = 0.0f;
@@ -541,7 +543,7 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
 }
 
 // grid (nunits, k)
-__global__ void __launch_bounds__(64 * kEW) k_eval_a(View V, int32_t k) {
+__global__ void __launch_bounds__(kEvalThreads) k_eval_a(View V, int32_t k) {
     extern __shared__ float sm[];
     eval_a_body(V, V.dn, blockIdx.y, blockIdx.x, sm, nullptr);
 }
@@ -786,7 +788,7 @@ __global__ void __launch_bounds__(256) k_delta_batch(BatchArgs A) {
     }
 }
 
-__global__ void __launch_bounds__(64 * kEW) k_eval_a_batch(BatchArgs A) {
+__global__ void __launch_bounds__(kEvalThreads) k_eval_a_batch(BatchArgs A) {
     extern __shared__ float sm[];
     const int2 m = A.map_a[blockIdx.x];
     const BatchWin &W = A.win[m.x];
@@ -937,7 +939,7 @@ int launch_eval(const View &V, int32_t k, hipStream_t st) {
     const size_t shm_a = eval_lds_bytes(V);
     if (shm_a > 48 * 1024)
         (void)hipFuncSetAttribute((const void *)k_eval_a, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_a);
-    k_eval_a<<<dim3((unsigned)V.nunits, (unsigned)k), 64 * kEW, shm_a, st>>>(V, k);
+    k_eval_a<<<dim3((unsigned)V.nunits, (unsigned)k), kEvalThreads, shm_a, st>>>(V, k);
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;
 }
@@ -958,7 +960,7 @@ int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t 
     if (shm_a > 48 * 1024)
         (void)hipFuncSetAttribute((const void *)k_eval_a_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shm_a);
-    k_eval_a_batch<<<(unsigned)na, 64 * kEW, shm_a, st>>>(A);
+    k_eval_a_batch<<<(unsigned)na, kEvalThreads, shm_a, st>>>(A);
     FRMC_CHECK_LAUNCH();
     if (nb == 0) return FRMC_OK;  // every window's total came from k_eval_a_batch
     if (shm_b > 48 * 1024)
