@@ -229,17 +229,19 @@ def kernel_roofline(name, avg_s, n, props_per_launch):
     if not avg_s > 0:
         return None
     tf = flops / avg_s / 1e12
+    # HBM bytes per launch: the PMC per-proposal figure times this timed
+    # region's proposals per launch
     traffic = None
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic_engine_b2_13500.json")) as f:
-            traffic = json.load(f)["kernels"][name]["traffic_bytes"]
+            traffic = int(round(json.load(f)["kernels"][name]["traffic_bytes_per_proposal"] * props_per_launch))
     except (OSError, KeyError, ValueError):
         pass
     return {"bound": "mfma", "compute": "fp32 VALU (no matrix instructions; MI355X fp32 vector and dense "
             "matrix peaks are both 157.3 TFLOP/s)", "achieved": round(tf, 4), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32_TFLOPS, 6), "traffic": traffic,
             "traffic_source": "profiles/hbm_traffic_engine_b2_13500.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                              "copy-calibrated)" if traffic is not None else None,
+                              "copy-calibrated, per proposal x proposals per launch)" if traffic is not None else None,
             "kernel": name, "avg_launch_us": round(avg_s * 1e6, 2), "flops_per_launch": flops}
 
 
