@@ -126,7 +126,10 @@ struct CommitCtl {
     float u[kInline];
 };
 
-constexpr int kDP = 4;  // proposals per k_delta block
+// proposals per k_delta block: 1 measured fastest on the 13,500-atom
+// workload (k_delta 17.3 us; 2: 17.5, 4: 18.7) -- more blocks, fewer
+// proposals' atomics per thread
+constexpr int kDP = 1;
 
 // Windows of many engines (replicas) launched together, one launch per kernel
 // (single-atom moves).  The views are uploaded once per run; each step
