@@ -559,8 +559,16 @@ __device__ __forceinline__ void eval_b_body(const View &V, int32_t *dn_zero, int
     // zero a slice of the other delta buffer for the next window
     for (int64_t s = (int64_t)p * blockDim.x + threadIdx.x; s < zero_len; s += (int64_t)nb * blockDim.x)
         dn_zero[s] = 0;
+    // the r-space chi-squares (k_eval_a's partials) load first, under the
+    // S(Q) sums
+    float rchi[kMaxC];
+#pragma unroll
+    for (int ci = 0; ci < kMaxC; ci++)
+        rchi[ci] = (ci < V.C && V.c[ci].rspace) ? load_dev(V.chi2d + (int64_t)p * V.C + ci) : 0.0f;
     float tse = 0.0f;
-    for (int32_t ci = 0; ci < V.C; ci++) {
+#pragma unroll
+    for (int ci = 0; ci < kMaxC; ci++) {
+        if (ci >= V.C) break;
         const DevC &c = V.c[ci];
         float chi;
         if (!c.rspace) {
@@ -574,7 +582,7 @@ __device__ __forceinline__ void eval_b_body(const View &V, int32_t *dn_zero, int
             chi = threadIdx.x == 0 ? run_prog(lprog, c.nprog, scratch, scratch + c.nleaves) : 0.0f;
             __syncthreads();
         } else {
-            chi = load_dev(V.chi2d + (int64_t)p * V.C + ci);
+            chi = rchi[ci];
         }
         // np.sum([SD_c / varianceSquared]) with varianceSquared == 1: fewer
         // than 8 terms are summed sequentially from 0
