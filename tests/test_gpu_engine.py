@@ -513,6 +513,44 @@ def test_48_replicas_of_1000_to_20000_atoms_match_independent_runs():
         assert c.data["inter"].tobytes() == cs.data["inter"].tobytes()
 
 
+def test_batched_replicas_with_tolerance_match_independent_runs():
+    """A tolerance above zero makes the accept rule draw for worse moves and
+    keep some of them: the batched walk rewinds both streams past those
+    draws exactly as the single-engine loop does."""
+    from fullrmc_amd.Engine import run_replicas
+    boxes = []
+    for i, n in enumerate((350, 900, 600)):
+        L = (n / 0.05) ** (1 / 3)
+        b = systems.random_box(n, L, 2.0, seed=600 + i)
+        b["exp"] = pdc_target(b, 0.1, 610 + i, 0.025, 0.05, 150)
+        boxes.append(b)
+
+    def engine(b):
+        E = device_engine(b, [PairDistributionConstraint(b["exp"])])
+        E.set_move_generator(TranslationGenerator(0.2))
+        E.set_tolerance(25.0)
+        return E
+
+    reps = [engine(b) for b in boxes]
+    seeds, nsteps = [620, 621, 622], 300
+    stats = run_replicas(reps, nsteps, k=16, seeds=seeds)
+    assert all(st["generated"] == nsteps for st in stats)
+    tolerated = 0
+    for E, b, seed in zip(reps, boxes, seeds):
+        E.sync_from_device()
+        tolerated += E.tolerated
+        random.seed(seed)
+        np.random.seed(seed)
+        S = engine(b)
+        S.run_device(nsteps, k=16)
+        assert E.realCoordinates.tobytes() == S.realCoordinates.tobytes()
+        assert np.float32(E.totalStandardError).tobytes() == np.float32(S.totalStandardError).tobytes()
+        assert (E.accepted, E.tolerated) == (S.accepted, S.tolerated)
+        assert E._replica_rng[0] == random.getstate()
+        assert np.array_equal(E._replica_rng[1][1], np.random.get_state()[1])
+    assert tolerated > 0
+
+
 def test_batched_replicas_of_mixed_cells_and_constraints(small_triclinic):
     """One batched run over replicas that differ in everything a window
     kernel reads: atom count, cubic or triclinic cell, one or two elements,
