@@ -101,10 +101,13 @@ static inline void to_box(const float M[9], const float r[3], float b[3]) {
 constexpr int32_t kReplicaThreads = 16;  // host threads driving independent replicas
 
 // Window size from the running acceptance rate: enough proposals that the
-// window holds an acceptance with ~95 % probability, within [kmin, kmax].
+// window holds an acceptance with ~90 % probability, within [kmin, kmax].
+// (90 % measured 2-3 % faster than 95 % on the 13,500-atom workload, and
+// 98 % slower: fewer proposals evaluated after the first acceptance against
+// more windows; the trajectory does not depend on it.)
 static int32_t window_size(int64_t generated, int64_t accepted, int32_t kmax) {
     const double p = (accepted + 1.0) / (generated + 2.0);
-    double k = p >= 1.0 ? 1.0 : log(0.05) / log(1.0 - p);
+    double k = p >= 1.0 ? 1.0 : log(0.10) / log(1.0 - p);
     const int32_t kmin = std::min<int32_t>(kmax, 4);
     int32_t kk = (int32_t)std::ceil(k);
     return std::max(kmin, std::min(kmax, kk));
