@@ -198,6 +198,11 @@ class PDC:
         return chain.total_gr(intra, inter, s.elements, counts, w, s.volume, rho0, self.vols, self.centers, self.sf)
 
     def hist_full(self, box, mol, elem):
+        if len(box) > 10000:
+            # large boxes: the cell-list form on every core (equal to the
+            # all-pairs loop, pinned in tests/test_oracle_pins.py)
+            return refcpu.full_pairs_histograms_coords_fast(box, self.s.basis, True, mol, elem, self.s.nel,
+                                                            self.min, self.max, self.bin, self.H)
         return refcpu.full_pairs_histograms_coords(box, self.s.basis, True, mol, elem, self.s.nel, self.min,
                                                    self.max, self.bin, self.H)
 

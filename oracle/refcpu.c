@@ -361,6 +361,126 @@ int64_t ref_full_pairs_counts_f64(const float *coords, int32_t n, const float *B
     return overflow;
 }
 
+/* Cell-list form of the periodic full histogram, for large boxes: the same
+   per-pair arithmetic on the same unordered pairs (row atom i = the earlier
+   index, column j > i) as full_pairs_histograms_coords (f64 == 0, float32
+   reference binning with ref_bin_one) or ref_full_pairs_counts_f64
+   (f64 != 0), but a pair is evaluated only if the fractional cells of i and
+   j are neighbours (index offsets -1, 0, +1 along every axis, periodic).
+   Axis a gets m_a = floor(h_a / w) cells, h_a = V / |b x c| the spacing of
+   the cell's lattice planes and w = maxD (1 + 1e-5) + 0.01 A.  Two atoms
+   whose cells are not neighbours along axis a have a periodic fractional
+   difference of at least 1/m_a there; any image vector r = b.B then has a
+   component of at least h_a / m_a >= w along the plane normal, so its
+   length is above maxD by more than the float32 rounding of the reference's
+   difference, image shift and norm (about 1e-5 A at these sizes): such a
+   pair is out of range in both arithmetics and counts nowhere.  counts:
+   int32 [2][nel][nel][H] (zeroed here).  Returns the number of past-the-end
+   bins, or -1 when an axis has fewer than 3 cells (the all-pairs loops must
+   be used then). */
+int64_t ref_full_pairs_counts_cells(const float *coords, int32_t n, const float *B, const int32_t *mol,
+                                    const int32_t *elem, int32_t nel, float minD, float maxD, float bin, int32_t H,
+                                    int f64, int32_t *counts, int nthreads) {
+    const int64_t size = (int64_t)nel * nel * H;
+    double Bd[9];
+    for (int k = 0; k < 9; k++) Bd[k] = (double)B[k];
+    /* plane spacings h_a = V / |row_b x row_c| */
+    int32_t m[3];
+    const double w = (double)maxD * (1.0 + 1e-5) + 0.01;
+    const double vol = fabs(Bd[0] * (Bd[4] * Bd[8] - Bd[5] * Bd[7]) - Bd[1] * (Bd[3] * Bd[8] - Bd[5] * Bd[6]) +
+                            Bd[2] * (Bd[3] * Bd[7] - Bd[4] * Bd[6]));
+    for (int a = 0; a < 3; a++) {
+        const double *u = Bd + 3 * ((a + 1) % 3), *v = Bd + 3 * ((a + 2) % 3);
+        const double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
+        const double h = vol / sqrt(cx * cx + cy * cy + cz * cz);
+        m[a] = (int32_t)floor(h / w);
+        if (m[a] < 3) return -1;
+    }
+    const int64_t ncell = (int64_t)m[0] * m[1] * m[2];
+    int64_t *start = (int64_t *)calloc(ncell + 1, sizeof(int64_t));
+    int32_t *cell_of = (int32_t *)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+    for (int32_t i = 0; i < n; i++) {
+        int32_t c3[3];
+        for (int a = 0; a < 3; a++) {
+            const double x = (double)coords[3 * (int64_t)i + a];
+            int32_t c = (int32_t)((x - floor(x)) * m[a]);
+            c3[a] = c < 0 ? 0 : (c >= m[a] ? m[a] - 1 : c);
+        }
+        cell_of[i] = (c3[0] * m[1] + c3[1]) * m[2] + c3[2];
+        start[cell_of[i] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; c++) start[c + 1] += start[c];
+    {
+        int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * ncell);
+        memcpy(fill, start, sizeof(int64_t) * ncell);
+        for (int32_t i = 0; i < n; i++) order[fill[cell_of[i]]++] = i;
+        free(fill);
+    }
+    memset(counts, 0, sizeof(int32_t) * 2 * size);
+    const double mn = minD, mx = maxD, bn = bin;
+    int64_t overflow = 0;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads) reduction(+ : overflow)
+    {
+        int32_t *pc = counts;
+#ifdef _OPENMP
+        if (omp_get_num_threads() > 1) pc = (int32_t *)calloc(2 * size, sizeof(int32_t));
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int64_t c = 0; c < ncell; c++) {
+            const int32_t cz0 = (int32_t)(c % m[2]), cy0 = (int32_t)((c / m[2]) % m[1]), cx0 = (int32_t)(c / ((int64_t)m[1] * m[2]));
+            for (int o = 0; o < 27; o++) {
+                const int32_t nx = (cx0 + o / 9 - 1 + m[0]) % m[0];
+                const int32_t ny = (cy0 + (o / 3) % 3 - 1 + m[1]) % m[1];
+                const int32_t nz = (cz0 + o % 3 - 1 + m[2]) % m[2];
+                const int64_t c2 = ((int64_t)nx * m[1] + ny) * m[2] + nz;
+                for (int64_t p = start[c]; p < start[c + 1]; p++) {
+                    const int32_t i = order[p];
+                    const float *a = coords + 3 * (int64_t)i;
+                    for (int64_t q = start[c2]; q < start[c2 + 1]; q++) {
+                        const int32_t j = order[q];
+                        if (j <= i) continue;
+                        if (!f64) {
+                            const float d = ref_pbc_dist(a[0], a[1], a[2], coords + 3 * (int64_t)j, B);
+                            ref_bin_one(d, elem[i], mol[i], mol[j], elem[j], nel, H, minD, maxD, bin, pc, pc + size,
+                                        &overflow);
+                            continue;
+                        }
+                        const double dx = (double)a[0] - (double)coords[3 * (int64_t)j];
+                        const double dy = (double)a[1] - (double)coords[3 * (int64_t)j + 1];
+                        const double dz = (double)a[2] - (double)coords[3 * (int64_t)j + 2];
+                        const double bx = dx - ref_round64(dx), by = dy - ref_round64(dy), bz = dz - ref_round64(dz);
+                        const double rx = (bx * Bd[0] + by * Bd[3]) + bz * Bd[6];
+                        const double ry = (bx * Bd[1] + by * Bd[4]) + bz * Bd[7];
+                        const double rz = (bx * Bd[2] + by * Bd[5]) + bz * Bd[8];
+                        const double d = sqrt((rx * rx + ry * ry) + rz * rz);
+                        if (d < mn || d >= mx) continue;
+                        const int64_t b = (int64_t)((d - mn) / bn);
+                        const int64_t flat = ((int64_t)elem[i] * nel + elem[j]) * H + b;
+                        if (flat >= size) {
+                            overflow++;
+                            continue;
+                        }
+                        pc[(mol[i] == mol[j] ? 0 : size) + flat]++;
+                    }
+                }
+            }
+        }
+#ifdef _OPENMP
+        if (pc != counts) {
+#pragma omp critical
+            for (int64_t s = 0; s < 2 * size; s++) counts[s] += pc[s];
+            free(pc);
+        }
+#endif
+    }
+    free(start);
+    free(cell_of);
+    free(order);
+    return overflow;
+}
+
 /* ---------------------------------------------------- boundary conditions */
 
 /* get_reciprocal_basis (boundary_conditions_collection.pyx:45-80).  The

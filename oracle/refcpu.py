@@ -42,6 +42,7 @@ def lib():
             "ref_multiple_pairs_histograms_dists": ([_p, _i, _p, _i, _i, _p, _p, _i, _f, _f, _f, _i, _int, _p, _p],
                                                     ctypes.c_int64),
             "ref_full_pairs_counts_f64": ([_p, _i, _p, _int, _p, _p, _i, _f, _f, _f, _i, _p, _int], ctypes.c_int64),
+            "ref_full_pairs_counts_cells": ([_p, _i, _p, _p, _p, _i, _f, _f, _f, _i, _int, _p, _int], ctypes.c_int64),
             "ref_get_reciprocal_basis": ([_p, _p], ctypes.c_double),
             "ref_transform_coordinates": ([_p, _p, _i, _p], None),
             "ref_box_coordinates_real_distances": ([_i, _p, _i, _p, _p, _p], None),
@@ -213,6 +214,46 @@ def full_pairs_counts_f64(boxCoords, basis, isPBC, moleculeIndex, elementIndex, 
                                          _ptr(counts), int(ncores))
     counts = counts.astype(np.int64).reshape(2, nel, nel, H)
     return counts[0], counts[1], ov
+
+
+def full_pairs_counts_cells(boxCoords, basis, moleculeIndex, elementIndex, numberOfElements, minDistance,
+                            maxDistance, bin, histSize, float64=False, ncores=None):
+    """Periodic full histogram by a cell list (refcpu.c
+    ref_full_pairs_counts_cells): the float32 reference binning of
+    full_pairs_histograms_coords, or with float64=True the binary64 recompute
+    of full_pairs_counts_f64, evaluating only pairs in neighbouring cells
+    (the others are provably out of range).  Returns int64 (intra, inter)
+    counts and the past-the-end count, or None when the cell is too small
+    for three cells per axis.  ncores None: every CPU of the affinity set."""
+    c, b = _c(boxCoords, np.float32), _c(basis, np.float32)
+    m, e = _c(moleculeIndex, np.int32), _c(elementIndex, np.int32)
+    nel, H = int(numberOfElements), int(histSize)
+    if ncores is None:
+        ncores = len(os.sched_getaffinity(0))
+    counts = np.zeros(2 * nel * nel * H, np.int32)
+    ov = lib().ref_full_pairs_counts_cells(_ptr(c), c.shape[0], _ptr(b), _ptr(m), _ptr(e), nel,
+                                           np.float32(minDistance), np.float32(maxDistance), np.float32(bin), H,
+                                           int(bool(float64)), _ptr(counts), int(ncores))
+    if ov < 0:
+        return None
+    counts = counts.astype(np.int64).reshape(2, nel, nel, H)
+    return counts[0], counts[1], ov
+
+
+def full_pairs_histograms_coords_fast(boxCoords, basis, isPBC, moleculeIndex, elementIndex, numberOfElements,
+                                      minDistance, maxDistance, bin, histSize, ncores=None):
+    """full_pairs_histograms_coords (pairs_histograms.pyx:289-335) through
+    the cell list when the periodic cell holds three cells per axis, else the
+    all-pairs loop on ncores threads: the same float32 (hintra, hinter)."""
+    if ncores is None:
+        ncores = len(os.sched_getaffinity(0))
+    if isPBC:
+        r = full_pairs_counts_cells(boxCoords, basis, moleculeIndex, elementIndex, numberOfElements, minDistance,
+                                    maxDistance, bin, histSize, ncores=ncores)
+        if r is not None:
+            return r[0].astype(np.float32), r[1].astype(np.float32)
+    return full_pairs_histograms_coords(boxCoords, basis, isPBC, moleculeIndex, elementIndex, numberOfElements,
+                                        minDistance, maxDistance, bin, histSize, ncores=ncores)
 
 
 def multiple_pairs_histograms_dists(indexes, distances, moleculeIndex, elementIndex, numberOfElements,

@@ -539,3 +539,34 @@ def test_float64_recompute_oracle_vs_brute_force(small_triclinic):
     assert hi.sum() + he.sum() == fi.sum() + fe.sum()
     moved = np.abs(hi - fi).sum() + np.abs(he - fe).sum()
     assert moved <= 0.002 * (hi.sum() + he.sum())
+
+
+@pytest.mark.parametrize("cell", ["cubic", "triclinic"])
+def test_cell_list_full_histogram_equals_all_pairs(cell):
+    """The cell-list form of the full histogram (the checker used at
+    200,000 atoms) counts exactly the pairs of the all-pairs loops: float32
+    reference binning and the binary64 recompute, box coordinates outside
+    [0, 1) included (cells are assigned after wrapping), three or more cells
+    per axis; with fewer it declines (None)."""
+    rng = np.random.default_rng(17 if cell == "cubic" else 18)
+    if cell == "cubic":
+        B = systems.cubic_basis(38.0)
+    else:
+        B = systems.triclinic_basis(40.0, 42.0, 44.0, 80.0, 95.0, 105.0)
+    R = systems.reciprocal(B)
+    n = 4000
+    frac = (rng.random((n, 3)) * 3.0 - 1.0).astype(F32)  # several images of the cell
+    real = (frac.astype(np.float64) @ B.astype(np.float64)).astype(F32)
+    box = systems.box_coordinates(R, real)
+    el = rng.integers(0, 3, n).astype(np.int32)
+    mol = (np.arange(n) // 5).astype(np.int32)
+    for maxD, H in ((9.5, 950), (12.0, 1200)):
+        c32 = refcpu.full_pairs_counts_cells(box, B, mol, el, 3, 0.0, maxD, 0.01, H)
+        assert c32 is not None
+        hi, he = refcpu.full_pairs_histograms_coords(box, B, True, mol, el, 3, 0.0, maxD, 0.01, H, ncores=4)
+        assert np.array_equal(c32[0], hi) and np.array_equal(c32[1], he)
+        assert c32[1].sum() > 100000
+        c64 = refcpu.full_pairs_counts_cells(box, B, mol, el, 3, 0.0, maxD, 0.01, H, float64=True)
+        r64 = refcpu.full_pairs_counts_f64(box, B, True, mol, el, 3, 0.0, maxD, 0.01, H, ncores=4)
+        assert np.array_equal(c64[0], r64[0]) and np.array_equal(c64[1], r64[1]) and c64[2] == r64[2]
+    assert refcpu.full_pairs_counts_cells(box, B, mol, el, 3, 0.0, 14.0, 0.01, 1400) is None
