@@ -1,19 +1,28 @@
 """Benchmark of the pair-distance / pair-histogram hot path on one MI355X.
 
-Workload (N=1): the binary B2 alloy of 13,500 atoms (15x15x30 cells,
-a = 3.01 A, 50/50 species, Gaussian jitter sigma = 0.08 A) with a
-PairDistributionConstraint (r in [0,20] A at 0.02 A) and a
-StructureFactorConstraint (own grid r in [0,20] A at 0.02 A, Q = 0.5..25 1/A,
-500 points), single-atom translation moves of amplitude 0.1 A.  The targets
-are float64 G(r)/S(Q) of an independently jittered copy of the lattice.
+Headline workload (N=1): the SiO2-like box of 200,000 atoms (66,667 Si,
+133,333 O, uniform in a 145 A cube with a 1.5 A minimum distance), the
+largest configuration of the benchmark set.  One O(N^2) full histogram on
+r in [0, 20) at 0.01 A (2,000 bins; float32 reference binning and the
+binary64-recompute mode), then single-atom translation moves of 0.1 A with a
+PairDistributionConstraint (the same 2,000 bins) and a
+StructureFactorConstraint (own r grid [0, 20) at 0.01 A, Q = 0.3..30 1/A,
+1,000 points), windows of up to 128 speculative proposals.  The targets are
+the float64 G(r) / S(Q) of an independent box of the same composition.
 
-One step = ``--moves`` generated RMC moves through the device-resident
-engine with k speculative proposals per window (bit-exact trajectory).
-``value`` = generated moves per second over the timed steps.  Also reported:
-the plain one-proposal-per-window form and the drop-in per-move loop on the
-same GPU, full-histogram Gpairs/s (one O(N^2) recompute), the roofline of the
-dominant kernel from HIP events recorded around every launch during the
-timed region, and the CPU oracle timed on this host.
+One step = ``--moves`` generated moves (default 5,000) through the
+device-resident engine; the warm-up steps are the run's fixed
+pre-equilibration, and the timed steps (default 20) are the 100,000 moves of
+the workload.  ``value`` = generated moves per second over the timed steps,
+with every input resident in HBM.  The trajectory of the first timed step is
+replayed by the CPU oracle from the state at the start of the timed region
+(same random streams) for the CPU baseline, and its moves are checked
+against the device's (``parity_checked_moves``).
+
+Also reported: the plain one-proposal-per-window form on the same box; the
+13,500-atom B2 alloy (G(r) + S(Q)), the 256 batched replicas and the 6,500-
+atom molecular triclinic box with their own CPU baselines; the S(Q)
+transform entry; the roofline of the dominant kernel.
 """
 import argparse
 import ctypes
@@ -152,30 +161,39 @@ def gpu_full_histogram(start, elem, dr, H, reps=3, float64=False, autotune=True)
 
 
 def cpu_engine(start, pdc, sfc, dr, seed=7, init_from=None):
-    """The oracle engine on the workload; init_from (a device Engine, synced to
-    the host) supplies the coordinates and slot-exact histograms instead of an
-    O(N^2) oracle recompute."""
+    """The oracle engine on the workload; init_from (a snapshot of a device
+    engine, see snapshot()) supplies the coordinates and slot-exact
+    histograms instead of an O(N^2) oracle recompute, and the oracle's
+    total must then equal the snapshot's."""
     from oracle import chain
     from oracle import engine as oeng
     from fullrmc_amd.Constraints.PairDistributionConstraints import ATOMIC_NUMBER
     random.seed(seed)
     np.random.seed(seed)
-    real = start["real"] if init_from is None else init_from.realCoordinates
+    real = start["real"] if init_from is None else init_from["real"]
     osys = oeng.System(real, start["elements"], start["molecules"], start["basis"], start["recip"])
     w = chain.normalized_weighting(osys.counts, {e: ATOMIC_NUMBER[e] for e in osys.elements})
     oe = oeng.Engine(osys, [[i] for i in range(len(start["real"]))], oeng.Translation(0.1),
                      [oeng.PDC(osys, pdc, w), oeng.SFC(osys, sfc, w, 0.0, 20.0, dr)])
-    if init_from is not None:
-        assert osys.box.tobytes() == np.asarray(init_from.boxCoordinates, np.float32).tobytes()
     if init_from is None:
         oe.initialize()
-    else:
-        for c, dc in zip(oe.constraints, init_from.constraints):
-            c.intra = np.array(dc.data["intra"], dtype=np.float32)
-            c.inter = np.array(dc.data["inter"], dtype=np.float32)
-            c.sd = c.chi2(c.intra, c.inter)
-        oe.tse = chain.total_error([c.sd for c in oe.constraints])
+        return oe
+    assert osys.box.tobytes() == init_from["box"].tobytes()
+    for c, (hi, he) in zip(oe.constraints, init_from["hist"]):
+        c.intra, c.inter = np.array(hi, np.float32), np.array(he, np.float32)
+        c.sd = c.chi2(c.intra, c.inter)
+    oe.tse = chain.total_error([c.sd for c in oe.constraints])
+    assert np.float32(oe.tse).tobytes() == np.float32(init_from["tse"]).tobytes()
     return oe
+
+
+def snapshot(E):
+    """The device engine's state (coordinates, histograms, total) and both
+    global random streams, for an oracle replay."""
+    E.sync_from_device()
+    return {"real": np.array(E.realCoordinates, np.float32), "box": np.array(E.boxCoordinates, np.float32),
+            "hist": [(np.array(c.data["intra"]), np.array(c.data["inter"])) for c in E.constraints],
+            "tse": np.float32(E.totalStandardError), "py": random.getstate(), "np": np.random.get_state()}
 
 
 def cpu_moves_per_s(oe, seconds):
@@ -205,44 +223,58 @@ def cpu_full_histogram_sample(start, elem, dr, H, rows, threads):
     return pairs / (time.perf_counter() - t0), pairs
 
 
-def eval_flops_per_proposal(H_pdc=1000, H_sfc=1000, nQ=500, T=3):
-    """k_eval per proposal: G(r) (T multiply-divide-add + divide, subtract,
+# fp32 VALU without FMA and without packed FP32 (the reference's float32
+# operation order forbids contraction): one operation per lane per cycle,
+# 256 CUs x 64 lanes x 2.4 GHz
+ATTAINABLE_FP32_TOPS = 39.3
+
+
+def eval_flops_per_proposal(H_pdc=2000, H_sfc=2000, nQ=1000, T=3):
+    """k_eval_a per proposal: G(r) (T multiply-divide-add + divide, subtract,
     multiply per bin), S(Q) (one product and one add per (r, Q)), squared
     deviations (3 per point) and the pairwise sums (1 per point)."""
     gr = (3 * T + 3) * (H_pdc + H_sfc)
     return gr + 2.0 * H_sfc * nQ + 4.0 * (H_pdc + nQ)
 
 
-def kernel_roofline(name, avg_s, n, props_per_launch):
-    """Algorithmic work of one launch of the named engine kernel, against the
-    FP32 vector peak (the pair path issues no matrix instructions)."""
+def kernel_roofline(name, avg_s, n, props_per_launch, shape=None, traffic=None, traffic_source=None):
+    """Algorithmic work of one launch of the named engine kernel against the
+    FP32 vector peak: the pair path issues no matrix instructions.  shape:
+    (H_pdc, H_sfc, nQ, T) of the evaluation.  traffic: measured HBM bytes per
+    launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of the same command), if any."""
+    shape = shape or (2000, 2000, 1000, 3)
     if name == "k_delta":
         flops = OPS_PER_PAIR * 2.0 * n * props_per_launch      # old + new position vs all atoms
+        limit = "latency of the per-pair atomics into the proposal's delta block; VALU distance arithmetic"
     elif name == "k_eval_a":
-        flops = eval_flops_per_proposal() * props_per_launch
-    elif name == "k_eval_b":
-        flops = 2.0 * 500 * props_per_launch                   # SFC squared deviations summed, totals
-    elif name == "k_commit":
-        flops = OPS_PER_PAIR * 2.0 * n
+        flops = eval_flops_per_proposal(*shape) * props_per_launch
+        limit = ("latency: the S(Q) sum of each Q is H dependent float32 adds in the reference's "
+                 "sequential order, after the G(r) build")
     else:
         return None
     if not avg_s > 0:
         return None
     tf = flops / avg_s / 1e12
-    # HBM bytes per launch: the PMC per-proposal figure times this timed
-    # region's proposals per launch
-    traffic = None
+    return {"bound": "valu", "limited_by": limit,
+            "compute": "fp32 VALU, no matrix instructions (bit-exact float32 forbids FMA and packed FP32)",
+            "achieved": round(tf, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / PEAK_FP32_TFLOPS, 6),
+            "attainable_peak": ATTAINABLE_FP32_TOPS,
+            "frac_of_attainable": round(tf / ATTAINABLE_FP32_TOPS, 6),
+            "traffic": traffic, "traffic_source": traffic_source,
+            "kernel": name, "avg_launch_us": round(avg_s * 1e6, 2), "flops_per_launch": flops,
+            "proposals_per_launch": round(props_per_launch, 2)}
+
+
+def committed_traffic(profile, kernel):
+    """HBM bytes per launch of kernel from a committed PMC summary (JSON
+    beside the .md), with its source, or (None, None)."""
     try:
-        with open(os.path.join(ROOT, "profiles", "hbm_traffic_engine_b2_13500.json")) as f:
-            traffic = int(round(json.load(f)["kernels"][name]["traffic_bytes_per_proposal"] * props_per_launch))
+        with open(os.path.join(ROOT, "profiles", profile)) as f:
+            rec = json.load(f)["kernels"][kernel]
+        return int(rec["traffic_bytes_per_launch"]), "profiles/%s (%s)" % (profile, rec["how"])
     except (OSError, KeyError, ValueError):
-        pass
-    return {"bound": "mfma", "compute": "fp32 VALU (no matrix instructions; MI355X fp32 vector and dense "
-            "matrix peaks are both 157.3 TFLOP/s)", "achieved": round(tf, 4), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32_TFLOPS, 6), "traffic": traffic,
-            "traffic_source": "profiles/hbm_traffic_engine_b2_13500.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                              "copy-calibrated, per proposal x proposals per launch)" if traffic is not None else None,
-            "kernel": name, "avg_launch_us": round(avg_s * 1e6, 2), "flops_per_launch": flops}
+        return None, None
 
 
 def _replica_boxes(args):
@@ -412,61 +444,6 @@ def molecular_section(args):
     return out
 
 
-def sio2_section(args):
-    """200,000-atom SiO2 box: full histogram (float32 reference binning and
-    the binary64-recompute mode) and device moves with k <= 128, beside the
-    CPU oracle on bounded samples."""
-    import torch
-    t0 = time.perf_counter()
-    start, elem, pdc, sfc = build_sio2_workload()
-    n = len(elem)
-    print("bench: SiO2 workload built (%d atoms, %.1f s)" % (n, time.perf_counter() - t0), file=sys.stderr,
-          flush=True)
-    out = {"workload": "SiO2-like 200000 atoms (66667 Si), cube 145 A, min distance 1.5 A; PDC r[0,20]@0.01 "
-                       "(2000 bins) + SFC Q[0.3,30]x1000, single-atom translation 0.1 A, k<=128"}
-    gps, fms, _, sched = gpu_full_histogram(start, elem, 0.01, 2000)
-    gps0, fms0, _, sched0 = gpu_full_histogram(start, elem, 0.01, 2000, autotune=False)
-    gps64, fms64, re64, sched64 = gpu_full_histogram(start, elem, 0.01, 2000, float64=True)
-    tf = gps * OPS_PER_PAIR / 1e12
-    out["full_histogram"] = {"gpairs_per_s": round(gps / 1e9, 2), "kernel_ms": round(fms, 3),
-                             "pairs": n * (n - 1) // 2, "schedule": sched,
-                             "fixed_rule": {"gpairs_per_s": round(gps0 / 1e9, 2), "kernel_ms": round(fms0, 3),
-                                            "schedule": sched0},
-                             "roofline": {"bound": "mfma", "compute": "fp32 VALU", "achieved": round(tf, 3),
-                                          "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                                          "frac": round(tf / PEAK_FP32_TFLOPS, 4)},
-                             "float64_recompute_mode": {"gpairs_per_s": round(gps64 / 1e9, 2),
-                                                        "kernel_ms": round(fms64, 3), "rechecked_pairs": re64,
-                                                        "schedule": sched64}}
-    E = make_engine(start, pdc, sfc, seed=3, dr=0.01)
-    E.run_device(500, k=128, pull=False)
-    torch.cuda.synchronize()
-    acc0 = E.accepted
-    t1 = time.perf_counter()
-    st = E.run_device(args.sio2_moves, k=128, pull=False)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t1
-    out["moves_per_s"] = round(args.sio2_moves / el, 1)
-    out["moves"] = args.sio2_moves
-    out["accepted_fraction"] = round((E.accepted - acc0) / args.sio2_moves, 4)
-    out["windows"] = st["windows"]
-    out["proposals_evaluated"] = st["evaluated"]
-    print("bench: SiO2 moves done (%.1f moves/s)" % out["moves_per_s"], file=sys.stderr, flush=True)
-    if not args.no_cpu:
-        E.sync_from_device()
-        oe = cpu_engine(start, pdc, sfc, 0.01, init_from=E)
-        cps, cn, cel = cpu_moves_per_s(oe, args.cpu_seconds / 2)
-        threads = len(os.sched_getaffinity(0))
-        cgp, cpairs = cpu_full_histogram_sample(start, elem, 0.01, 2000, 200, threads)
-        out["cpu_baseline"] = {"value": round(cps, 2), "unit": "moves/s", "cores": 1, "kind": "port",
-                               "sample": "%d moves (oracle, 1 thread) in %.1f s" % (cn, cel),
-                               "full_histogram_gpairs_per_s": round(cgp / 1e9, 4), "full_histogram_cores": threads,
-                               "full_histogram_sample": "200 row atoms vs all later atoms (%.3g pairs)" % cpairs}
-        out["speedup_vs_cpu"] = round(out["moves_per_s"] / cps, 1)
-        out["full_histogram_speedup_vs_cpu_all_cores"] = round(gps / cgp, 1)
-    return out
-
-
 def sq_transform_section(reps=50):
     """G(r) -> S(Q) for k proposals at once: the exact float32 sequential
     transform (bit-identical to the constraint) and the matrix-core
@@ -529,26 +506,216 @@ def sq_transform_section(reps=50):
     return out
 
 
+def marker():
+    """A short torch kernel on the current stream between synchronisations:
+    in a rocprofv3 kernel trace its two launches bracket the timed region."""
+    import torch
+    torch.cuda.synchronize()
+    torch.full((7,), 3, dtype=torch.int64, device="cuda").cumsum_(0)
+    torch.cuda.synchronize()
+
+
+def full_histogram_record(start, elem, dr, H, with_f64):
+    """Full O(N^2) histogram: tuned and fixed-rule schedules (and the
+    binary64-recompute mode), Gpairs/s on N(N-1)/2 pairs."""
+    n = len(elem)
+    gps, fms, _, sched = gpu_full_histogram(start, elem, dr, H)
+    gps0, fms0, _, sched0 = gpu_full_histogram(start, elem, dr, H, autotune=False)
+    tf = gps * OPS_PER_PAIR / 1e12
+    out = {"gpairs_per_s": round(gps / 1e9, 2), "kernel_ms": round(fms, 3), "pairs": n * (n - 1) // 2,
+           "schedule": sched,
+           "fixed_rule": {"gpairs_per_s": round(gps0 / 1e9, 2), "kernel_ms": round(fms0, 3), "schedule": sched0},
+           "roofline": {"bound": "valu", "compute": "fp32 VALU, 35 ops per pair", "achieved": round(tf, 3),
+                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32_TFLOPS, 4),
+                        "attainable_peak": ATTAINABLE_FP32_TOPS,
+                        "frac_of_attainable": round(tf / ATTAINABLE_FP32_TOPS, 4)}}
+    if with_f64:
+        gps64, fms64, re64, sched64 = gpu_full_histogram(start, elem, dr, H, float64=True)
+        out["float64_recompute_mode"] = {"gpairs_per_s": round(gps64 / 1e9, 2), "kernel_ms": round(fms64, 3),
+                                         "rechecked_pairs": re64, "schedule": sched64}
+    return out
+
+
+def engine_timed_run(args, E, n, shape, traffic_profile, steps, moves, k, log_first=None):
+    """steps x moves generated moves through E (k speculative proposals per
+    window) between two synchronisations, with HIP events around the
+    kernels of every profile_every-th window; returns the counters, the
+    per-kernel event times and the k_eval_a roofline."""
+    import ctypes
+    import torch
+    from fullrmc_amd import _lib
+    L = _lib.lib()
+    h = E._device_engine(k)
+    acc0 = E.accepted
+    _lib.check(L.frmc_engine_profile(h, args.profile_every))
+    marker()
+    t0 = time.perf_counter()
+    evaluated = windows = 0
+    for i in range(steps):
+        st = E.run_device(moves, k=k, pull=False, log=log_first if i == 0 else None)
+        evaluated += st["evaluated"]
+        windows += st["windows"]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    marker()
+    nk = len(_lib.KERNEL_NAMES)
+    ms = (ctypes.c_double * nk)()
+    launches = (ctypes.c_int64 * nk)()
+    _lib.check(L.frmc_engine_kernel_times(h, ms, launches))
+    _lib.check(L.frmc_engine_profile(h, 0))
+    kernels = {_lib.KERNEL_NAMES[i]: {"ms_total": round(ms[i], 3), "launches": int(launches[i]),
+                                      "us_avg_with_events": round(1e3 * ms[i] / launches[i], 2)
+                                      if launches[i] else None}
+               for i in range(nk)}
+    kmean = evaluated / max(1, windows)
+    kwin = max(1, int(round(kmean)))
+    # k_eval_a launched back to back at the run's mean window size, between
+    # two HIP events on the engine stream: launch gaps and event markers
+    # amortised, the figure a kernel trace's average duration compares with
+    b2b = ctypes.c_float(0.0)
+    _lib.check(L.frmc_engine_time_eval(h, kwin, 200, ctypes.byref(b2b)))
+    avg_s = b2b.value * 1e-3  # ms per launch
+    traffic, tsrc = committed_traffic(traffic_profile, "k_eval_a")
+    roof = kernel_roofline("k_eval_a", avg_s, n, kwin, shape, traffic, tsrc)
+    if roof is not None:
+        roof["duration_source"] = ("HIP events around 200 back-to-back launches at the run's mean window size "
+                                   "(engine stream)")
+        roof["in_run_us_with_events"] = kernels["k_eval_a"]["us_avg_with_events"]
+    idel = _lib.KERNEL_NAMES.index("k_delta")
+    roof_delta = kernel_roofline("k_delta", ms[idel] * 1e-3 / max(1, launches[idel]), n, kmean, shape)
+    if roof_delta is not None:
+        roof_delta["duration_source"] = "HIP events around the launches of every %d-th window" % args.profile_every
+    return {"elapsed": elapsed, "moves": steps * moves, "accepted": E.accepted - acc0, "windows": windows,
+            "evaluated": evaluated, "kernels": kernels, "roofline": roof, "roofline_k_delta": roof_delta}
+
+
+def headline(args):
+    """The 200,000-atom SiO2 box: full histograms, then the timed moves."""
+    import torch
+    t0 = time.perf_counter()
+    start, elem, pdc, sfc = build_sio2_workload()
+    n = len(elem)
+    print("bench: SiO2 workload built (%d atoms, %.1f s)" % (n, time.perf_counter() - t0), file=sys.stderr,
+          flush=True)
+    fh = full_histogram_record(start, elem, 0.01, 2000, with_f64=True)
+    print("bench: full histograms done (%.1f Gpairs/s)" % fh["gpairs_per_s"], file=sys.stderr, flush=True)
+    E = make_engine(start, pdc, sfc, seed=3, dr=0.01)
+    for _ in range(args.warmup):
+        E.run_device(args.moves, k=args.k, pull=False)
+    E._device_engine(args.k)  # created here when there is no warm-up
+    snap = snapshot(E) if not args.no_cpu else None
+    dlog = [] if snap is not None else None
+    run = engine_timed_run(args, E, n, (2000, 2000, 1000, 3), "hbm_traffic_engine_sio2_200k.json", args.steps,
+                           args.moves, args.k, log_first=dlog)
+    value = run["moves"] / run["elapsed"]
+    print("bench: timed region done: %.1f moves/s" % value, file=sys.stderr, flush=True)
+    res = {"value": value, "elapsed": run["elapsed"], "n": n, "run": run, "full_histogram": fh}
+    # plain form: one proposal per window, same box and constraints
+    E1 = make_engine(start, pdc, sfc, seed=3, dr=0.01)
+    E1.run_device(20, k=1, pull=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    E1.run_device(args.plain_moves, k=1, pull=False)
+    torch.cuda.synchronize()
+    plain = args.plain_moves / (time.perf_counter() - t1)
+    E1._drop_device()
+    res["plain"] = plain
+    if snap is not None:
+        # the oracle replays the first timed step from the same state and
+        # random streams, for the CPU baseline and a parity check of the
+        # timed trajectory
+        oe = cpu_engine(start, pdc, sfc, 0.01, init_from=snap)
+        random.setstate(snap["py"])
+        np.random.set_state(snap["np"])
+        olog = []
+        tc = time.perf_counter()
+        while time.perf_counter() - tc < args.cpu_seconds and len(olog) + 5 <= len(dlog):
+            oe.run(5, olog)
+        cel = time.perf_counter() - tc
+        same = next((i for i, (a, b) in enumerate(zip(olog, dlog)) if a != b), len(olog))
+        if same != len(olog):
+            raise SystemExit("bench: the timed trajectory differs from the oracle at move %d: %r vs %r"
+                             % (same, dlog[same], olog[same]))
+        threads = len(os.sched_getaffinity(0))
+        cgp, cpairs = cpu_full_histogram_sample(start, elem, 0.01, 2000, 200, threads)
+        res["cpu"] = {"value": round(len(olog) / cel, 2), "unit": "moves/s", "cores": 1, "kind": "port",
+                      "sample": "the first %d moves of the timed region replayed by the oracle (C restatement of "
+                                "the reference loops + NumPy chain, 1 thread) from the same state and random "
+                                "streams, in %.1f s" % (len(olog), cel),
+                      "parity_checked_moves": len(olog),
+                      "full_histogram_gpairs_per_s": round(cgp / 1e9, 4), "full_histogram_cores": threads,
+                      "full_histogram_sample": "200 row atoms vs all later atoms (%.3g pairs), oracle C loop, "
+                                               "OpenMP with private bins" % cpairs}
+        res["cpu_full_hist_gps"] = cgp
+    E._drop_device()
+    return res
+
+
+def b2_section(args):
+    """The 13,500-atom B2 alloy: engine moves (k <= 64), plain form, the
+    reference's loop over the drop-in calls, full histogram, CPU oracle."""
+    import torch
+    start, elem, pdc, sfc = build_workload()
+    n = len(elem)
+    E = make_engine(start, pdc, sfc, seed=1)
+    E.run_device(args.b2_moves, k=64, pull=False)
+    run = engine_timed_run(args, E, n, (1000, 1000, 500, 3), "hbm_traffic_engine_b2_13500.json", 5,
+                           args.b2_moves, 64)
+    out = {"workload": "B2 alloy 13500 atoms (15x15x30 cells, a = 3.01 A), PDC r[0,20]@0.02 + SFC Q[0.5,25]x500, "
+                       "single-atom translation 0.1 A, k<=64",
+           "moves_per_s": round(run["moves"] / run["elapsed"], 1), "moves": run["moves"],
+           "accepted_fraction": round(run["accepted"] / run["moves"], 4), "windows": run["windows"],
+           "proposals_evaluated": run["evaluated"], "roofline_k_eval_a": run["roofline"],
+           "full_histogram": full_histogram_record(start, elem, 0.02, 1000, with_f64=False)}
+    E._drop_device()
+    E1 = make_engine(start, pdc, sfc, seed=1)
+    E1.run_device(50, k=1, pull=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    E1.run_device(1000, k=1, pull=False)
+    torch.cuda.synchronize()
+    plain = 1000 / (time.perf_counter() - t1)
+    E1._drop_device()
+    out["plain_one_move_per_window_moves_per_s"] = round(plain, 1)
+    # algorithmic bytes of one move's delta: 20 B per other atom (xyz, molecule, element)
+    out["plain_one_move_per_window_gb_per_s"] = round(20.0 * n * plain / 1e9, 3)
+    E2 = make_engine(start, pdc, sfc, seed=1)
+    E2.run(2)
+    t2 = time.perf_counter()
+    E2.run(30)
+    out["dropin_per_move_loop_moves_per_s"] = round(30 / (time.perf_counter() - t2), 1)
+    if not args.no_cpu:
+        oe = cpu_engine(start, pdc, sfc, 0.02)
+        cps, cn, cel = cpu_moves_per_s(oe, args.cpu_seconds / 2)
+        out["cpu_baseline"] = {"value": round(cps, 2), "unit": "moves/s", "cores": 1, "kind": "port",
+                               "sample": "%d moves of the same workload (oracle, 1 thread) in %.1f s" % (cn, cel)}
+        out["speedup_vs_cpu"] = round(out["moves_per_s"] / cps, 1)
+    print("bench: B2 done (%.1f moves/s)" % out["moves_per_s"], file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--moves", type=int, default=4000, help="generated moves per step")
-    ap.add_argument("--k", type=int, default=64, help="speculative proposals per window")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--moves", type=int, default=5000, help="generated moves per step")
+    ap.add_argument("--k", type=int, default=128, help="speculative proposals per window (at most)")
+    ap.add_argument("--plain-moves", type=int, default=1000, help="moves of the one-proposal-per-window form")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true")
-    # every 32nd window: about 100 timed launches per kernel; every 8th cost
-    # 4 % of the timed region's moves/s in event records
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline (no B2, transform, replica or molecular sections)")
+    # every 32nd window: about 100 timed launches per kernel per 3,000
+    # windows; every 8th cost 4 % of the timed region's moves/s
     ap.add_argument("--profile-every", type=int, default=32,
                     help="time the kernels of every n-th window with HIP events (0: off)")
-    ap.add_argument("--sio2", action=argparse.BooleanOptionalAction, default=True,
-                    help="also run the 200,000-atom SiO2 box (full histograms and k=128 moves)")
-    ap.add_argument("--sio2-moves", type=int, default=3000)
+    ap.add_argument("--b2-moves", type=int, default=4000, help="moves per timed step of the B2 section (0: skip)")
     ap.add_argument("--replicas", type=int, default=256, help="independent replicas run together (0: skip)")
     ap.add_argument("--replica-moves", type=int, default=3000, help="generated moves per replica")
-    ap.add_argument("--replica-streams", action=argparse.BooleanOptionalAction, default=True,
+    # off by default: under a rocprofv3 kernel trace this form (256 engine
+    # streams fed by 16 host threads) has crashed in the runtime's launch path
+    ap.add_argument("--replica-streams", action=argparse.BooleanOptionalAction, default=False,
                     help="also time the replicas with one stream per engine")
     ap.add_argument("--molecular-moves", type=int, default=1000,
                     help="moves of the 6,500-atom molecular triclinic box (0: skip)")
@@ -559,9 +726,10 @@ def main():
         print("bench.py measures one GPU: the pair path runs on a single MI355X; "
               "a sharded multi-GPU run is not implemented.", file=sys.stderr)
         sys.exit(2)
+    extras = not args.no_extras
 
     replica_boxes = None
-    if args.replicas > 0:
+    if extras and args.replicas > 0:
         t0 = time.perf_counter()
         replica_boxes = build_replica_boxes(args.replicas)  # before the GPU is initialised (fork pool)
         print("bench: %d replica boxes built (%.1f s)" % (len(replica_boxes), time.perf_counter() - t0),
@@ -571,7 +739,7 @@ def main():
     from fullrmc_amd import _lib
     from fullrmc_amd._device import require_gpu
     require_gpu()
-    L = _lib.lib()
+    _lib.lib()
 
     if args.copy_calibration:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.int32, device="cuda").fill_(1)
@@ -581,128 +749,40 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    start, elem, pdc, sfc = build_workload()
-    n = len(elem)
-    print("bench: workload built (%d atoms)" % n, file=sys.stderr, flush=True)
-    E = make_engine(start, pdc, sfc, seed=1)
-    for _ in range(args.warmup):
-        E.run_device(args.moves, k=args.k, pull=False)
-    h = E._device_engine(args.k)  # created here when there is no warm-up
-    acc0 = E.accepted
-    _lib.check(L.frmc_engine_profile(h, args.profile_every))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evaluated = windows = 0
-    for _ in range(args.steps):
-        st = E.run_device(args.moves, k=args.k, pull=False)
-        evaluated += st["evaluated"]
-        windows += st["windows"]
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    nk = len(_lib.KERNEL_NAMES)
-    ms = (ctypes.c_double * nk)()
-    launches = (ctypes.c_int64 * nk)()
-    _lib.check(L.frmc_engine_kernel_times(h, ms, launches))
-    _lib.check(L.frmc_engine_profile(h, 0))
-    moves = args.steps * args.moves
-    value = moves / elapsed
-    accepted = E.accepted - acc0
-    print("bench: timed region done: %.1f moves/s" % value, file=sys.stderr, flush=True)
-    kernels = {_lib.KERNEL_NAMES[i]: {"ms_total": round(ms[i], 3), "launches": int(launches[i]),
-                                      "us_avg": round(1e3 * ms[i] / launches[i], 2) if launches[i] else None}
-               for i in range(nk)}
-    # the roofline kernel: k_eval_a (G(r), S(Q) and chi-square of every proposal)
-    dom = _lib.KERNEL_NAMES.index("k_eval_a")
-    name = _lib.KERNEL_NAMES[dom]
-    avg_s = ms[dom] * 1e-3 / max(1, launches[dom])
-    roof = kernel_roofline(name, avg_s, n, evaluated / max(1, windows))
-    # the same kernel launched back to back (launch gaps amortised): the figure
-    # a rocprofv3 kernel trace's average duration compares with
-    kwin = max(1, int(round(evaluated / max(1, windows))))
-    b2b = ctypes.c_float(0.0)
-    _lib.check(L.frmc_engine_time_eval(h, kwin, 200, ctypes.byref(b2b)))
-    if roof is not None:
-        roof["back_to_back_us"] = round(1e3 * b2b.value, 2)
-        roof["back_to_back_k"] = kwin
-    # the batched delta kernel against the same FP32 peak (70 ops per
-    # proposal and other atom, old and new position)
-    idel = _lib.KERNEL_NAMES.index("k_delta")
-    roof_delta = kernel_roofline("k_delta", ms[idel] * 1e-3 / max(1, launches[idel]), n,
-                                 evaluated / max(1, windows))
-
+    H = headline(args)
+    run = H["run"]
+    moves = run["moves"]
     result = {"metric": "bit-exact-trajectory RMC moves/s + full-histogram Gpairs/s, GPU vs host CPU",
-              "value": round(value, 1), "unit": "moves/s", "n_gpus": 1, "steps": args.steps,
-              "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+              "value": round(H["value"], 1), "unit": "moves/s", "n_gpus": 1, "steps": args.steps,
+              "warmup": args.warmup, "ms_per_step": round(1e3 * H["elapsed"] / args.steps, 3),
               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
               "data": "synthetic",
-              "config": {"workload": "B2 alloy 13500 atoms, PDC r[0,20]@0.02 + SFC Q[0.5,25]x500, "
-                                     "single-atom translation 0.1 A, k=%d speculative windows" % args.k,
-                         "atoms": n, "moves_per_step": args.moves, "k": args.k},
-              "roofline": roof,
-              "roofline_k_delta": roof_delta,
-              "accepted_fraction": round(accepted / moves, 4),
-              "windows": windows, "proposals_evaluated": evaluated, "kernels": kernels}
-
-    if not args.no_extras:
-        # full-histogram recompute (one O(N^2) pass on the PDC grid)
-        gps, fms, _, sched = gpu_full_histogram(start, elem, 0.02, 1000)
-        gps0, fms0, _, sched0 = gpu_full_histogram(start, elem, 0.02, 1000, autotune=False)
-        tf = gps * OPS_PER_PAIR / 1e12
-        result["full_histogram"] = {"gpairs_per_s": round(gps / 1e9, 3), "kernel_ms": round(fms, 3),
-                                    "pairs": n * (n - 1) // 2, "schedule": sched,
-                                    "fixed_rule": {"gpairs_per_s": round(gps0 / 1e9, 3), "kernel_ms": round(fms0, 3),
-                                                   "schedule": sched0},
-                                    "roofline": {"bound": "mfma", "compute": "fp32 VALU", "achieved": round(tf, 3),
-                                                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                                                 "frac": round(tf / PEAK_FP32_TFLOPS, 4)}}
-        # plain form: one proposal per window
-        E1 = make_engine(start, pdc, sfc, seed=1)
-        E1.run_device(50, k=1, pull=False)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        E1.run_device(1000, k=1, pull=False)
-        torch.cuda.synchronize()
-        plain = 1000 / (time.perf_counter() - t1)
-        result["plain_one_move_per_window_moves_per_s"] = round(plain, 1)
-        # algorithmic bytes of one move's delta: 20 B per other atom (xyz,
-        # molecule, element)
-        result["plain_one_move_per_window_gb_per_s"] = round(20.0 * n * plain / 1e9, 3)
-        # the reference's loop structure calling the drop-in extensions per move
-        E2 = make_engine(start, pdc, sfc, seed=1)
-        E2.run(2)
-        t2 = time.perf_counter()
-        E2.run(30)
-        result["dropin_per_move_loop_moves_per_s"] = round(30 / (time.perf_counter() - t2), 1)
+              "config": {"workload": "SiO2-like 200000 atoms (66667 Si + 133333 O), cube 145 A, min distance "
+                                     "1.5 A; PDC r[0,20]@0.01 (2000 bins) + SFC Q[0.3,30]x1000 (own r grid "
+                                     "[0,20]@0.01), single-atom translation 0.1 A, k<=%d speculative windows; "
+                                     "%d warm-up moves (pre-equilibration), %d timed moves"
+                                     % (args.k, args.warmup * args.moves, moves),
+                         "atoms": H["n"], "moves_per_step": args.moves, "k": args.k},
+              "roofline": run["roofline"],
+              "roofline_k_delta": run["roofline_k_delta"],
+              "accepted_fraction": round(run["accepted"] / moves, 4),
+              "windows": run["windows"], "proposals_evaluated": run["evaluated"], "kernels": run["kernels"],
+              "full_histogram": H["full_histogram"],
+              "plain_one_move_per_window_moves_per_s": round(H["plain"], 1),
+              "plain_one_move_per_window_gb_per_s": round(20.0 * H["n"] * H["plain"] / 1e9, 3)}
+    if "cpu" in H:
+        result["cpu_baseline"] = H["cpu"]
+        result["speedup_vs_cpu"] = round(H["value"] / H["cpu"]["value"], 1)
+        result["full_histogram_speedup_vs_cpu_all_cores"] = round(
+            1e9 * H["full_histogram"]["gpairs_per_s"] / H["cpu_full_hist_gps"], 1)
+    if extras:
+        if args.b2_moves > 0:
+            result["b2_13500"] = b2_section(args)
         result["sq_transform"] = sq_transform_section()
-        print("bench: extras done", file=sys.stderr, flush=True)
-
-    if args.sio2:
-        result["sio2_200k"] = sio2_section(args)
-    if args.replicas > 0:
-        result["replicas"] = replicas_section(args, replica_boxes)
-    if args.molecular_moves > 0:
-        result["molecular_6500"] = molecular_section(args)
-
-    if not args.no_cpu:
-        oe = cpu_engine(start, pdc, sfc, 0.02)
-        cps, cn, cel = cpu_moves_per_s(oe, args.cpu_seconds)
-        threads = len(os.sched_getaffinity(0))
-        # the same oracle loop with its per-move histograms on every core
-        oe.s.ncores = threads
-        aps, an, ael = cpu_moves_per_s(oe, args.cpu_seconds / 3)
-        cgp, cpairs = cpu_full_histogram_sample(start, elem, 0.02, 1000, 400, threads)
-        result["cpu_baseline"] = {"value": round(cps, 2), "unit": "moves/s", "cores": 1, "kind": "port",
-                                  "sample": "%d moves of the same workload (oracle: C restatement of the reference "
-                                            "loops + NumPy chain, 1 thread) in %.1f s" % (cn, cel),
-                                  "all_cores_value": round(aps, 2), "all_cores": threads,
-                                  "all_cores_sample": "%d moves, per-move histogram loops on %d threads (OpenMP, "
-                                                      "private bins) in %.1f s" % (an, threads, ael),
-                                  "full_histogram_gpairs_per_s": round(cgp / 1e9, 4),
-                                  "full_histogram_cores": threads,
-                                  "full_histogram_sample": "%d row atoms vs all later atoms (%.3g pairs), oracle C "
-                                                           "loop, OpenMP with private bins" % (400, cpairs)}
-        result["speedup_vs_cpu"] = round(value / cps, 1)
-        result["speedup_vs_cpu_all_cores"] = round(value / aps, 1)
+        if args.replicas > 0:
+            result["replicas"] = replicas_section(args, replica_boxes)
+        if args.molecular_moves > 0:
+            result["molecular_6500"] = molecular_section(args)
     print(json.dumps(result))
 
 

@@ -138,11 +138,24 @@ class Engine:
         else:
             gen = generator if generator is not None else TranslationGenerator(amplitude=0.2)
             self.generators = [gen] * len(self.groups)
+        self._check_generators()
         self._drop_device()
 
     def set_move_generator(self, generator):
         """Use one move generator for every group."""
         self.generators = [generator] * len(self.groups)
+        self._check_generators()
+
+    def _check_generators(self):
+        """Each group's generator must accept the group: the reference's
+        MoveGenerator.set_group raises Exception with the check's message
+        (Core/MoveGenerator.py:56-61), e.g. for a rotation on a one-atom group
+        (Generators/Rotations.py:133-143)."""
+        for grp, gen in zip(self.groups, self.generators):
+            if hasattr(gen, "check_group"):
+                ok, msg = gen.check_group(grp)
+                if not ok:
+                    raise Exception(msg)
 
     def _distinct_generators(self):
         """The distinct move generators of the groups (one C-level pass when

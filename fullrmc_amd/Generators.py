@@ -92,9 +92,21 @@ class RotationGenerator:
             raise ValueError("rotation amplitude must be in (0, 360) degrees")
         self.amplitude = FLOAT_TYPE(PI * a / 180.)
 
+    def check_group(self, group):
+        """Generators/Rotations.py:133-143: a rotation needs two atoms."""
+        if len(group) <= 1:
+            return False, "At least two atoms needed in a group to perform rotation."
+        return True, ""
+
     def move(self, coordinates):
         if coordinates.shape[0] <= 1:
-            raise ValueError("rotation needs at least two atoms in the group")
+            # The reference's fallback for a group reduced to one atom
+            # (Generators/Rotations.py:155-157) passes self.__amplitude[0] to
+            # generate_random_vector, but the amplitude is a float32 scalar:
+            # the indexing raises IndexError before any random draw.  The
+            # same error, with no draw, here.
+            raise IndexError("invalid index to scalar variable: the reference's one-atom rotation fallback "
+                             "indexes its scalar amplitude (a rotation needs at least two atoms)")
         n = 0
         while n < PRECISION:
             axis = 1 - 2 * np.random.random(3)
@@ -118,6 +130,10 @@ class MoveGeneratorCombinator:
         if shuffle:
             raise NotImplementedError("shuffled combinations consume random.shuffle draws; not supported")
         self.combination = list(combination)
+
+    def check_group(self, group):
+        """Core/MoveGenerator.py:741-752: a combination accepts every group."""
+        return True, ""
 
     def move(self, coordinates):
         for g in self.combination:

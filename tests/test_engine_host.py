@@ -58,3 +58,31 @@ def test_distinct_generators():
     # equal-looking but distinct generators are distinct
     t2 = TranslationGenerator(0.1)
     assert len(_Stub([t, t2])._distinct_generators()) == 2
+
+
+def test_rotation_of_one_atom_raises_before_any_draw():
+    """A group reduced to one atom: the reference's fallback indexes its
+    float32 amplitude and raises IndexError before drawing
+    (Generators/Rotations.py:155-157); assigning a rotation to a one-atom
+    group is refused with the reference's message."""
+    r = RotationGenerator(10)
+    random.seed(1)
+    np.random.seed(1)
+    py, npst = random.getstate(), np.random.get_state()
+    try:
+        r.move(np.zeros((1, 3), np.float32))
+    except IndexError:
+        pass
+    else:
+        raise AssertionError("IndexError expected")
+    assert random.getstate() == py and np.array_equal(np.random.get_state()[1], npst[1])
+    E = eng.Engine()
+    E.set_system(np.zeros((3, 3), np.float32) + np.arange(3, dtype=np.float32)[:, None], ["Ar"] * 3,
+                 basisVectors=np.eye(3, dtype=np.float32) * 10)
+    try:
+        E.set_groups([[0], [1, 2]], RotationGenerator(10))
+    except Exception as e:
+        assert "At least two atoms" in str(e)
+    else:
+        raise AssertionError("a rotation on a one-atom group must be refused")
+    E.set_groups([[0, 1], [1, 2]], RotationGenerator(10))
