@@ -109,6 +109,7 @@ _PROTOS = {
     "frmc_pair_elements_stats": [c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp],
     # G(r) -> S(Q) transform, reciprocal_space
     "frmc_sq_transform": [c_vp, c_i32, c_i32, c_vp, c_i32, c_int, c_int, c_vp, c_vp],
+    "frmc_sq_transform_noalloc": [c_vp, c_i32, c_i32, c_vp, c_i32, c_int, c_int, c_vp, c_vp],
     "frmc_Gr_to_sq": [c_vp, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp],
     "frmc_gr_to_sq": [c_vp, c_vp, c_i32, c_vp, c_i32, c_f32, c_vp, c_vp],
     "frmc_sq_to_Gr": [c_vp, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp],

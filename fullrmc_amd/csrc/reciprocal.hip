@@ -22,6 +22,7 @@
 //   k_sq_to_Gr   the transform of reciprocal_space.pyx:131-162 (which
 //                raises for any input there), in double, rounded once.
 #include <algorithm>
+#include <mutex>
 
 #include "frmc_device.h"
 #include "frmc_host.h"
@@ -361,6 +362,87 @@ extern "C" int frmc_sq_transform(const float *Gr, int32_t k, int32_t H, const fl
     if (le != hipSuccess) {
         set_error("sq_transform failed: %s", hipGetErrorString(le));
         return FRMC_EHIP;
+    }
+    return FRMC_OK;
+}
+
+// ------------------------------------------------- no-allocation variant
+//
+// The matrix-core transform's split partials in a file-scope device array
+// instead of stream-ordered pool memory, for callers that cannot allocate
+// (code under HIP graph capture, allocation-free serving loops).  Each GPU
+// the library runs on holds its own copy (16 MiB) until the process ends.
+// Every call on a GPU shares that GPU's copy, and the next call overwrites
+// it: calls that overlap on one GPU (other streams, other threads) race.
+// k_sq_mfma stores every partial element it owns, so nothing needs clearing.
+constexpr int64_t kSqScratchFloats = (int64_t)1 << 22;
+static __device__ __align__(256) float scratch[kSqScratchFloats];
+
+static std::mutex g_sq_scratch_mu;
+static float *g_sq_scratch[64] = {nullptr};  // device address of scratch, per device
+
+// This device's copy of scratch: its address is taken once per device
+// (with that device current), then reused; host code never uses the
+// symbol itself, which is only a host-side stand-in.
+static int sq_scratch_of_current_device(float **out) {
+    int dev = 0;
+    FRMC_CHECK_HIP(hipGetDevice(&dev));
+    FRMC_REQUIRE(dev >= 0 && dev < 64, "sq_transform_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_sq_scratch_mu);
+    if (!g_sq_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("sq_transform_noalloc: hipGetSymbolAddress of the scratch array failed: %s",
+                      hipGetErrorString(e));
+            return FRMC_EHIP;
+        }
+        g_sq_scratch[dev] = (float *)p;
+    }
+    *out = g_sq_scratch[dev];
+    return FRMC_OK;
+}
+
+extern "C" int frmc_sq_transform_noalloc(const float *Gr, int32_t k, int32_t H, const float *sqMatrix, int32_t nQ,
+                                         int addOne, int mode, float *Sq, void *stream) {
+    FRMC_REQUIRE(k >= 0 && H >= 1 && nQ >= 0, "sq_transform_noalloc: bad sizes k=%d H=%d nQ=%d", k, H, nQ);
+    FRMC_REQUIRE(mode == FRMC_SQ_EXACT || mode == FRMC_SQ_MFMA, "sq_transform_noalloc: unknown mode %d", mode);
+    if (k == 0 || nQ == 0) return FRMC_OK;
+    FRMC_REQUIRE(Gr && sqMatrix && Sq, "sq_transform_noalloc: NULL array");
+    FRMC_REQUIRE((int64_t)H * nQ < ((int64_t)1 << 40), "sq_transform_noalloc: matrix too large");
+    hipStream_t st = as_stream(stream);
+    if (mode == FRMC_SQ_EXACT) {
+        // no scratch: the exact kernel of frmc_sq_transform
+        FRMC_REQUIRE(k <= 65535, "sq_transform_noalloc: at most 65535 curves per call");
+        hipLaunchKernelGGL(k_sq_exact, dim3(blocks_for(nQ, kSqT), k), dim3(kSqT), 0, st, Gr, k, H, sqMatrix, nQ,
+                           (int32_t)(addOne != 0), Sq);
+        FRMC_CHECK_LAUNCH();
+        return FRMC_OK;
+    }
+    // the same r split as frmc_sq_transform, so the results are identical
+    const int64_t bq = (nQ + kTQ - 1) / kTQ, bp = (k + kTP - 1) / kTP;
+    FRMC_REQUIRE(bp <= 65535, "sq_transform_noalloc: too many curves per call");
+    const int64_t chunks = (H + kKC - 1) / kKC;
+    int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>((1024 + bq * bp - 1) / (bq * bp), chunks));
+    const int32_t per = (int32_t)(((chunks + nsplit - 1) / nsplit) * kKC);
+    nsplit = (H + per - 1) / per;
+    float *part = nullptr;
+    if (nsplit > 1) {
+        const int64_t need = nsplit * (int64_t)k * nQ;
+        FRMC_REQUIRE(need <= kSqScratchFloats,
+                     "sq_transform_noalloc: k=%d, H=%d, nQ=%d need %lld floats of split partials; the "
+                     "no-allocation entry holds at most %lld (16 MiB): use frmc_sq_transform",
+                     k, H, nQ, (long long)need, (long long)kSqScratchFloats);
+        if (int rc = sq_scratch_of_current_device(&part)) return rc;
+    }
+    hipLaunchKernelGGL(k_sq_mfma, dim3(bq, bp, nsplit), dim3(256), 0, st, Gr, k, H, sqMatrix, nQ, per,
+                       (int32_t)(addOne != 0), nsplit > 1 ? part : Sq, (int32_t)(nsplit == 1));
+    FRMC_CHECK_LAUNCH();
+    if (nsplit > 1) {
+        const int64_t len = (int64_t)k * nQ;
+        hipLaunchKernelGGL(k_sq_splitsum, dim3((unsigned)std::min<int64_t>(blocks_for(len, 256), 4096)), dim3(256),
+                           0, st, part, (int32_t)nsplit, len, (int32_t)(addOne != 0), Sq);
+        FRMC_CHECK_LAUNCH();
     }
     return FRMC_OK;
 }

@@ -283,6 +283,17 @@ int frmc_pair_elements_stats(const int32_t *elementIndex, const int32_t *molecul
 int frmc_sq_transform(const float *Gr, int32_t k, int32_t H, const float *sqMatrix, int32_t nQ, int addOne,
                       int mode, float *Sq, void *stream);
 
+/* frmc_sq_transform without any allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, allocation-free serving loops).  Same arguments
+   and results (bit-identical in both modes); the matrix-core mode keeps its
+   split partials in a library-owned device array of at most 2^22 floats
+   (16 MiB) per GPU, held until the process ends, and returns FRMC_EINVAL
+   when a shape needs more.  All calls on one GPU share that array: calls
+   that overlap on one GPU race. */
+int frmc_sq_transform_noalloc(const float *Gr, int32_t k, int32_t H, const float *sqMatrix, int32_t nQ,
+                              int addOne, int mode, float *Sq, void *stream);
+
 /* Gr_to_sq (reciprocal_space.pyx:89-122): sq (m,) = 1 + sum_r
    dr*sin(q*r)/q*Gr[r], dr = distances[1]-distances[0], each term and each
    accumulation in double, rounded to float32 after each accumulation as
