@@ -575,12 +575,27 @@ def engine_timed_run(args, E, n, shape, traffic_profile, steps, moves, k, log_fi
     b2b = ctypes.c_float(0.0)
     _lib.check(L.frmc_engine_time_eval(h, kwin, 200, ctypes.byref(b2b)))
     avg_s = b2b.value * 1e-3  # ms per launch
+    # the tuned k_eval_a shape per window-size bucket, and every shape's time
+    # at the mean window size
+    shapes = (ctypes.c_int32 * 8)()
+    _lib.check(L.frmc_engine_eval_shapes(h, shapes, 8))
+    per_shape = {}
+    try:
+        for sh in (0, 1, 2, 4, 5, 6):
+            _lib.check(L.frmc_set_eval_shape(sh))
+            t = ctypes.c_float(0.0)
+            _lib.check(L.frmc_engine_time_eval(h, kwin, 50, ctypes.byref(t)))
+            per_shape[str(sh)] = round(1e3 * t.value, 2)
+    finally:
+        _lib.check(L.frmc_set_eval_shape(-1))
     traffic, tsrc = committed_traffic(traffic_profile, "k_eval_a")
     roof = kernel_roofline("k_eval_a", avg_s, n, kwin, shape, traffic, tsrc)
     if roof is not None:
         roof["duration_source"] = ("HIP events around 200 back-to-back launches at the run's mean window size "
                                    "(engine stream)")
         roof["in_run_us_with_events"] = kernels["k_eval_a"]["us_avg_with_events"]
+        roof["tuned_shape_per_bucket"] = list(shapes)
+        roof["us_per_shape_at_mean_k"] = per_shape
     idel = _lib.KERNEL_NAMES.index("k_delta")
     roof_delta = kernel_roofline("k_delta", ms[idel] * 1e-3 / max(1, launches[idel]), n, kmean, shape)
     if roof_delta is not None:

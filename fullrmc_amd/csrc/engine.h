@@ -30,7 +30,8 @@ constexpr int kMaxC = 4;   // constraints per engine
 constexpr int kMaxG = 64;  // atoms per group
 constexpr int kQC = 64;    // Q points per k_eval_a wave (one per lane)
 constexpr int kEW = 1;     // S(Q) waves per k_eval_a block: a work unit is kEW*kQC Q points
-constexpr int kEvalThreads = 256;  // k_eval_a block: all threads build G(r), wave 0 runs the S(Q) sum
+constexpr int kEvalThreads = 256;  // k_eval_a block: all threads build G(r), one wave per proposal runs its S(Q) sum
+constexpr int kEP = 4;             // most proposals per k_eval_a block (one S(Q) wave each)
 
 struct DevC {
     int32_t kind;   // FRMC_PDC, FRMC_SFC, FRMC_PCF or FRMC_RSFC
@@ -165,7 +166,7 @@ struct BatchArgs {
 
 // launchers (engine_kernels.hip)
 int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hipStream_t st);
-int launch_eval(const View &V, int32_t k, hipStream_t st);
+int launch_eval(const View &V, int32_t k, hipStream_t st, int32_t shape);
 int launch_eval_b(const View &V, int32_t k, hipStream_t st);
 int launch_commit(const View &V, const Inline &I, const CommitCtl &C, hipStream_t st);
 int launch_symmetrise(int32_t *base, const int32_t *slot, int32_t nel, int32_t H, hipStream_t st);
@@ -175,7 +176,7 @@ int launch_full(const float *coords, int32_t n, const Basis &B, int isPBC, bool 
 int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t shm_d, size_t shm_a, size_t shm_b,
                  hipStream_t st);
 void pairwise_plan(int32_t n, std::vector<int32_t> &leaves, std::vector<int32_t> &prog);
-size_t eval_lds_bytes(const View &V);
+size_t eval_lds_bytes(const View &V, int32_t np = 1);  // np G(r) arrays
 size_t eval_b_lds_bytes(const View &V);
 
 }  // namespace frmc
@@ -220,6 +221,8 @@ struct frmc_engine {
     float recip[9] = {0};
     bool have_real = false;
     bool adaptive = true;  // window size follows the acceptance rate
+    bool eval_tuned = false;     // eval_shape holds the tuned k_eval_a shapes
+    int8_t eval_shape[16] = {0};  // k_eval_a shape per window-size bucket ceil(log2 k)
     std::vector<void *> allocs, host_allocs;
     // optional per-kernel timing with HIP events on the engine stream
     bool profile = false;
@@ -236,6 +239,7 @@ struct frmc_engine {
 
 namespace frmc {
 View make_view(frmc_engine *e, int stage_idx);
+int32_t eval_shape(frmc_engine *e, int32_t k);
 int launch_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
                           const float *u, float tol);
 int finish_device_decided(frmc_engine *e, int32_t k, float *tse_out);

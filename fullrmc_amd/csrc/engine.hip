@@ -16,6 +16,9 @@
 #include <time.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "engine.h"
@@ -49,7 +52,7 @@ View make_view(frmc_engine *e, int s) {
     V.dn = e->dnbuf[e->cur_dn];
     V.dn_zero = e->dnbuf[1 - e->cur_dn];
     V.zero_len = (int64_t)e->dn_used[1 - e->cur_dn] * e->dnStride;
-    V.cnt = nullptr;
+    V.cnt = e->cnt;
     V.sqbuf = e->sqbuf;
     V.chi2d = e->chi2d;
     V.out_res = e->h_res;
@@ -205,6 +208,97 @@ static int check_device_decision(frmc_engine *e) {
     return FRMC_OK;
 }
 
+// ------------------------------------------------ k_eval_a shape tuning
+//
+// A shape of k_eval_a is (bins per thread per G(r) pass, proposals per
+// block); every shape gives the same results.  The best shape depends on the
+// window size k: few proposals want one chain per block spread over the
+// chip, many want blocks whose waves share the matrix columns.  For each
+// window-size bucket (ceil(log2 k), k up to kmax) every shape is timed on the
+// engine stream, and the fastest is kept per shape class (device, work
+// units, bins, pair types), so engines of one class tune once.
+static const int32_t kEvalShapes[6] = {0, 1, 2, 4, 5, 6};
+static std::mutex g_eval_mu;
+static std::map<std::tuple<int, int32_t, int32_t, int32_t, int32_t>, int32_t> g_eval_tuned;
+static int g_eval_forced = -1;  // frmc_set_eval_shape
+
+static int32_t k_bucket(int32_t k) {
+    int32_t b = 0;
+    while (b < 15 && (1 << b) < k) b++;
+    return b;
+}
+
+static int tune_eval(frmc_engine *e) {
+    const View V = make_view(e, e->cur_stage);
+    const int32_t nb = k_bucket(e->kmax) + 1;
+    const int32_t ns = V.Hmax > 4 * kEvalThreads ? 6 : 3;  // U = 8 differs from 4 only above 1,024 bins
+    constexpr int reps = 6;
+    std::vector<hipEvent_t> ev(2 * nb * ns, nullptr);
+    int rc = FRMC_OK;
+    for (auto &x : ev)
+        if (rc == FRMC_OK && hipEventCreate(&x) != hipSuccess) rc = FRMC_EHIP;
+    for (int32_t b = 0; b < nb && rc == FRMC_OK; b++) {
+        const int32_t k = std::min<int32_t>(1 << b, e->kmax);
+        for (int32_t i = 0; i < ns && rc == FRMC_OK; i++) {
+            rc = launch_eval(V, k, e->st, kEvalShapes[i]);  // warm
+            if (rc == FRMC_OK && hipEventRecord(ev[2 * (b * ns + i)], e->st) != hipSuccess) rc = FRMC_EHIP;
+            for (int r = 0; r < reps && rc == FRMC_OK; r++) rc = launch_eval(V, k, e->st, kEvalShapes[i]);
+            if (rc == FRMC_OK && hipEventRecord(ev[2 * (b * ns + i) + 1], e->st) != hipSuccess) rc = FRMC_EHIP;
+        }
+    }
+    if (rc == FRMC_OK && hipStreamSynchronize(e->st) != hipSuccess) rc = FRMC_EHIP;
+    for (int32_t b = 0; b < nb && rc == FRMC_OK; b++) {
+        float best = 0.f;
+        int32_t pick = 0;
+        for (int32_t i = 0; i < ns; i++) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * (b * ns + i)], ev[2 * (b * ns + i) + 1]) != hipSuccess) {
+                rc = FRMC_EHIP;
+                break;
+            }
+            if (i == 0 || ms < best) {
+                best = ms;
+                pick = kEvalShapes[i];
+            }
+        }
+        e->eval_shape[b] = pick;
+    }
+    for (auto x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (rc) set_error("timing the k_eval_a shapes failed");
+    return rc;
+}
+
+// the k_eval_a shape for a window of k proposals (tuned on first use)
+int32_t eval_shape(frmc_engine *e, int32_t k) {
+    if (g_eval_forced >= 0) return g_eval_forced;
+    if (!e->eval_tuned) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        const auto key = std::make_tuple(dev, e->nunits, e->Hmax, e->T, e->kmax);
+        bool have = false;
+        {
+            std::lock_guard<std::mutex> lk(g_eval_mu);
+            auto it = g_eval_tuned.find(key);
+            if (it != g_eval_tuned.end()) {
+                for (int b = 0; b < 16; b++) e->eval_shape[b] = (int8_t)((it->second >> (3 * (b % 10))) & 7);
+                have = true;
+            }
+        }
+        if (!have) {
+            for (int b = 0; b < 16; b++) e->eval_shape[b] = 0;
+            if (tune_eval(e) == FRMC_OK) {
+                int32_t packed = 0;
+                for (int b = 0; b < 10; b++) packed |= (int32_t)e->eval_shape[b] << (3 * b);
+                std::lock_guard<std::mutex> lk(g_eval_mu);
+                g_eval_tuned[key] = packed;
+            }
+        }
+        e->eval_tuned = true;
+    }
+    return e->eval_shape[std::min<int32_t>(k_bucket(k), 9)];
+}
+
 // launch delta + eval for the k proposals in staging buffer s (and, with C,
 // the commit of the proposal the device accepts); finish_window waits for
 // the totals
@@ -221,7 +315,7 @@ static int launch_window(frmc_engine *e, int s, int32_t k, const Inline *I, cons
     }
     {
         KTimer t(e, FRMC_K_EVAL);
-        rc = launch_eval(V, k, e->st);
+        rc = launch_eval(V, k, e->st, eval_shape(e, k));
     }
     if (rc) return rc;
     {
@@ -653,7 +747,13 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         e->nunits = units;
         {
             View V = make_view(e, 0);
-            if (eval_lds_bytes(V) > 150 * 1024) EFAIL(FRMC_EINVAL, "model arrays need more than 150 KB of LDS");
+            // the evaluation kernels keep a constraint's model array in LDS
+            // (150 KB leaves room for the batched kernels' staged records)
+            if (eval_lds_bytes(V, kEP) > 150 * 1024)
+                EFAIL(FRMC_EINVAL, "model arrays need more than 150 KB of LDS");
+            if (eval_b_lds_bytes(V) > 150 * 1024)
+                EFAIL(FRMC_EINVAL, "an S(Q) constraint with %d Q points needs %zu bytes of LDS in the chi-square "
+                      "kernel (at most 150 KB)", V.Mmax, eval_b_lds_bytes(V));
         }
         const size_t K = (size_t)e->kmax;
         for (int b = 0; b < 2; b++) {
@@ -678,6 +778,9 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         }
         ECHK(dalloc(e, &e->sqbuf, K * e->C * e->Mmax));
         ECHK(dalloc(e, &e->chi2d, K * e->C));
+        // blocks of k_eval finished per proposal (the last one resets it)
+        ECHK(dalloc(e, &e->cnt, K));
+        ECHK(hipMemset(e->cnt, 0, sizeof(int32_t) * K));
         ECHK(halloc(e, &e->h_res, K * (e->C + 1)));
         for (size_t q = 0; q < K * (size_t)(e->C + 1); q++) e->h_res[q] = 0;
         e->h_tse.assign(K, 0.0f);
@@ -860,7 +963,8 @@ int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out
     hipError_t err = hipEventCreate(&b);
     int rc = FRMC_OK;
     if (err == hipSuccess) err = hipEventRecord(a, e->st);
-    for (int32_t r = 0; r < reps && err == hipSuccess && rc == FRMC_OK; r++) rc = launch_eval(V, k, e->st);
+    const int32_t shape = eval_shape(e, k);
+    for (int32_t r = 0; r < reps && err == hipSuccess && rc == FRMC_OK; r++) rc = launch_eval(V, k, e->st, shape);
     if (err == hipSuccess && rc == FRMC_OK) err = hipEventRecord(b, e->st);
     if (err == hipSuccess && rc == FRMC_OK) err = hipEventSynchronize(b);
     float ms = 0.f;
@@ -873,6 +977,21 @@ int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out
         return FRMC_EHIP;
     }
     *ms_out = ms / reps;
+    return FRMC_OK;
+}
+
+int frmc_set_eval_shape(int32_t shape) {
+    FRMC_REQUIRE(shape == -1 || shape == 0 || shape == 1 || shape == 2 || shape == 4 || shape == 5 || shape == 6,
+                 "eval shape must be -1 (tuned), 0, 1, 2, 4, 5 or 6 (got %d)", shape);
+    g_eval_forced = shape;
+    return FRMC_OK;
+}
+
+int frmc_engine_eval_shapes(frmc_engine *e, int32_t *shapes_out, int32_t n) {
+    FRMC_REQUIRE(e && shapes_out && n >= 1 && n <= 16, "bad arguments");
+    (void)eval_shape(e, 1);  // tunes on first use
+    for (int32_t b = 0; b < n; b++)
+        shapes_out[b] = g_eval_forced >= 0 ? g_eval_forced : e->eval_shape[std::min<int32_t>(b, 9)];
     return FRMC_OK;
 }
 

@@ -347,14 +347,20 @@ struct Fuse {
     int32_t nb;  // blocks of the window
 };
 
-// work unit u of proposal p; dnw is the window's delta buffer, sm the
-// block's dynamic LDS (eval_lds_bytes); F: also k_eval_b's work (one r-space
-// constraint)
-__device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, int32_t p, int32_t u, float *sm,
-                                            const Fuse *F) {
+// Work unit u for NP proposals at once (np <= NP of them used, from p0):
+// every thread builds each proposal's G(r) into LDS in turn, then the r-space
+// constraint's chi-square is taken per proposal, or wave w runs the S(Q) sum
+// of proposal p0 + w over the unit's Q columns (the waves stream the same
+// matrix columns, so one wave's loads are L1 hits for the others).  dnw is
+// the window's delta buffer, sm the block's dynamic LDS (eval_lds_bytes
+// with NP G(r) arrays); F (NP = 1 only): also k_eval_b's work for one
+// r-space constraint.
+template <int U, int NP>
+__device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, int32_t p0, int32_t np, int32_t u,
+                                            float *sm, const Fuse *F) {
     const int32_t Hp = (V.Hmax + 3) & ~3;        // keeps float4 reads aligned
-    float *Gs = sm;                              // [Hp] G(r) or PDC squared deviations
-    float *scratch = Gs + Hp;                    // [maxLeaves + kStack]
+    float *Gs = sm;                              // [NP][Hp] G(r) or PDC squared deviations
+    float *scratch = Gs + NP * Hp;               // [maxLeaves + kStack]
     int32_t *lprog = (int32_t *)(scratch + V.maxLeaves + kStack);  // [2*maxLeaves]
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
     int32_t ci = 0;
@@ -363,11 +369,12 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
     const DevC c = V.c[ci];
     const int32_t H = c.H, M = c.M, T = V.T;
     const bool rsp = c.rspace;
-    const int32_t *dn = dnw + (int64_t)p * V.dnStride + c.dnOff;
+    const int32_t *dn0 = dnw + (int64_t)p0 * V.dnStride + c.dnOff;
     const int32_t *base = c.base;
     // S(Q) for q = qc*kQC + lane (SFC units): the first rows of the matrix
     // column and the experimental value load now, under the G(r) phase
-    static_assert(kEW == 1, "one S(Q) wave per block");
+    static_assert(kEW == 1, "one S(Q) wave per proposal");
+    static_assert(NP <= kEvalThreads / 64, "one wave per proposal");
     const int32_t q = (u - c.uOff) * kQC + lane;
     const bool qok = !rsp && q < M;
     const float *mcol = c.sqm + (int64_t)min(q, M - 1) * c.hpb;  // M[r][q] at mcol[r], 16-byte aligned
@@ -383,7 +390,7 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
         }
     };
     float ex_q = 0.0f, dw_q = 1.0f;
-    if (!rsp && w == 0) {
+    if (!rsp && w < np) {
         load_m(ma, 0);
         ex_q = c.expv[min(q, M - 1)];
         if (c.hasW) dw_q = c.dataW[min(q, M - 1)];
@@ -393,23 +400,22 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
         // valid pointer): a load under a divergent branch is followed by its
         // own wait, which serialises the batch
         constexpr int NT = kEvalThreads;
-        constexpr int U = (1024 + NT - 1) / NT;  // bins per thread per pass: 1024 per pass
+        constexpr int TMAX = 3;  // pair types (two elements) whose counts load in one round
         const float *expv = rsp ? c.expv : c.shellV;                 // S(Q): unused (expv has M entries)
         const float *dataW = (rsp && c.hasW) ? c.dataW : c.shellV;   // unused unless hasW
         for (int32_t r0 = 0; r0 < H; r0 += NT * U) {
             int32_t rr[U];
 #pragma unroll
             for (int i = 0; i < U; i++) rr[i] = min(r0 + tid + NT * i, H - 1);
-            float Gr[U];
-            int32_t bv[U], dv[U];
             float sv[U], pf[U], ex[U], dw[U];
+            // the pair-type sums and per-bin constants, shared by the proposals
+            int32_t bv[TMAX][U];
 #pragma unroll
-            for (int i = 0; i < U; i++) {
-                Gr[i] = 0.0f;
-                bv[i] = base[rr[i]];
-                dv[i] = dn[rr[i]];
+            for (int t = 0; t < TMAX; t++) {
+                const int32_t tc = min(t, T - 1);
+#pragma unroll
+                for (int i = 0; i < U; i++) bv[t][i] = base[tc * H + rr[i]];
             }
-            // the per-bin constants load with the first counts
 #pragma unroll
             for (int i = 0; i < U; i++) {
                 sv[i] = c.shellV[rr[i]];
@@ -417,71 +423,114 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
                 ex[i] = expv[rr[i]];
                 dw[i] = dataW[rr[i]];
             }
-            // pair types in order; the next type's counts load while this one is summed
-            for (int32_t t = 0; t < T; t++) {
-                const int32_t tn = min(t + 1, T - 1);
-                int32_t nb[U], nd[U];
+            for (int32_t j = 0; j < np; j++) {
+                const int32_t *dn = dn0 + (int64_t)j * V.dnStride;
+                float Gr[U];
 #pragma unroll
-                for (int i = 0; i < U; i++) {
-                    nb[i] = base[tn * H + rr[i]];
-                    nd[i] = dn[tn * H + rr[i]];
-                }
-                const float wt = c.w[t], Dt = c.D[t];
+                for (int i = 0; i < U; i++) Gr[i] = 0.0f;
+                if (T <= TMAX) {
+                    // every type's deltas in one round of loads (types past T
+                    // reload the last one and are not summed)
+                    int32_t dv[TMAX][U];
 #pragma unroll
-                for (int i = 0; i < U; i++) {
-                    // Gr += wij*nij/Dij with nij integer-valued (exact in float32)
-                    const float nij = (float)(bv[i] + dv[i]);
-                    float term = wt * nij;
-                    term = term / Dt;
-                    Gr[i] = Gr[i] + term;
-                    bv[i] = nb[i];
-                    dv[i] = nd[i];
-                }
-            }
+                    for (int t = 0; t < TMAX; t++) {
+                        const int32_t tc = min(t, T - 1);
 #pragma unroll
-            for (int i = 0; i < U; i++) {
-                float g = Gr[i] / sv[i];           // Gr /= shellVolumes
-                float G = pf[i] * (g - 1.0f);      // (4*PI*r*rho0)*(Gr-1)
-                float out = G;
-                if (rsp) {
-                    float model;
-                    if (c.kind == FRMC_PCF) {
-                        // g(r); a scale factor acts on G(r): g = 1 + (sf*G)/(4*PI*r*rho0)
-                        model = g;
-                        if (c.hasSF) {
-                            const float Gsf = G * c.sf;
-                            model = 1.0f + Gsf / pf[i];
-                        }
-                    } else {
-                        model = c.hasSF ? G * c.sf : G;
+                        for (int i = 0; i < U; i++) dv[t][i] = dn[tc * H + rr[i]];
                     }
-                    const float diff = ex[i] - model;
-                    float v = diff * diff;
-                    if (c.hasW) v = dw[i] * v;
-                    out = v;
+                    // pair types in order: Gr += wij*nij/Dij with nij integer-valued (exact in float32)
+#pragma unroll
+                    for (int t = 0; t < TMAX; t++) {
+                        if (t >= T) break;
+                        const float wt = c.w[t], Dt = c.D[t];
+#pragma unroll
+                        for (int i = 0; i < U; i++) {
+                            const float nij = (float)(bv[t][i] + dv[t][i]);
+                            float term = wt * nij;
+                            term = term / Dt;
+                            Gr[i] = Gr[i] + term;
+                        }
+                    }
+                } else {
+                    int32_t b1[U], d1[U];
+#pragma unroll
+                    for (int i = 0; i < U; i++) {
+                        b1[i] = base[rr[i]];
+                        d1[i] = dn[rr[i]];
+                    }
+                    // pair types in order; the next type's counts load while this one is summed
+                    for (int32_t t = 0; t < T; t++) {
+                        const int32_t tn = min(t + 1, T - 1);
+                        int32_t nb[U], nd[U];
+#pragma unroll
+                        for (int i = 0; i < U; i++) {
+                            nb[i] = base[tn * H + rr[i]];
+                            nd[i] = dn[tn * H + rr[i]];
+                        }
+                        const float wt = c.w[t], Dt = c.D[t];
+#pragma unroll
+                        for (int i = 0; i < U; i++) {
+                            // Gr += wij*nij/Dij with nij integer-valued (exact in float32)
+                            const float nij = (float)(b1[i] + d1[i]);
+                            float term = wt * nij;
+                            term = term / Dt;
+                            Gr[i] = Gr[i] + term;
+                            b1[i] = nb[i];
+                            d1[i] = nd[i];
+                        }
+                    }
                 }
-                if (r0 + tid + NT * i < H) Gs[r0 + tid + NT * i] = out;
+#pragma unroll
+                for (int i = 0; i < U; i++) {
+                    float g = Gr[i] / sv[i];           // Gr /= shellVolumes
+                    float G = pf[i] * (g - 1.0f);      // (4*PI*r*rho0)*(Gr-1)
+                    float out = G;
+                    if (rsp) {
+                        float model;
+                        if (c.kind == FRMC_PCF) {
+                            // g(r); a scale factor acts on G(r): g = 1 + (sf*G)/(4*PI*r*rho0)
+                            model = g;
+                            if (c.hasSF) {
+                                const float Gsf = G * c.sf;
+                                model = 1.0f + Gsf / pf[i];
+                            }
+                        } else {
+                            model = c.hasSF ? G * c.sf : G;
+                        }
+                        const float diff = ex[i] - model;
+                        float v = diff * diff;
+                        if (c.hasW) v = dw[i] * v;
+                        out = v;
+                    }
+                    if (r0 + tid + NT * i < H) Gs[j * Hp + r0 + tid + NT * i] = out;
+                }
             }
         }
     }
     __syncthreads();
     if (rsp) {
-        const float chi = block_pairwise(Gs, c, scratch, lprog);
-        if (tid == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
-        if (F) {
-            // np.sum([chi]) from 0, as k_eval_b sums it
-            if (tid == 0) {
-                const float tse = 0.0f + chi;
-                publish(V.out_res + (int64_t)p * 2 + 1, chi, F->seq);
-                V.tse_dev[p] = tse;
-                publish(V.out_res + (int64_t)p * 2, tse, F->seq);
+        for (int32_t j = 0; j < np; j++) {
+            const float chi = block_pairwise(Gs + j * Hp, c, scratch, lprog);
+            const int32_t p = p0 + j;
+            if (tid == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
+            if (F) {
+                // np.sum([chi]) from 0, as k_eval_b sums it
+                if (tid == 0) {
+                    const float tse = 0.0f + chi;
+                    publish(V.out_res + (int64_t)p * 2 + 1, chi, F->seq);
+                    V.tse_dev[p] = tse;
+                    publish(V.out_res + (int64_t)p * 2, tse, F->seq);
+                }
+                for (int64_t s = (int64_t)p * blockDim.x + tid; s < F->zero_len; s += (int64_t)F->nb * blockDim.x)
+                    F->dn_zero[s] = 0;
             }
-            for (int64_t s = (int64_t)p * blockDim.x + tid; s < F->zero_len; s += (int64_t)F->nb * blockDim.x)
-                F->dn_zero[s] = 0;
+            __syncthreads();  // thread 0's combine read scratch; the next proposal rewrites it
         }
         return;
     }
-    if (w != 0) return;  // the S(Q) sum: one Q per lane of wave 0
+    if (w >= np) return;  // the S(Q) sum: wave w for proposal p0 + w, one Q per lane
+    const int32_t p = p0 + w;
+    Gs += w * Hp;
     // The S(Q) sum starts from +0, as NumPy's add reduction does (a sum of
     // -0 products is +0).
     float acc = 0.0f;
@@ -542,11 +591,6 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
     }
 }
 
-// grid (nunits, k)
-__global__ void __launch_bounds__(kEvalThreads) k_eval_a(View V, int32_t k) {
-    extern __shared__ float sm[];
-    eval_a_body(V, V.dn, blockIdx.y, blockIdx.x, sm, nullptr);
-}
 
 // --------------------------------------------------------------- k_eval_b
 
@@ -573,7 +617,7 @@ __device__ __forceinline__ void eval_b_body(const View &V, int32_t *dn_zero, int
         float chi;
         if (!c.rspace) {
             const float *src = V.sqbuf + ((int64_t)p * V.C + ci) * V.Mmax;
-            for (int32_t q = threadIdx.x; q < c.M; q += blockDim.x) sm[q] = src[q];
+            for (int32_t q = threadIdx.x; q < c.M; q += blockDim.x) sm[q] = load_dev(src + q);
             for (int32_t q = threadIdx.x; q < c.nprog; q += blockDim.x) lprog[q] = c.prog[q];
             __syncthreads();
             for (int32_t l = threadIdx.x; l < c.nleaves; l += blockDim.x)
@@ -603,6 +647,17 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
     eval_b_body(V, V.dn_zero, V.zero_len, V.seq, blockIdx.x, gridDim.x, sm);
 }
 
+// --------------------------------------------------------------- k_eval_a
+
+// grid (nunits, ceil(k/EP)): block (u, g) evaluates unit u for the EP
+// proposals g*EP.. (one S(Q) wave each); U bins per thread per G(r) pass
+template <int U, int EP>
+__global__ void __launch_bounds__(kEvalThreads) k_eval_a(View V, int32_t k) {
+    extern __shared__ float sm[];
+    const int32_t p0 = blockIdx.y * EP, np = min(EP, k - p0);
+    eval_a_body<U, EP>(V, V.dn, p0, np, blockIdx.x, sm, nullptr);
+}
+
 // ----------------------------------------------------------- batched windows
 
 // Windows of many engines (replicas) in one launch per kernel: the blocks of
@@ -610,6 +665,34 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
 // map gives each block its window (index into the step's windows) and its
 // block index within that window.  A block runs the single-engine kernel's
 // body on that replica's view, window and proposals.
+//
+// The map entry, the window record, the replica's view and its proposals are
+// copied by the host into device memory (the step block is rewritten at the
+// same address every other step, the views once per run).  Every block
+// stages what it reads of them into LDS with coherent lane-varying loads:
+// uniform reads of copied data would otherwise compile to scalar loads
+// through the scalar data cache, which has returned stale values here.
+template <class T>
+__device__ __forceinline__ void stage(T &dst, const T *src) {
+    static_assert(sizeof(T) % 4 == 0, "staged records are whole 32-bit words");
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(src);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&dst);
+    for (int i = threadIdx.x; i < (int)(sizeof(T) / 4); i += blockDim.x) d[i] = load_dev(s + i);
+    __syncthreads();
+}
+
+// map entry, window record and view of this block of a batched kernel
+struct BatchLds {
+    int2 m;
+    BatchWin W;
+    View V;
+};
+
+__device__ __forceinline__ void stage_block(BatchLds &S, const int2 *map, const BatchArgs &A) {
+    stage(S.m, map + blockIdx.x);
+    stage(S.W, A.win + S.m.x);
+    stage(S.V, A.views + S.W.rep);
+}
 
 // The fields of a replica's view that the batched delta loop reads, copied
 // to registers once per workgroup: the view itself sits in global memory,
@@ -694,6 +777,7 @@ __device__ __forceinline__ void commit_pair_x(const DeltaCtx &X, float s, int32_
 template <bool PBC, bool ORTHO, bool LDSB>
 __device__ __forceinline__ void delta_chunk(const View &V, const BatchWin &W, const Prop *props, int32_t pb,
                                             int32_t ch, int32_t nch, int32_t *bins) {
+    // props: this workgroup's kBP proposals (staged in LDS), props[q] for q < np
     const int tid = threadIdx.x;
     DeltaCtx X;
     load_ctx(V, X);
@@ -705,7 +789,7 @@ __device__ __forceinline__ void delta_chunk(const View &V, const BatchWin &W, co
     float ox[kBP], oy[kBP], oz[kBP], nx[kBP], ny[kBP], nz[kBP];
 #pragma unroll
     for (int q = 0; q < kBP; q++) {
-        const Prop &pr = props[p0 + (q < np ? q : 0)];
+        const Prop &pr = props[q < np ? q : 0];
         qa[q] = q < np ? pr.atom : -1;
         qe[q] = pr.el;
         ox[q] = pr.old[0];
@@ -782,12 +866,22 @@ __device__ __forceinline__ void delta_chunk_any(const View &V, const BatchWin &W
 
 __global__ void __launch_bounds__(256) k_delta_batch(BatchArgs A) {
     extern __shared__ int32_t bins[];  // kBP delta blocks (the largest that fits kBatchLdsInts)
-    const int2 m = A.map_d[blockIdx.x];
-    const BatchWin &W = A.win[m.x];
-    const View &V = A.views[W.rep];
+    __shared__ BatchLds S;
+    __shared__ Prop sp[kBP];
+    stage_block(S, A.map_d, A);
+    const BatchWin &W = S.W;
+    const View &V = S.V;
     const int32_t nch = (V.n + kChunk - 1) / kChunk;
-    const int32_t pb = m.y / nch, ch = m.y % nch;
-    const Prop *props = A.props + W.prop;
+    const int32_t pb = S.m.y / nch, ch = S.m.y % nch;
+    {
+        // the workgroup's proposals (those past the window's end are not read)
+        const int32_t p0 = pb * kBP, np = min(kBP, W.k - p0);
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(A.props + W.prop + p0);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(sp);
+        for (int i = threadIdx.x; i < np * (int)(sizeof(Prop) / 4); i += blockDim.x) dst[i] = load_dev(src + i);
+        __syncthreads();
+    }
+    const Prop *props = sp;
     if (V.isPBC) {
         if (V.ortho) delta_chunk_any<true, true>(V, W, props, pb, ch, nch, bins);
         else delta_chunk_any<true, false>(V, W, props, pb, ch, nch, bins);
@@ -798,23 +892,24 @@ __global__ void __launch_bounds__(256) k_delta_batch(BatchArgs A) {
 
 __global__ void __launch_bounds__(kEvalThreads) k_eval_a_batch(BatchArgs A) {
     extern __shared__ float sm[];
-    const int2 m = A.map_a[blockIdx.x];
-    const BatchWin &W = A.win[m.x];
-    const View &V = A.views[W.rep];
+    __shared__ BatchLds S;
+    stage_block(S, A.map_a, A);
+    const BatchWin &W = S.W;
+    const View &V = S.V;
+    const int2 m = S.m;
     if (V.C == 1 && V.c[0].rspace) {
         const Fuse F{W.dn_zero, W.zero_len, W.seq, W.k};
-        eval_a_body(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, &F);
+        eval_a_body<4, 1>(V, W.dn, m.y / V.nunits, 1, m.y % V.nunits, sm, &F);
     } else {
-        eval_a_body(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, nullptr);
+        eval_a_body<4, 1>(V, W.dn, m.y / V.nunits, 1, m.y % V.nunits, sm, nullptr);
     }
 }
 
 __global__ void __launch_bounds__(256) k_eval_b_batch(BatchArgs A) {
     extern __shared__ float sm[];
-    const int2 m = A.map_b[blockIdx.x];
-    const BatchWin &W = A.win[m.x];
-    const View &V = A.views[W.rep];
-    eval_b_body(V, W.dn_zero, W.zero_len, W.seq, m.y, W.k, sm);
+    __shared__ BatchLds S;
+    stage_block(S, A.map_b, A);
+    eval_b_body(S.V, S.W.dn_zero, S.W.zero_len, S.W.seq, S.m.y, S.W.k, sm);
 }
 
 // ---------------------------------------------------------------- k_commit
@@ -939,15 +1034,29 @@ int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hi
     return FRMC_OK;
 }
 
-size_t eval_lds_bytes(const View &V) {
-    return sizeof(float) * ((size_t)((V.Hmax + 3) & ~3) + V.maxLeaves + kStack + (size_t)2 * V.maxLeaves);
+size_t eval_lds_bytes(const View &V, int32_t np) {
+    return sizeof(float) * ((size_t)np * ((V.Hmax + 3) & ~3) + V.maxLeaves + kStack + (size_t)2 * V.maxLeaves);
 }
 
-int launch_eval(const View &V, int32_t k, hipStream_t st) {
-    const size_t shm_a = eval_lds_bytes(V);
-    if (shm_a > 48 * 1024)
-        (void)hipFuncSetAttribute((const void *)k_eval_a, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm_a);
-    k_eval_a<<<dim3((unsigned)V.nunits, (unsigned)k), kEvalThreads, shm_a, st>>>(V, k);
+template <int U, int EP>
+static void launch_k_eval_a(const View &V, int32_t k, hipStream_t st) {
+    const size_t shm = eval_lds_bytes(V, EP);
+    if (shm > 48 * 1024)
+        (void)hipFuncSetAttribute((const void *)k_eval_a<U, EP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm);
+    k_eval_a<U, EP><<<dim3((unsigned)V.nunits, (unsigned)((k + EP - 1) / EP)), kEvalThreads, shm, st>>>(V, k);
+}
+
+// shape s of k_eval_a: bins per thread per G(r) pass U = s / 4 ? 8 : 4,
+// proposals per block EP = 1 << (s % 4) (1, 2 or 4); every shape gives the
+// same results
+int launch_eval(const View &V, int32_t k, hipStream_t st, int32_t shape) {
+    const bool u8 = shape >= 4 && V.Hmax > 4 * kEvalThreads;
+    switch (shape % 4) {
+        case 0: if (u8) launch_k_eval_a<8, 1>(V, k, st); else launch_k_eval_a<4, 1>(V, k, st); break;
+        case 1: if (u8) launch_k_eval_a<8, 2>(V, k, st); else launch_k_eval_a<4, 2>(V, k, st); break;
+        default: if (u8) launch_k_eval_a<8, 4>(V, k, st); else launch_k_eval_a<4, 4>(V, k, st); break;
+    }
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;
 }

@@ -305,6 +305,223 @@ __global__ void __launch_bounds__(NT) k_full_pairs(const float *__restrict__ coo
         atomicAdd(reinterpret_cast<unsigned long long *>(&counts[2 * A.size + 2]), lds_rechecked);
 }
 
+// ------------------------------------------------ cell-list full histogram
+//
+// For periodic cells much larger than the histogram range, most pairs are
+// out of range.  The atoms are sorted into a grid of m0 x m1 x m2 fractional
+// cells, each at least the range (plus a margin) across: an atom's in-range
+// partners then lie in its own or the 26 neighbouring cells (the margin
+// covers the float32 rounding of the difference, image shift and norm).  A
+// work item is (cell c, one of 14 offsets: itself and 13 of the 26
+// neighbours, one of each opposite pair), so every unordered cell pair is
+// visited once; pairs whose cells are not neighbours are never evaluated and
+// count nowhere in either arithmetic.  Each pair keeps the reference's slot
+// [element(earlier index)][element(later index)] and intra/inter by molecule;
+// its distance is the same float32 (or binary64) value in either order.
+struct CellGrid {
+    int32_t m0, m1, m2;
+    int64_t ncell;
+    int64_t items;      // 14 per cell
+    int64_t per_block;  // items per workgroup
+};
+
+__device__ __forceinline__ int32_t cell_of(float x, int32_t m) {
+    const float f = x - floorf(x);  // exact: the fraction of a float
+    const int32_t c = (int32_t)(f * (float)m);
+    return c < 0 ? 0 : (c >= m ? m - 1 : c);
+}
+
+__global__ void __launch_bounds__(256) k_cell_count(const float *__restrict__ coords, int32_t n, CellGrid G,
+                                                    int32_t *__restrict__ cid, int32_t *__restrict__ cnt) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t cx = cell_of(coords[3 * (int64_t)i], G.m0), cy = cell_of(coords[3 * (int64_t)i + 1], G.m1),
+                  cz = cell_of(coords[3 * (int64_t)i + 2], G.m2);
+    const int32_t c = (cx * G.m1 + cy) * G.m2 + cz;
+    cid[i] = c;
+    atomicAdd(&cnt[c], 1);
+}
+
+// exclusive scan of ncell counts into start[0..ncell] (one block); fill <- start
+__global__ void __launch_bounds__(1024) k_cell_scan(const int32_t *__restrict__ cnt, int64_t ncell,
+                                                    int32_t *__restrict__ start, int32_t *__restrict__ fill) {
+    __shared__ int32_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (ncell + 1023) / 1024;
+    const int64_t c0 = t * per, c1 = min(ncell, c0 + per);
+    int32_t sum = 0;
+    for (int64_t c = c0; c < c1; c++) sum += cnt[c];
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int32_t acc = 0;
+        for (int q = 0; q < 1024; q++) {
+            const int32_t v = part[q];
+            part[q] = acc;
+            acc += v;
+        }
+        start[ncell] = acc;
+    }
+    __syncthreads();
+    int32_t acc = part[t];
+    for (int64_t c = c0; c < c1; c++) {
+        start[c] = acc;
+        fill[c] = acc;
+        acc += cnt[c];
+    }
+}
+
+// cell-sorted copies: (x, y, z, molecule bits), element, original index
+__global__ void __launch_bounds__(256) k_cell_scatter(const float *__restrict__ coords, int32_t n,
+                                                      const int32_t *__restrict__ mol,
+                                                      const int32_t *__restrict__ elem,
+                                                      const int32_t *__restrict__ cid, int32_t *__restrict__ fill,
+                                                      float4 *__restrict__ sx, int32_t *__restrict__ se,
+                                                      int32_t *__restrict__ sid) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t pos = atomicAdd(&fill[cid[i]], 1);
+    sx[pos] = make_float4(coords[3 * (int64_t)i], coords[3 * (int64_t)i + 1], coords[3 * (int64_t)i + 2],
+                          __int_as_float(mol[i]));
+    se[pos] = elem[i];
+    sid[pos] = i;
+}
+
+// offset o of the 14 visited per cell: 0 the cell itself, 1..13 the
+// neighbours (dx, dy, dz) with dx > 0, or dx == 0 and dy > 0, or dx == dy == 0
+// and dz > 0
+__device__ __forceinline__ void cell_offset(int32_t o, int32_t &dx, int32_t &dy, int32_t &dz) {
+    if (o == 0) {
+        dx = dy = dz = 0;
+        return;
+    }
+    if (o <= 9) {  // dx = 1: the 9 (dy, dz)
+        dx = 1;
+        dy = (o - 1) / 3 - 1;
+        dz = (o - 1) % 3 - 1;
+    } else if (o <= 12) {  // dx = 0, dy = 1
+        dx = 0;
+        dy = 1;
+        dz = o - 11;
+    } else {  // dx = dy = 0, dz = 1
+        dx = 0;
+        dy = 0;
+        dz = 1;
+    }
+}
+
+// NT threads, up to R rows per thread per pass (a cell of more than R*NT
+// atoms takes several passes); LDS_HIST: workgroup-private bins as in
+// k_full_pairs, written to this workgroup's partial at the end.
+template <int NT, int R, bool ORTHO, bool LDS_HIST, bool EDGE>
+__global__ void __launch_bounds__(NT) k_full_cells(const float4 *__restrict__ sx, const int32_t *__restrict__ se,
+                                                    const int32_t *__restrict__ sid,
+                                                    const int32_t *__restrict__ cstart, CellGrid G, Basis B,
+                                                    HistArgs A, Edge64 E, int32_t *__restrict__ partial,
+                                                    int32_t *__restrict__ counts) {
+    extern __shared__ int32_t lds_hist[];
+    __shared__ float4 tile_x[NT];
+    __shared__ int32_t tile_e[NT], tile_id[NT];
+    __shared__ int32_t lds_overflow;
+    __shared__ unsigned long long lds_rechecked;
+    const int tid = threadIdx.x;
+    const int32_t size = (int32_t)A.size;
+    if (LDS_HIST)
+        for (int32_t s = tid; s < 2 * size; s += NT) lds_hist[s] = 0;
+    if (tid == 0) {
+        lds_overflow = 0;
+        lds_rechecked = 0;
+    }
+    const int64_t it0 = (int64_t)blockIdx.x * G.per_block;
+    const int64_t it1 = min(G.items, it0 + G.per_block);
+    int32_t lost = 0;
+    uint32_t nre = 0;
+    for (int64_t item = it0; item < it1; item++) {
+        const int64_t c = item / 14;
+        const int32_t o = (int32_t)(item % 14);
+        const int32_t cx = (int32_t)(c / ((int64_t)G.m1 * G.m2)), cy = (int32_t)((c / G.m2) % G.m1),
+                      cz = (int32_t)(c % G.m2);
+        int32_t dx, dy, dz;
+        cell_offset(o, dx, dy, dz);
+        const int64_t c2 = ((int64_t)((cx + dx + G.m0) % G.m0) * G.m1 + (cy + dy + G.m1) % G.m1) * G.m2 +
+                           (cz + dz + G.m2) % G.m2;
+        const bool self = o == 0;
+        const int32_t r0 = cstart[c], r1 = cstart[c + 1], q0 = cstart[c2], q1 = cstart[c2 + 1];
+        for (int32_t rp = r0; rp < r1; rp += NT * R) {
+            float rx[R], ry[R], rz[R];
+            int32_t rm[R], re[R], ri[R], rpos[R];
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                rpos[k] = rp + tid + NT * k;
+                const int32_t pr = min(rpos[k], r1 - 1);  // clamped; rows past the cell never count
+                const float4 v = sx[pr];
+                rx[k] = v.x;
+                ry[k] = v.y;
+                rz[k] = v.z;
+                rm[k] = __float_as_int(v.w);
+                re[k] = se[pr];
+                ri[k] = sid[pr];
+            }
+            for (int32_t tq = q0; tq < q1; tq += NT) {
+                __syncthreads();  // previous tile consumed
+                {
+                    const int32_t j = min(tq + tid, q1 - 1);
+                    tile_x[tid] = sx[j];
+                    tile_e[tid] = se[j];
+                    tile_id[tid] = sid[j];
+                }
+                __syncthreads();
+                const int32_t jn = min(NT, q1 - tq);
+#pragma unroll 2
+                for (int32_t t = 0; t < jn; t++) {
+                    const float4 q = tile_x[t];
+                    const int32_t ej = tile_e[t], idj = tile_id[t];
+#pragma unroll
+                    for (int k = 0; k < R; k++) {
+                        const float s = pair_s<true, ORTHO>(rx[k], ry[k], rz[k], q.x, q.y, q.z, B);
+                        const bool cand = EDGE ? (s >= E.s_lo && s < E.s_hi) : in_range(s, A.g);
+                        const bool ok = cand && rpos[k] < r1 && (!self || tq + t > rpos[k]);
+                        if (ok) {
+                            // the earlier atom is the reference's atomIndex: its element first
+                            const bool rfirst = ri[k] < idj;
+                            int64_t b;
+                            if (EDGE) {
+                                bool re64 = false;
+                                b = rfirst ? edge_bin<true>(s, rx[k], ry[k], rz[k], q.x, q.y, q.z, B, E, re64)
+                                           : edge_bin<true>(s, q.x, q.y, q.z, rx[k], ry[k], rz[k], B, E, re64);
+                                nre += re64;
+                                if (b < 0) continue;
+                            } else {
+                                b = bin_of(s, A.g);
+                            }
+                            const int32_t ea = rfirst ? re[k] : ej, eb = rfirst ? ej : re[k];
+                            const int64_t flat = ((int64_t)ea * A.nel + eb) * A.g.H + b;
+                            const bool intra = __float_as_int(q.w) == rm[k];
+                            if (flat >= A.size) {
+                                lost++;
+                            } else if (LDS_HIST) {
+                                atomicAdd(&lds_hist[(intra ? 0 : size) + (int32_t)flat], 1);
+                            } else {
+                                atomicAdd(&counts[(intra ? 0 : A.size) + flat], 1);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (lost) atomicAdd(&lds_overflow, lost);
+    if (EDGE && nre) atomicAdd(&lds_rechecked, (unsigned long long)nre);
+    __syncthreads();
+    if (LDS_HIST) {
+        int32_t *dst = partial + (int64_t)blockIdx.x * 2 * size;
+        for (int32_t s = tid; s < 2 * size; s += NT) dst[s] = lds_hist[s];
+    }
+    if (tid == 0 && lds_overflow) atomicAdd(&counts[2 * A.size], lds_overflow);
+    if (EDGE && tid == 0 && lds_rechecked)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&counts[2 * A.size + 2]), lds_rechecked);
+}
+
 // counts[s] += sum over the workgroup partials p in this split of partial[p][s]
 __global__ void __launch_bounds__(256) k_sum_partials(const int32_t *__restrict__ partial, int32_t nparts,
                                                       int64_t len, int32_t per_split,
@@ -467,7 +684,31 @@ static FullSched full_sched(int32_t n, int32_t nt, int32_t R, int64_t blocks) {
 // counts (integer atomics), so the choice only changes the time.
 struct FullChoice {
     int32_t nt = 256, R = 2, per_cu = 2;
+    bool cells = false;  // the cell-list kernel (R = 1024 / nt rows per thread per pass)
 };
+
+// Cells per axis for pairs up to maxR apart in the periodic cell B: each
+// axis gets floor(h / w) cells, h the spacing of the cell's lattice planes
+// across it and w = maxR (1 + 1e-5) + 0.01 A (the margin covers the float32
+// rounding of a pair's difference, image shift and norm).  False when an
+// axis would have fewer than 3 cells (neighbour offsets -1, 0, +1 must name
+// distinct cells) or the grid has fewer than 125 cells (too few to prune).
+static bool cell_grid(const Basis &B, double maxR, int32_t m[3]) {
+    double b[9];
+    for (int q = 0; q < 9; q++) b[q] = B.b[q];
+    const double vol = fabs(b[0] * (b[4] * b[8] - b[5] * b[7]) - b[1] * (b[3] * b[8] - b[5] * b[6]) +
+                            b[2] * (b[3] * b[7] - b[4] * b[6]));
+    const double w = maxR * (1.0 + 1e-5) + 0.01;
+    for (int a = 0; a < 3; a++) {
+        const double *u = b + 3 * ((a + 1) % 3), *v = b + 3 * ((a + 2) % 3);
+        const double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
+        const double h = vol / sqrt(cx * cx + cy * cy + cz * cz);
+        const double f = floor(h / w);
+        if (!(f >= 3.0)) return false;
+        m[a] = (int32_t)std::min(f, 64.0);
+    }
+    return (int64_t)m[0] * m[1] * m[2] >= 125;
+}
 
 constexpr int32_t kMaxPerCu = 16;  // workgroups per CU a schedule may ask for
 
@@ -501,9 +742,18 @@ static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, int64_t, int, int>, FullChoice> g_tuned;
 
 // One full-histogram pass with schedule C: counts += all pairs' bins.
+static int run_cells(const FullChoice &C, const float *coords, int32_t n, const Basis &B, bool ortho,
+                     const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E, int ncu,
+                     int32_t *counts, hipStream_t st, int64_t *nblocks);
+
+static double range_of(const HistArgs &A, const Edge64 *E) {
+    return E ? sqrt((double)E->s_hi) : (double)A.g.maxD;
+}
+
 static int run_full(const FullChoice &C, const float *coords, int32_t n, const Basis &B, int isPBC, bool ortho,
                     const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E, int ncu,
                     int32_t *counts, hipStream_t st, int64_t *nblocks) {
+    if (C.cells) return run_cells(C, coords, n, B, ortho, mol, elem, A, E, ncu, counts, st, nblocks);
     const size_t lds_bytes = sizeof(int32_t) * 2 * A.size;
     const bool lds = lds_bytes <= kMaxLdsHistBytes;
     const FullSched S = full_sched(n, C.nt, C.R, (int64_t)ncu * C.per_cu);
@@ -554,6 +804,98 @@ static int run_full(const FullChoice &C, const float *coords, int32_t n, const B
     return rc;
 }
 
+// The cell-list pass: sort the atoms into the cell grid (count, scan,
+// scatter), then k_full_cells over (cell, offset) items and the partials'
+// sum.  The caller checked cell_grid().
+static int run_cells(const FullChoice &C, const float *coords, int32_t n, const Basis &B, bool ortho,
+                     const int32_t *mol, const int32_t *elem, const HistArgs &A, const Edge64 *E, int ncu,
+                     int32_t *counts, hipStream_t st, int64_t *nblocks) {
+    int32_t m[3];
+    if (!cell_grid(B, range_of(A, E), m)) {
+        set_error("cell-list schedule on a cell too small for three cells per axis");
+        return FRMC_EINVAL;
+    }
+    CellGrid G;
+    G.m0 = m[0];
+    G.m1 = m[1];
+    G.m2 = m[2];
+    G.ncell = (int64_t)m[0] * m[1] * m[2];
+    G.items = 14 * G.ncell;
+    int64_t blocks = std::min<int64_t>(G.items, (int64_t)ncu * C.per_cu);
+    G.per_block = (G.items + blocks - 1) / blocks;
+    blocks = (G.items + G.per_block - 1) / G.per_block;
+    if (nblocks) *nblocks = blocks;
+    const size_t lds_bytes = sizeof(int32_t) * 2 * A.size;
+    const bool lds = lds_bytes <= kMaxLdsHistBytes;
+    // scratch: cell of each atom, cell counts / fill cursors / starts, the
+    // sorted atoms, then the workgroup partials
+    const size_t nn = (size_t)(n > 0 ? n : 1), nc = (size_t)G.ncell;
+    const size_t off_cnt = 16 * nn, off_fill = off_cnt + 4 * nc, off_start = off_fill + 4 * nc,
+                 off_se = (off_start + 4 * (nc + 1) + 15) & ~(size_t)15, off_sid = off_se + 4 * nn,
+                 off_cid = off_sid + 4 * nn, off_part = (off_cid + 4 * nn + 255) & ~(size_t)255;
+    const size_t bytes = off_part + (lds ? lds_bytes * (size_t)blocks : 0);
+    char *scr = nullptr;
+    hipError_t e = pool_malloc((void **)&scr, bytes, st);
+    if (e != hipSuccess) {
+        set_error("hipMallocAsync of %zu bytes for the cell-list histogram failed: %s", bytes, hipGetErrorString(e));
+        return FRMC_ENOMEM;
+    }
+    float4 *sx = (float4 *)scr;
+    int32_t *cnt = (int32_t *)(scr + off_cnt), *fill = (int32_t *)(scr + off_fill),
+            *cstart = (int32_t *)(scr + off_start), *se = (int32_t *)(scr + off_se), *sid = (int32_t *)(scr + off_sid),
+            *cid = (int32_t *)(scr + off_cid), *partial = lds ? (int32_t *)(scr + off_part) : nullptr;
+    k_fill_i32<<<blocks_for((int64_t)nc, 256), 256, 0, st>>>(cnt, (int64_t)nc, 0);
+    k_cell_count<<<blocks_for(n, 256), 256, 0, st>>>(coords, n, G, cid, cnt);
+    k_cell_scan<<<1, 1024, 0, st>>>(cnt, G.ncell, cstart, fill);
+    k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(coords, n, mol, elem, cid, fill, sx, se, sid);
+    hipError_t le = hipGetLastError();
+    Edge64 none;
+    memset(&none, 0, sizeof(none));
+    const Edge64 &EE = E ? *E : none;
+    const size_t shm = lds ? lds_bytes : 0;
+#define FRMC_CELLS_LAUNCH(NT_, O, L, ED)                                                                        \
+    do {                                                                                                        \
+        if (shm > 48 * 1024)                                                                                    \
+            (void)hipFuncSetAttribute((const void *)k_full_cells<NT_, 1024 / NT_, O, L, ED>,                    \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);                    \
+        k_full_cells<NT_, 1024 / NT_, O, L, ED><<<(unsigned)blocks, NT_, shm, st>>>(sx, se, sid, cstart, G, B, \
+                                                                                    A, EE, partial, counts);    \
+    } while (0)
+#define FRMC_CELLS_NT(NT_)                                                                                      \
+    do {                                                                                                        \
+        if (ortho) {                                                                                            \
+            if (lds) { if (E) FRMC_CELLS_LAUNCH(NT_, true, true, true); else FRMC_CELLS_LAUNCH(NT_, true, true, false); } \
+            else     { if (E) FRMC_CELLS_LAUNCH(NT_, true, false, true); else FRMC_CELLS_LAUNCH(NT_, true, false, false); } \
+        } else {                                                                                                \
+            if (lds) { if (E) FRMC_CELLS_LAUNCH(NT_, false, true, true); else FRMC_CELLS_LAUNCH(NT_, false, true, false); } \
+            else     { if (E) FRMC_CELLS_LAUNCH(NT_, false, false, true); else FRMC_CELLS_LAUNCH(NT_, false, false, false); } \
+        }                                                                                                       \
+    } while (0)
+    int rc = FRMC_OK;
+    if (le == hipSuccess) {
+        if (C.nt == 256) FRMC_CELLS_NT(256);
+        else if (C.nt == 512) FRMC_CELLS_NT(512);
+        else rc = FRMC_EINVAL;
+        le = hipGetLastError();
+    }
+#undef FRMC_CELLS_NT
+#undef FRMC_CELLS_LAUNCH
+    if (rc == FRMC_EINVAL) set_error("no cell-list kernel for %d threads", C.nt);
+    if (rc == FRMC_OK && le == hipSuccess && lds) {
+        const int32_t per_split = 16;
+        const int64_t len = 2 * A.size;
+        dim3 grid(blocks_for(len, 256), (unsigned)((blocks + per_split - 1) / per_split));
+        k_sum_partials<<<grid, 256, 0, st>>>(partial, (int32_t)blocks, len, per_split, counts);
+        le = hipGetLastError();
+    }
+    if (rc == FRMC_OK && le != hipSuccess) {
+        set_error("cell-list histogram launch failed: %s", hipGetErrorString(le));
+        rc = FRMC_EHIP;
+    }
+    (void)hipFreeAsync(scr, st);
+    return rc;
+}
+
 // Times every candidate schedule on the caller's inputs (into scratch
 // counts, so the caller's counts are untouched): each candidate runs twice
 // between HIP events on the call stream and its faster run is kept.
@@ -579,9 +921,25 @@ static int tune_full(const float *coords, int32_t n, const Basis &B, int isPBC, 
                 // a schedule with fewer items than CUs leaves the chip idle
                 if (full_sched(n, nt, R, 1).items < ncu) continue;
                 bool dup = false;
-                for (const FullChoice &D : cand) dup |= D.nt == nt && D.R == R && D.per_cu == pc;
+                for (const FullChoice &D : cand) dup |= !D.cells && D.nt == nt && D.R == R && D.per_cu == pc;
                 if (!dup) cand.push_back(C);
             }
+    }
+    // the cell-list kernel, when the cell is large enough against the range
+    int32_t mg[3];
+    if (isPBC && cell_grid(B, range_of(A, E), mg)) {
+        const int32_t cnts[2] = {256, 512};
+        for (int32_t nt : cnts) {
+            const int32_t pmax = max_per_cu(lds_bytes + 4 * (size_t)nt, lds, nt);
+            for (int32_t pc = pmax; pc <= 2 * pmax && pc <= kMaxPerCu; pc *= 2) {
+                FullChoice C;
+                C.nt = nt;
+                C.R = 1024 / nt;
+                C.per_cu = pc;
+                C.cells = true;
+                cand.push_back(C);
+            }
+        }
     }
     const int64_t len = 2 * A.size + 4;
     int32_t *scratch = nullptr;
@@ -953,11 +1311,14 @@ int frmc_force_full_schedule(int32_t threads, int32_t rowAtoms, int32_t workgrou
     }
     FRMC_REQUIRE(threads == 256 || threads == 512 || threads == 1024, "threads must be 256, 512 or 1024 (got %d)",
                  threads);
-    FRMC_REQUIRE(rowAtoms == 1 || rowAtoms == 2 || rowAtoms == 4, "rowAtoms must be 1, 2 or 4 (got %d)", rowAtoms);
+    FRMC_REQUIRE(rowAtoms == 1 || rowAtoms == 2 || rowAtoms == 4 || rowAtoms == -1,
+                 "rowAtoms must be 1, 2, 4 or -1 (cell list) (got %d)", rowAtoms);
+    FRMC_REQUIRE(rowAtoms != -1 || threads != 1024, "the cell-list kernel takes 256 or 512 threads");
     FRMC_REQUIRE(workgroupsPerCU >= 1 && workgroupsPerCU <= kMaxPerCu, "workgroupsPerCU must be in 1..%d (got %d)",
                  kMaxPerCu, workgroupsPerCU);
     g_forced.nt = threads;
-    g_forced.R = rowAtoms;
+    g_forced.cells = rowAtoms == -1;
+    g_forced.R = g_forced.cells ? 1024 / threads : rowAtoms;
     g_forced.per_cu = workgroupsPerCU;
     g_forced_on = true;
     return FRMC_OK;
@@ -966,7 +1327,7 @@ int frmc_force_full_schedule(int32_t threads, int32_t rowAtoms, int32_t workgrou
 int frmc_last_full_schedule(int32_t *threads, int32_t *rowAtoms, int64_t *workgroups, int32_t *tuned) {
     FRMC_REQUIRE(threads && rowAtoms && workgroups && tuned, "NULL pointer argument");
     *threads = g_last.nt;
-    *rowAtoms = g_last.R;
+    *rowAtoms = g_last.cells ? -1 : g_last.R;
     *workgroups = g_last_blocks;
     *tuned = g_last_tuned;
     return FRMC_OK;

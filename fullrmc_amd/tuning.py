@@ -4,6 +4,10 @@
 
 A schedule of the all-pairs histogram kernel is (threads per workgroup,
 which is also the column tile; row atoms per thread; workgroups per CU).
+Row atoms -1 selects the cell-list kernel (256 or 512 threads), which
+evaluates only pairs in neighbouring cells of a grid at least the
+histogram range wide; it exists for periodic cells with three or more such
+cells per axis (125 or more in all).
 Every schedule gives the same counts, so tuning changes only the time.  With
 tuning on (the default), the first all-pairs call of a shape class times each
 candidate schedule with HIP events and later calls of the class reuse the
@@ -39,4 +43,5 @@ def last_full_schedule():
     w = ctypes.c_int64(0)
     _lib.check(_lib.lib().frmc_last_full_schedule(ctypes.byref(t), ctypes.byref(r), ctypes.byref(w),
                                                   ctypes.byref(tuned)))
-    return {"threads": t.value, "rowAtoms": r.value, "workgroups": w.value, "tuned": bool(tuned.value)}
+    return {"threads": t.value, "rowAtoms": r.value, "workgroups": w.value, "tuned": bool(tuned.value),
+            "cells": r.value == -1}

@@ -50,8 +50,10 @@ float frmc_last_kernel_ms(void);
 /* Full-histogram schedule autotuner.  No counterpart in the reference (its
    pairs_histograms.pyx:86-176 has one fixed OpenMP loop); every schedule
    gives the same counts.  A schedule is (threads per workgroup = column
-   tile: 256, 512 or 1024; row atoms per thread: 1, 2 or 4; workgroups per
-   CU: 1..16, more than fit at once allowed).  With tuning on (the
+   tile: 256, 512 or 1024; row atoms per thread: 1, 2 or 4, or -1 for the
+   cell-list kernel with 256 or 512 threads, available when a periodic cell
+   holds at least three cells of the range's width per axis and 125 in all;
+   workgroups per CU: 1..16, more than fit at once allowed).  With tuning on (the
    default) the first all-pairs call of a shape class (device, floor(log2 n),
    bins, boundary kind, float64-recompute mode) times each candidate twice
    with HIP events on the call's stream, outside the frmc_set_timing window,
@@ -452,6 +454,17 @@ int frmc_engine_kernel_times(frmc_engine *e, double *ms_out, int64_t *launches_o
    chi-square partials), so the engine state is unchanged.  No reference
    counterpart (measurement only). */
 int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out);
+/* k_eval_a shape tuning.  No reference counterpart (the reference evaluates
+   one move at a time).  A shape is (bins per thread per G(r) pass: 4, or 8
+   when above 1,024 bins; proposals per block, one S(Q) wave each: 1, 2 or
+   4), numbered 0, 1, 2 (4 bins) and 4, 5, 6 (8 bins); every shape gives the
+   same results.  An engine times every shape for each window-size bucket
+   ceil(log2 k) on its first window and keeps the fastest (cached per
+   device, work units, bins and pair types).  frmc_set_eval_shape forces one
+   shape for every engine (-1: tuned); frmc_engine_eval_shapes reports the
+   engine's shape for buckets 0..n-1 (tuning first if needed). */
+int frmc_set_eval_shape(int32_t shape);
+int frmc_engine_eval_shapes(frmc_engine *e, int32_t *shapes_out, int32_t n);
 /* window size of frmc_engine_run_translation: adaptive (default; k is the
    maximum, the size follows the acceptance rate) or always k */
 int frmc_engine_set_adaptive(frmc_engine *e, int enable);
