@@ -68,6 +68,7 @@ _PROTOS = {
     "frmc_replica_batch_times": [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64)],
     "frmc_force_full_schedule": [c_i32, c_i32, c_i32],
     "frmc_last_full_schedule": [c_i32p, c_i32p, ctypes.POINTER(c_i64), c_i32p],
+    "frmc_last_full_evaluated": [ctypes.POINTER(c_i64)],
     # pairs_distances
     "frmc_pairs_distances_to_indexcoords": [c_i32, c_vp, c_i32, c_vp, c_int, c_int, c_vp, c_vp],
     "frmc_pairs_distances_to_multi_indexcoords": [c_vp, c_i32, c_vp, c_i32, c_vp, c_int, c_int, c_vp, c_vp],

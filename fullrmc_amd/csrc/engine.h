@@ -29,9 +29,7 @@ namespace frmc {
 constexpr int kMaxC = 4;   // constraints per engine
 constexpr int kMaxG = 64;  // atoms per group
 constexpr int kQC = 64;    // Q points per k_eval_a wave (one per lane)
-constexpr int kEW = 1;     // S(Q) waves per k_eval_a block: a work unit is kEW*kQC Q points
-constexpr int kEvalThreads = 256;  // k_eval_a block: all threads build G(r), one wave per proposal runs its S(Q) sum
-constexpr int kEP = 4;             // most proposals per k_eval_a block (one S(Q) wave each)
+constexpr int kEvalThreads = 256;  // k_eval_a block: all threads build G(r), waves 0-1 run the S(Q) sums of a leaf
 
 struct DevC {
     int32_t kind;   // FRMC_PDC, FRMC_SFC, FRMC_PCF or FRMC_RSFC
@@ -47,7 +45,8 @@ struct DevC {
     const int32_t *prog;    // postfix combine program over the leaves
     int32_t nleaves, nprog;
     int32_t sameGrid;       // 1: same histogram grid as the previous constraint (bins reused)
-    int32_t uOff;           // first k_eval_a work unit (r space: 1 unit, S(Q): ceil(M/(kEW*kQC)) units)
+    int32_t uOff;           // first k_eval_a work unit (r space: 1 unit, S(Q): one per pairwise-sum leaf)
+    int32_t leafOff;        // S(Q): first leaf-sum word in a proposal's result block
     int32_t *base;          // [T][H]
     int32_t *slot;          // [2][nel][nel][H] + overflow counter
     int32_t dnOff;          // offset in a proposal's delta block
@@ -85,7 +84,11 @@ struct View {
     int32_t *cnt;                  // [kmax] blocks finished per proposal (reset by the last)
     float *sqbuf;                  // [kmax][C][Mmax]
     float *chi2d;                  // [kmax][C] device partials
-    unsigned long long *out_res;   // [kmax][C+1] host-mapped: (seq << 32 | float bits) of total, chi-squares
+    unsigned long long *out_res;   // [kmax][resW] host-mapped: (seq << 32 | float bits) of the total, the
+                                   // chi-squares, then the S(Q) leaf sums
+    int32_t resW;                  // result words per proposal
+    int32_t leafmode;              // 1: k_eval_a publishes chi-squares and leaf sums, the host combines them
+    int32_t leafunits;             // S(Q) work units: 1 pairwise-sum leaves (two waves), 0 64 Q points (one wave)
     float *tse_dev;                // [kmax] totals in device memory (read by k_commit)
     unsigned long long *out_commit;  // [2] host-mapped: (seq << 32 | proposal committed by k_commit)
     uint32_t seq;
@@ -176,7 +179,7 @@ int launch_full(const float *coords, int32_t n, const Basis &B, int isPBC, bool 
 int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t shm_d, size_t shm_a, size_t shm_b,
                  hipStream_t st);
 void pairwise_plan(int32_t n, std::vector<int32_t> &leaves, std::vector<int32_t> &prog);
-size_t eval_lds_bytes(const View &V, int32_t np = 1);  // np G(r) arrays
+size_t eval_lds_bytes(const View &V);
 size_t eval_b_lds_bytes(const View &V);
 
 }  // namespace frmc
@@ -201,7 +204,11 @@ struct frmc_engine {
     float *sqbuf = nullptr, *chi2d = nullptr;
     frmc::Stage stage[2];
     int cur_stage = 0;     // staging buffer of the last evaluated window
-    unsigned long long *h_res = nullptr;  // host-mapped results [kmax][C+1]
+    unsigned long long *h_res = nullptr;  // host-mapped results [kmax][resW]
+    int32_t resW = 0;
+    bool leaf_window = false;             // the last window's totals are combined on the host
+    bool leaf_units = false;              // S(Q) work units are pairwise-sum leaves (else 64 Q points)
+    std::vector<int32_t> h_prog[frmc::kMaxC];  // S(Q) pairwise combine programs (host copies)
     std::vector<float> h_tse, h_chi2;     // results of the last window, unpacked
     float *tse_dev = nullptr;
     unsigned long long *h_commit = nullptr;  // [2] written by k_commit (slot = window parity)
@@ -240,6 +247,7 @@ struct frmc_engine {
 namespace frmc {
 View make_view(frmc_engine *e, int stage_idx);
 int32_t eval_shape(frmc_engine *e, int32_t k);
+void use_unit_layout(frmc_engine *e, bool leaf);  // S(Q) work units: pairwise leaves or 64 Q points
 int launch_device_decided(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
                           const float *u, float tol);
 int finish_device_decided(frmc_engine *e, int32_t k, float *tse_out);

@@ -56,6 +56,9 @@ View make_view(frmc_engine *e, int s) {
     V.sqbuf = e->sqbuf;
     V.chi2d = e->chi2d;
     V.out_res = e->h_res;
+    V.resW = e->resW;
+    V.leafmode = 0;
+    V.leafunits = e->leaf_units ? 1 : 0;
     V.tse_dev = e->tse_dev;
     V.out_commit = e->h_commit;
     V.seq = e->seq;
@@ -158,13 +161,30 @@ static double now_s() {
 // the host-mapped result block carries the window's number.  A spin on pinned
 // memory returns within a microsecond of k_eval_b's stores; after 60 s it
 // synchronises the stream to surface a device error.
+// NumPy's pairwise combine of an S(Q) constraint's leaf sums (postfix
+// program over the leaves, float32 adds)
+static float combine_leaves(const std::vector<int32_t> &prog, const float *leafv) {
+    float stk[64];
+    int sp = 0;
+    for (int32_t op : prog) {
+        if (op >= 0) {
+            stk[sp++] = leafv[op];
+        } else {
+            const float b = stk[--sp];
+            const float x = stk[--sp];
+            stk[sp++] = x + b;
+        }
+    }
+    return stk[0];
+}
+
 static int wait_done(frmc_engine *e, int32_t k) {
     volatile unsigned long long *res = e->h_res;
     const uint32_t s = e->seq;
-    const int32_t W = e->C + 1;
+    const int32_t W = e->resW;
     double t0 = 0.0;
     uint64_t spins = 0;
-    for (int32_t q = 0; q < k * W; q++) {
+    auto wait_word = [&](size_t q) -> int {
         while ((uint32_t)(res[q] >> 32) != s) {
             if ((++spins & 0xffff) == 0) {
                 if (t0 == 0.0) t0 = now_s();
@@ -177,16 +197,51 @@ static int wait_done(frmc_engine *e, int32_t k) {
                 }
             }
         }
-    }
-    __sync_synchronize();
-    for (int32_t p = 0; p < k; p++) {
-        const unsigned long long *w = (const unsigned long long *)res + (size_t)p * W;
-        uint32_t bits = (uint32_t)w[0];
-        memcpy(&e->h_tse[p], &bits, 4);
-        for (int ci = 0; ci < e->C; ci++) {
-            bits = (uint32_t)w[1 + ci];
-            memcpy(&e->h_chi2[(size_t)p * e->C + ci], &bits, 4);
+        return FRMC_OK;
+    };
+    auto word = [&](size_t q) {
+        const uint32_t bits = (uint32_t)res[q];
+        float v;
+        memcpy(&v, &bits, 4);
+        return v;
+    };
+    if (!e->leaf_window) {
+        // k_eval_b published the totals and chi-squares
+        for (int32_t p = 0; p < k; p++)
+            for (int32_t q = 0; q <= e->C; q++)
+                if (int rc = wait_word((size_t)p * W + q)) return rc;
+        __sync_synchronize();
+        for (int32_t p = 0; p < k; p++) {
+            e->h_tse[p] = word((size_t)p * W);
+            for (int ci = 0; ci < e->C; ci++) e->h_chi2[(size_t)p * e->C + ci] = word((size_t)p * W + 1 + ci);
         }
+        return FRMC_OK;
+    }
+    // the r-space chi-squares and the S(Q) leaf sums; each S(Q) chi-square is
+    // their pairwise combine, the total np.sum([chi2_c]) from 0
+    // (Engine.py:3026-3029, StructureFactorConstraints.py:742-770)
+    float leafv[4096];
+    for (int32_t p = 0; p < k; p++) {
+        float tse = 0.0f;
+        for (int ci = 0; ci < e->C; ci++) {
+            const DevC &c = e->c[ci];
+            float chi;
+            if (c.rspace) {
+                const size_t q = (size_t)p * W + 1 + ci;
+                if (int rc = wait_word(q)) return rc;
+                __sync_synchronize();
+                chi = word(q);
+            } else {
+                for (int32_t l = 0; l < c.nleaves; l++)
+                    if (int rc = wait_word((size_t)p * W + c.leafOff + l)) return rc;
+                __sync_synchronize();
+                for (int32_t l = 0; l < c.nleaves; l++) leafv[l] = word((size_t)p * W + c.leafOff + l);
+                chi = combine_leaves(e->h_prog[ci], leafv);
+            }
+            e->h_chi2[(size_t)p * e->C + ci] = chi;
+            tse = tse + chi;
+        }
+        e->h_tse[p] = tse;
     }
     return FRMC_OK;
 }
@@ -217,9 +272,31 @@ static int check_device_decision(frmc_engine *e) {
 // window-size bucket (ceil(log2 k), k up to kmax) every shape is timed on the
 // engine stream, and the fastest is kept per shape class (device, work
 // units, bins, pair types), so engines of one class tune once.
-static const int32_t kEvalShapes[6] = {0, 1, 2, 4, 5, 6};
+// Work units of the current layout: an r-space constraint is one unit; an
+// S(Q) constraint has one per leaf of its pairwise sum (leaf layout: two
+// waves each, the leaf sums go to the host) or one per 64 Q points.
+static void set_units(frmc_engine *e, bool leaf) {
+    e->leaf_units = leaf;
+    int32_t units = 0;
+    for (int ci = 0; ci < e->C; ci++) {
+        e->c[ci].uOff = units;
+        units += e->c[ci].rspace ? 1 : (leaf ? e->c[ci].nleaves : (e->c[ci].M + kQC - 1) / kQC);
+    }
+    e->nunits = units;
+}
+
+// The window's evaluation, shape s: bins per thread per G(r) pass 8 (s & 4,
+// above 1,024 bins) or 4; units aligned to the XCDs (s & 8); leaf layout
+// (s & 16: k_eval_a alone, the host combines the leaf sums) or 64-Q units
+// (k_eval_a, then k_eval_b).  Every shape gives the same results; the best
+// one depends on the window size k.  For each window-size bucket
+// (ceil(log2 k), k up to kmax) every shape's evaluation is timed on the
+// engine stream, twice, and the fastest is kept per shape class (device,
+// constraint layout, pair types, largest window), so engines of one class
+// tune once.
+static const int32_t kEvalShapes[8] = {0, 8, 16, 24, 4, 12, 20, 28};
 static std::mutex g_eval_mu;
-static std::map<std::tuple<int, int32_t, int32_t, int32_t, int32_t>, int32_t> g_eval_tuned;
+static std::map<std::tuple<int, int32_t, int32_t, int32_t, int32_t, int32_t>, std::vector<int8_t>> g_eval_tuned;
 static int g_eval_forced = -1;  // frmc_set_eval_shape
 
 static int32_t k_bucket(int32_t k) {
@@ -228,75 +305,95 @@ static int32_t k_bucket(int32_t k) {
     return b;
 }
 
+// one timed evaluation of shape sh for k proposals (no effect on the engine
+// state: results are written with window number 0, which no window uses, and
+// no delta buffer is zeroed)
+static int launch_eval_shape(frmc_engine *e, int32_t k, int32_t sh) {
+    set_units(e, (sh & 16) != 0);
+    View V = make_view(e, e->cur_stage);
+    V.seq = 0;
+    V.zero_len = 0;
+    V.leafmode = (sh & 16) ? 1 : 0;
+    int rc = launch_eval(V, k, e->st, sh);
+    if (rc == FRMC_OK && !(sh & 16)) rc = launch_eval_b(V, k, e->st);
+    return rc;
+}
+
 static int tune_eval(frmc_engine *e) {
-    const View V = make_view(e, e->cur_stage);
+    const bool was_leaf = e->leaf_units;
     const int32_t nb = k_bucket(e->kmax) + 1;
-    const int32_t ns = V.Hmax > 4 * kEvalThreads ? 6 : 3;  // U = 8 differs from 4 only above 1,024 bins
-    constexpr int reps = 6;
-    std::vector<hipEvent_t> ev(2 * nb * ns, nullptr);
+    const int32_t ns = e->Hmax > 4 * kEvalThreads ? 8 : 4;  // U = 8 differs from 4 only above 1,024 bins
+    constexpr int reps = 5, rounds = 2;
+    std::vector<hipEvent_t> ev(2 * rounds * nb * ns, nullptr);
     int rc = FRMC_OK;
     for (auto &x : ev)
         if (rc == FRMC_OK && hipEventCreate(&x) != hipSuccess) rc = FRMC_EHIP;
     for (int32_t b = 0; b < nb && rc == FRMC_OK; b++) {
         const int32_t k = std::min<int32_t>(1 << b, e->kmax);
-        for (int32_t i = 0; i < ns && rc == FRMC_OK; i++) {
-            rc = launch_eval(V, k, e->st, kEvalShapes[i]);  // warm
-            if (rc == FRMC_OK && hipEventRecord(ev[2 * (b * ns + i)], e->st) != hipSuccess) rc = FRMC_EHIP;
-            for (int r = 0; r < reps && rc == FRMC_OK; r++) rc = launch_eval(V, k, e->st, kEvalShapes[i]);
-            if (rc == FRMC_OK && hipEventRecord(ev[2 * (b * ns + i) + 1], e->st) != hipSuccess) rc = FRMC_EHIP;
-        }
+        for (int round = 0; round < rounds && rc == FRMC_OK; round++)
+            for (int32_t i = 0; i < ns && rc == FRMC_OK; i++) {
+                const size_t q = 2 * (((size_t)b * rounds + round) * ns + i);
+                rc = launch_eval_shape(e, k, kEvalShapes[i]);  // warm
+                if (rc == FRMC_OK && hipEventRecord(ev[q], e->st) != hipSuccess) rc = FRMC_EHIP;
+                for (int r = 0; r < reps && rc == FRMC_OK; r++) rc = launch_eval_shape(e, k, kEvalShapes[i]);
+                if (rc == FRMC_OK && hipEventRecord(ev[q + 1], e->st) != hipSuccess) rc = FRMC_EHIP;
+            }
     }
     if (rc == FRMC_OK && hipStreamSynchronize(e->st) != hipSuccess) rc = FRMC_EHIP;
     for (int32_t b = 0; b < nb && rc == FRMC_OK; b++) {
         float best = 0.f;
         int32_t pick = 0;
-        for (int32_t i = 0; i < ns; i++) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, ev[2 * (b * ns + i)], ev[2 * (b * ns + i) + 1]) != hipSuccess) {
-                rc = FRMC_EHIP;
-                break;
+        for (int32_t i = 0; i < ns && rc == FRMC_OK; i++)
+            for (int round = 0; round < rounds; round++) {
+                const size_t q = 2 * (((size_t)b * rounds + round) * ns + i);
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, ev[q], ev[q + 1]) != hipSuccess) {
+                    rc = FRMC_EHIP;
+                    break;
+                }
+                if ((i == 0 && round == 0) || ms < best) {
+                    best = ms;
+                    pick = kEvalShapes[i];
+                }
             }
-            if (i == 0 || ms < best) {
-                best = ms;
-                pick = kEvalShapes[i];
-            }
-        }
-        e->eval_shape[b] = pick;
+        e->eval_shape[b] = (int8_t)pick;
     }
+    for (int32_t b = nb; b < 16; b++) e->eval_shape[b] = e->eval_shape[nb - 1];
     for (auto x : ev)
         if (x) (void)hipEventDestroy(x);
-    if (rc) set_error("timing the k_eval_a shapes failed");
+    set_units(e, was_leaf);
+    if (rc) set_error("timing the window evaluation shapes failed");
     return rc;
 }
 
-// the k_eval_a shape for a window of k proposals (tuned on first use)
+// the evaluation shape for a window of k proposals (tuned on first use)
 int32_t eval_shape(frmc_engine *e, int32_t k) {
     if (g_eval_forced >= 0) return g_eval_forced;
     if (!e->eval_tuned) {
         int dev = 0;
         (void)hipGetDevice(&dev);
-        const auto key = std::make_tuple(dev, e->nunits, e->Hmax, e->T, e->kmax);
+        int32_t sq_points = 0;
+        for (int ci = 0; ci < e->C; ci++) sq_points = sq_points * 7919 + (e->c[ci].rspace ? -e->c[ci].H : e->c[ci].M);
+        const auto key = std::make_tuple(dev, e->C, e->Hmax, sq_points, e->T, e->kmax);
         bool have = false;
         {
             std::lock_guard<std::mutex> lk(g_eval_mu);
             auto it = g_eval_tuned.find(key);
             if (it != g_eval_tuned.end()) {
-                for (int b = 0; b < 16; b++) e->eval_shape[b] = (int8_t)((it->second >> (3 * (b % 10))) & 7);
+                for (int b = 0; b < 16; b++) e->eval_shape[b] = it->second[b];
                 have = true;
             }
         }
         if (!have) {
             for (int b = 0; b < 16; b++) e->eval_shape[b] = 0;
             if (tune_eval(e) == FRMC_OK) {
-                int32_t packed = 0;
-                for (int b = 0; b < 10; b++) packed |= (int32_t)e->eval_shape[b] << (3 * b);
                 std::lock_guard<std::mutex> lk(g_eval_mu);
-                g_eval_tuned[key] = packed;
+                g_eval_tuned[key] = std::vector<int8_t>(e->eval_shape, e->eval_shape + 16);
             }
         }
         e->eval_tuned = true;
     }
-    return e->eval_shape[std::min<int32_t>(k_bucket(k), 9)];
+    return e->eval_shape[std::min<int32_t>(k_bucket(k), 15)];
 }
 
 // launch delta + eval for the k proposals in staging buffer s (and, with C,
@@ -304,8 +401,18 @@ int32_t eval_shape(frmc_engine *e, int32_t k) {
 // the totals
 static int launch_window(frmc_engine *e, int s, int32_t k, const Inline *I, const CommitCtl *C = nullptr) {
     const bool with_delta = I != nullptr;
+    // the evaluation shape (tuned on the first window) fixes the S(Q) units;
+    // a device-decided commit reads the totals k_eval_b leaves on the device,
+    // so those windows take 64-Q units
+    int32_t shape = eval_shape(e, k);
+    if (C) shape &= ~16;
+    set_units(e, (shape & 16) != 0);
     e->seq++;
     View V = make_view(e, s);
+    // leaf layout: k_eval_a publishes the chi-squares and S(Q) leaf sums, the
+    // host combines them, and k_eval_b is not needed
+    V.leafmode = (shape & 16) ? 1 : 0;
+    e->leaf_window = V.leafmode != 0;
     int rc;
     if (with_delta) {
         KTimer t(e, FRMC_K_DELTA);
@@ -315,10 +422,10 @@ static int launch_window(frmc_engine *e, int s, int32_t k, const Inline *I, cons
     }
     {
         KTimer t(e, FRMC_K_EVAL);
-        rc = launch_eval(V, k, e->st, eval_shape(e, k));
+        rc = launch_eval(V, k, e->st, shape);
     }
     if (rc) return rc;
-    {
+    if (!V.leafmode) {
         KTimer t(e, FRMC_K_EVAL_B);
         rc = launch_eval_b(V, k, e->st);
     }
@@ -328,7 +435,7 @@ static int launch_window(frmc_engine *e, int s, int32_t k, const Inline *I, cons
         rc = launch_commit(V, *I, *C, e->st);
         if (rc) return rc;
     }
-    // k_eval_b zeroes the other buffer for the next window
+    // k_eval_a (host-combined windows) or k_eval_b zeroes the other buffer
     e->dn_used[1 - e->cur_dn] = 0;
     e->dn_used[e->cur_dn] = with_delta ? k : 0;
     e->cur_dn = 1 - e->cur_dn;
@@ -468,7 +575,12 @@ int batch_prepare(frmc_engine *e, int32_t k, const int32_t *group_ids, const flo
     e->cur_dn = 1 - e->cur_dn;
     e->device_decided = false;
     e->deferred = true;
+    e->leaf_window = false;  // k_eval_b_batch publishes the totals
     return FRMC_OK;
+}
+
+void use_unit_layout(frmc_engine *e, bool leaf) {
+    set_units(e, leaf);
 }
 
 // Apply an accepted move still left for a next window's k_delta, as a
@@ -726,6 +838,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             dc.prog = pgp;
             dc.nleaves = (int32_t)(lv.size() / 2);
             dc.nprog = (int32_t)pg.size();
+            e->h_prog[ci] = pg;
             ECHK(dalloc(e, &dc.base, (size_t)e->T * dc.H));
             ECHK(dalloc(e, &dc.slot, 2 * (size_t)e->nel * e->nel * dc.H + 1));
             if (dc.sameGrid) {
@@ -739,18 +852,26 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             e->maxLeaves = std::max(e->maxLeaves, dc.nleaves);
         }
         e->dnStride = off;
-        int32_t units = 0;
+        // work units: an r-space constraint, or a leaf (at most 128 Q points,
+        // two waves) of an S(Q) constraint's pairwise sum; result words per
+        // proposal: total, chi-squares, then the leaf sums
+        // result words per proposal: total, chi-squares, then the leaf sums
+        int32_t words = 1 + e->C;
         for (int ci = 0; ci < e->C; ci++) {
-            e->c[ci].uOff = units;
-            units += e->c[ci].rspace ? 1 : (e->c[ci].M + kEW * kQC - 1) / (kEW * kQC);
+            e->c[ci].leafOff = words;
+            if (!e->c[ci].rspace) words += e->c[ci].nleaves;
         }
-        e->nunits = units;
+        e->resW = words;
+        // the LDS checks below take the larger (leaf) layout's needs; the
+        // tuned shape picks the layout per window
+        set_units(e, false);
+        for (int ci = 0; ci < e->C; ci++)
+            if (e->c[ci].nleaves > 4096) EFAIL(FRMC_EINVAL, "too many Q points (%d) in constraint %d", e->c[ci].M, ci);
         {
             View V = make_view(e, 0);
             // the evaluation kernels keep a constraint's model array in LDS
             // (150 KB leaves room for the batched kernels' staged records)
-            if (eval_lds_bytes(V, kEP) > 150 * 1024)
-                EFAIL(FRMC_EINVAL, "model arrays need more than 150 KB of LDS");
+            if (eval_lds_bytes(V) > 150 * 1024) EFAIL(FRMC_EINVAL, "model arrays need more than 150 KB of LDS");
             if (eval_b_lds_bytes(V) > 150 * 1024)
                 EFAIL(FRMC_EINVAL, "an S(Q) constraint with %d Q points needs %zu bytes of LDS in the chi-square "
                       "kernel (at most 150 KB)", V.Mmax, eval_b_lds_bytes(V));
@@ -781,8 +902,8 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         // blocks of k_eval finished per proposal (the last one resets it)
         ECHK(dalloc(e, &e->cnt, K));
         ECHK(hipMemset(e->cnt, 0, sizeof(int32_t) * K));
-        ECHK(halloc(e, &e->h_res, K * (e->C + 1)));
-        for (size_t q = 0; q < K * (size_t)(e->C + 1); q++) e->h_res[q] = 0;
+        ECHK(halloc(e, &e->h_res, K * e->resW));
+        for (size_t q = 0; q < K * (size_t)e->resW; q++) e->h_res[q] = 0;
         e->h_tse.assign(K, 0.0f);
         e->h_chi2.assign(K * e->C, 0.0f);
         ECHK(dalloc(e, &e->tse_dev, K));
@@ -957,13 +1078,17 @@ int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out
     FRMC_REQUIRE(reps >= 1 && reps <= 100000, "reps=%d out of range [1,100000]", reps);
     // k_eval_a writes only the window's S(Q) and chi-square scratch, which
     // the next window recomputes: the engine state is untouched
-    const View V = make_view(e, e->cur_stage);
+    const int32_t shape = eval_shape(e, k);
+    set_units(e, (shape & 16) != 0);
+    View V = make_view(e, e->cur_stage);
+    V.seq = 0;  // results under no window's number
+    V.zero_len = 0;
+    V.leafmode = (shape & 16) ? 1 : 0;
     hipEvent_t a = nullptr, b = nullptr;
     FRMC_CHECK_HIP(hipEventCreate(&a));
     hipError_t err = hipEventCreate(&b);
     int rc = FRMC_OK;
     if (err == hipSuccess) err = hipEventRecord(a, e->st);
-    const int32_t shape = eval_shape(e, k);
     for (int32_t r = 0; r < reps && err == hipSuccess && rc == FRMC_OK; r++) rc = launch_eval(V, k, e->st, shape);
     if (err == hipSuccess && rc == FRMC_OK) err = hipEventRecord(b, e->st);
     if (err == hipSuccess && rc == FRMC_OK) err = hipEventSynchronize(b);
@@ -981,8 +1106,8 @@ int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out
 }
 
 int frmc_set_eval_shape(int32_t shape) {
-    FRMC_REQUIRE(shape == -1 || shape == 0 || shape == 1 || shape == 2 || shape == 4 || shape == 5 || shape == 6,
-                 "eval shape must be -1 (tuned), 0, 1, 2, 4, 5 or 6 (got %d)", shape);
+    FRMC_REQUIRE(shape >= -1 && shape <= 28 && (shape == -1 || shape % 4 == 0),
+                 "eval shape must be -1 (tuned) or one of 0, 4, 8, ..., 28 (got %d)", shape);
     g_eval_forced = shape;
     return FRMC_OK;
 }
@@ -991,7 +1116,7 @@ int frmc_engine_eval_shapes(frmc_engine *e, int32_t *shapes_out, int32_t n) {
     FRMC_REQUIRE(e && shapes_out && n >= 1 && n <= 16, "bad arguments");
     (void)eval_shape(e, 1);  // tunes on first use
     for (int32_t b = 0; b < n; b++)
-        shapes_out[b] = g_eval_forced >= 0 ? g_eval_forced : e->eval_shape[std::min<int32_t>(b, 9)];
+        shapes_out[b] = g_eval_forced >= 0 ? g_eval_forced : e->eval_shape[b];
     return FRMC_OK;
 }
 

@@ -347,39 +347,56 @@ struct Fuse {
     int32_t nb;  // blocks of the window
 };
 
-// Work unit u for NP proposals at once (np <= NP of them used, from p0):
-// every thread builds each proposal's G(r) into LDS in turn, then the r-space
-// constraint's chi-square is taken per proposal, or wave w runs the S(Q) sum
-// of proposal p0 + w over the unit's Q columns (the waves stream the same
-// matrix columns, so one wave's loads are L1 hits for the others).  dnw is
-// the window's delta buffer, sm the block's dynamic LDS (eval_lds_bytes
-// with NP G(r) arrays); F (NP = 1 only): also k_eval_b's work for one
-// r-space constraint.
-template <int U, int NP>
-__device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, int32_t p0, int32_t np, int32_t u,
-                                            float *sm, const Fuse *F) {
-    const int32_t Hp = (V.Hmax + 3) & ~3;        // keeps float4 reads aligned
-    float *Gs = sm;                              // [NP][Hp] G(r) or PDC squared deviations
-    float *scratch = Gs + NP * Hp;               // [maxLeaves + kStack]
+// Work unit u of proposal p.  A unit is one r-space constraint (PDC, PCF) or
+// one leaf of NumPy's pairwise sum over an S(Q) constraint's Q points (at
+// most 128: waves 0 and 1 run the S(Q) sums of its first and second 64).
+// Every thread builds the proposal's G(r) into LDS first.  dnw is the
+// window's delta buffer, sm the block's dynamic LDS (eval_lds_bytes).
+// V.leafmode (windows whose totals the host combines): an r-space unit
+// publishes its chi-square, an S(Q) unit the sum of its leaf, and every
+// block zeroes a slice of the other delta buffer (nblk blocks, b this one);
+// otherwise the squared deviations go to sqbuf for k_eval_b.  F (batched
+// windows of one r-space constraint): also k_eval_b's work for that
+// constraint.
+template <int U>
+__device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, int32_t p, int32_t u, float *sm,
+                                            const Fuse *F, int64_t b, int64_t nblk) {
+    const int32_t Hp = (V.Hmax + 15) & ~15;      // whole batches of 16 rows (float4 reads aligned)
+    float *Gs = sm;                              // [Hp] G(r) or PDC squared deviations
+    float *scratch = Gs + Hp;                    // [maxLeaves + kStack] (S(Q): the leaf's 128 deviations)
     int32_t *lprog = (int32_t *)(scratch + V.maxLeaves + kStack);  // [2*maxLeaves]
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    if (V.leafmode)
+        for (int64_t s = b * blockDim.x + tid; s < V.zero_len; s += nblk * blockDim.x) V.dn_zero[s] = 0;
     int32_t ci = 0;
     while (ci + 1 < V.C && u >= V.c[ci + 1].uOff) ci++;
     // constraint fields in registers: keeps every load below unconditional
     const DevC c = V.c[ci];
     const int32_t H = c.H, M = c.M, T = V.T;
     const bool rsp = c.rspace;
-    const int32_t *dn0 = dnw + (int64_t)p0 * V.dnStride + c.dnOff;
+    const int32_t *dn = dnw + (int64_t)p * V.dnStride + c.dnOff;
     const int32_t *base = c.base;
-    // S(Q) for q = qc*kQC + lane (SFC units): the first rows of the matrix
-    // column and the experimental value load now, under the G(r) phase
-    static_assert(kEW == 1, "one S(Q) wave per proposal");
-    static_assert(NP <= kEvalThreads / 64, "one wave per proposal");
-    const int32_t q = (u - c.uOff) * kQC + lane;
-    const bool qok = !rsp && q < M;
+    // S(Q) unit: leaf l (Q points [q0, q0 + len), lane q = q0 + 64 w + lane,
+    // waves 0 and 1), or 64 Q points from q0 (wave 0)
+    int32_t leaf = 0, q0 = 0, len = 0;
+    if (!rsp) {
+        leaf = u - c.uOff;
+        if (V.leafunits) {
+            q0 = c.leaves[2 * leaf];
+            len = c.leaves[2 * leaf + 1];
+        } else {
+            q0 = leaf * kQC;
+            len = min(kQC, M - q0);
+        }
+    }
+    const bool chain_wave = !rsp && w < 2 && 64 * w < len;
+    const int32_t q = q0 + 64 * w + lane;
+    const bool qok = chain_wave && 64 * w + lane < len;
     const float *mcol = c.sqm + (int64_t)min(q, M - 1) * c.hpb;  // M[r][q] at mcol[r], 16-byte aligned
-    constexpr int B = 32;                        // rows per register batch (hpb is a multiple of B)
-    float ma[B], mb[B], ga[B], gb[B];
+    // the S(Q) sum streams the matrix column in batches of B rows, NBM of
+    // them in flight (16 x 4 measured 1.6x faster than 32 x 2 on 2,000 rows)
+    constexpr int B = 16, NBM = 4;               // hpb is a multiple of 32 >= B
+    float mr[NBM][B], gg[2][B];
     auto load_m = [&](float *m, int32_t r0) {
         // a batch past the end (a prefetch that is never used) rereads the last one
         const float *src = mcol + min(r0, c.hpb - B);
@@ -389,9 +406,12 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
             m[4 * i4] = v.x; m[4 * i4 + 1] = v.y; m[4 * i4 + 2] = v.z; m[4 * i4 + 3] = v.w;
         }
     };
+    // the first rows of the matrix column and the experimental value load
+    // now, under the G(r) phase
     float ex_q = 0.0f, dw_q = 1.0f;
-    if (!rsp && w < np) {
-        load_m(ma, 0);
+    if (chain_wave) {
+#pragma unroll
+        for (int bq = 0; bq < NBM - 1; bq++) load_m(mr[bq], bq * B);
         ex_q = c.expv[min(q, M - 1)];
         if (c.hasW) dw_q = c.dataW[min(q, M - 1)];
     }
@@ -407,190 +427,207 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
             int32_t rr[U];
 #pragma unroll
             for (int i = 0; i < U; i++) rr[i] = min(r0 + tid + NT * i, H - 1);
+            float Gr[U];
             float sv[U], pf[U], ex[U], dw[U];
-            // the pair-type sums and per-bin constants, shared by the proposals
-            int32_t bv[TMAX][U];
 #pragma unroll
-            for (int t = 0; t < TMAX; t++) {
-                const int32_t tc = min(t, T - 1);
+            for (int i = 0; i < U; i++) Gr[i] = 0.0f;
+            if (T <= TMAX) {
+                // every type's counts in one round of loads (types past T
+                // reload the last one and are not summed)
+                int32_t bv[TMAX][U], dv[TMAX][U];
 #pragma unroll
-                for (int i = 0; i < U; i++) bv[t][i] = base[tc * H + rr[i]];
-            }
-#pragma unroll
-            for (int i = 0; i < U; i++) {
-                sv[i] = c.shellV[rr[i]];
-                pf[i] = c.pref[rr[i]];
-                ex[i] = expv[rr[i]];
-                dw[i] = dataW[rr[i]];
-            }
-            for (int32_t j = 0; j < np; j++) {
-                const int32_t *dn = dn0 + (int64_t)j * V.dnStride;
-                float Gr[U];
-#pragma unroll
-                for (int i = 0; i < U; i++) Gr[i] = 0.0f;
-                if (T <= TMAX) {
-                    // every type's deltas in one round of loads (types past T
-                    // reload the last one and are not summed)
-                    int32_t dv[TMAX][U];
-#pragma unroll
-                    for (int t = 0; t < TMAX; t++) {
-                        const int32_t tc = min(t, T - 1);
-#pragma unroll
-                        for (int i = 0; i < U; i++) dv[t][i] = dn[tc * H + rr[i]];
-                    }
-                    // pair types in order: Gr += wij*nij/Dij with nij integer-valued (exact in float32)
-#pragma unroll
-                    for (int t = 0; t < TMAX; t++) {
-                        if (t >= T) break;
-                        const float wt = c.w[t], Dt = c.D[t];
-#pragma unroll
-                        for (int i = 0; i < U; i++) {
-                            const float nij = (float)(bv[t][i] + dv[t][i]);
-                            float term = wt * nij;
-                            term = term / Dt;
-                            Gr[i] = Gr[i] + term;
-                        }
-                    }
-                } else {
-                    int32_t b1[U], d1[U];
+                for (int t = 0; t < TMAX; t++) {
+                    const int32_t tc = min(t, T - 1);
 #pragma unroll
                     for (int i = 0; i < U; i++) {
-                        b1[i] = base[rr[i]];
-                        d1[i] = dn[rr[i]];
-                    }
-                    // pair types in order; the next type's counts load while this one is summed
-                    for (int32_t t = 0; t < T; t++) {
-                        const int32_t tn = min(t + 1, T - 1);
-                        int32_t nb[U], nd[U];
-#pragma unroll
-                        for (int i = 0; i < U; i++) {
-                            nb[i] = base[tn * H + rr[i]];
-                            nd[i] = dn[tn * H + rr[i]];
-                        }
-                        const float wt = c.w[t], Dt = c.D[t];
-#pragma unroll
-                        for (int i = 0; i < U; i++) {
-                            // Gr += wij*nij/Dij with nij integer-valued (exact in float32)
-                            const float nij = (float)(b1[i] + d1[i]);
-                            float term = wt * nij;
-                            term = term / Dt;
-                            Gr[i] = Gr[i] + term;
-                            b1[i] = nb[i];
-                            d1[i] = nd[i];
-                        }
+                        bv[t][i] = base[tc * H + rr[i]];
+                        dv[t][i] = dn[tc * H + rr[i]];
                     }
                 }
 #pragma unroll
                 for (int i = 0; i < U; i++) {
-                    float g = Gr[i] / sv[i];           // Gr /= shellVolumes
-                    float G = pf[i] * (g - 1.0f);      // (4*PI*r*rho0)*(Gr-1)
-                    float out = G;
-                    if (rsp) {
-                        float model;
-                        if (c.kind == FRMC_PCF) {
-                            // g(r); a scale factor acts on G(r): g = 1 + (sf*G)/(4*PI*r*rho0)
-                            model = g;
-                            if (c.hasSF) {
-                                const float Gsf = G * c.sf;
-                                model = 1.0f + Gsf / pf[i];
-                            }
-                        } else {
-                            model = c.hasSF ? G * c.sf : G;
-                        }
-                        const float diff = ex[i] - model;
-                        float v = diff * diff;
-                        if (c.hasW) v = dw[i] * v;
-                        out = v;
-                    }
-                    if (r0 + tid + NT * i < H) Gs[j * Hp + r0 + tid + NT * i] = out;
+                    sv[i] = c.shellV[rr[i]];
+                    pf[i] = c.pref[rr[i]];
+                    ex[i] = expv[rr[i]];
+                    dw[i] = dataW[rr[i]];
                 }
+                // pair types in order: Gr += wij*nij/Dij with nij integer-valued (exact in float32)
+#pragma unroll
+                for (int t = 0; t < TMAX; t++) {
+                    if (t >= T) break;
+                    const float wt = c.w[t], Dt = c.D[t];
+#pragma unroll
+                    for (int i = 0; i < U; i++) {
+                        const float nij = (float)(bv[t][i] + dv[t][i]);
+                        float term = wt * nij;
+                        term = term / Dt;
+                        Gr[i] = Gr[i] + term;
+                    }
+                }
+            } else {
+                int32_t bv[U], dv[U];
+#pragma unroll
+                for (int i = 0; i < U; i++) {
+                    bv[i] = base[rr[i]];
+                    dv[i] = dn[rr[i]];
+                }
+                // the per-bin constants load with the first counts
+#pragma unroll
+                for (int i = 0; i < U; i++) {
+                    sv[i] = c.shellV[rr[i]];
+                    pf[i] = c.pref[rr[i]];
+                    ex[i] = expv[rr[i]];
+                    dw[i] = dataW[rr[i]];
+                }
+                // pair types in order; the next type's counts load while this one is summed
+                for (int32_t t = 0; t < T; t++) {
+                    const int32_t tn = min(t + 1, T - 1);
+                    int32_t nb[U], nd[U];
+#pragma unroll
+                    for (int i = 0; i < U; i++) {
+                        nb[i] = base[tn * H + rr[i]];
+                        nd[i] = dn[tn * H + rr[i]];
+                    }
+                    const float wt = c.w[t], Dt = c.D[t];
+#pragma unroll
+                    for (int i = 0; i < U; i++) {
+                        // Gr += wij*nij/Dij with nij integer-valued (exact in float32)
+                        const float nij = (float)(bv[i] + dv[i]);
+                        float term = wt * nij;
+                        term = term / Dt;
+                        Gr[i] = Gr[i] + term;
+                        bv[i] = nb[i];
+                        dv[i] = nd[i];
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < U; i++) {
+                float g = Gr[i] / sv[i];           // Gr /= shellVolumes
+                float G = pf[i] * (g - 1.0f);      // (4*PI*r*rho0)*(Gr-1)
+                float out = G;
+                if (rsp) {
+                    float model;
+                    if (c.kind == FRMC_PCF) {
+                        // g(r); a scale factor acts on G(r): g = 1 + (sf*G)/(4*PI*r*rho0)
+                        model = g;
+                        if (c.hasSF) {
+                            const float Gsf = G * c.sf;
+                            model = 1.0f + Gsf / pf[i];
+                        }
+                    } else {
+                        model = c.hasSF ? G * c.sf : G;
+                    }
+                    const float diff = ex[i] - model;
+                    float v = diff * diff;
+                    if (c.hasW) v = dw[i] * v;
+                    out = v;
+                }
+                if (r0 + tid + NT * i < H) Gs[r0 + tid + NT * i] = out;
             }
         }
     }
     __syncthreads();
     if (rsp) {
-        for (int32_t j = 0; j < np; j++) {
-            const float chi = block_pairwise(Gs + j * Hp, c, scratch, lprog);
-            const int32_t p = p0 + j;
-            if (tid == 0) V.chi2d[(int64_t)p * V.C + ci] = chi;
-            if (F) {
-                // np.sum([chi]) from 0, as k_eval_b sums it
-                if (tid == 0) {
-                    const float tse = 0.0f + chi;
-                    publish(V.out_res + (int64_t)p * 2 + 1, chi, F->seq);
-                    V.tse_dev[p] = tse;
-                    publish(V.out_res + (int64_t)p * 2, tse, F->seq);
-                }
-                for (int64_t s = (int64_t)p * blockDim.x + tid; s < F->zero_len; s += (int64_t)F->nb * blockDim.x)
-                    F->dn_zero[s] = 0;
+        const float chi = block_pairwise(Gs, c, scratch, lprog);
+        if (tid == 0) {
+            V.chi2d[(int64_t)p * V.C + ci] = chi;
+            if (V.leafmode) publish(V.out_res + (int64_t)p * V.resW + 1 + ci, chi, V.seq);
+        }
+        if (F) {
+            // np.sum([chi]) from 0, as k_eval_b sums it
+            if (tid == 0) {
+                const float tse = 0.0f + chi;
+                publish(V.out_res + (int64_t)p * V.resW + 1, chi, F->seq);
+                V.tse_dev[p] = tse;
+                publish(V.out_res + (int64_t)p * V.resW, tse, F->seq);
             }
-            __syncthreads();  // thread 0's combine read scratch; the next proposal rewrites it
+            for (int64_t s = (int64_t)p * blockDim.x + tid; s < F->zero_len; s += (int64_t)F->nb * blockDim.x)
+                F->dn_zero[s] = 0;
         }
         return;
     }
-    if (w >= np) return;  // the S(Q) sum: wave w for proposal p0 + w, one Q per lane
-    const int32_t p = p0 + w;
-    Gs += w * Hp;
-    // The S(Q) sum starts from +0, as NumPy's add reduction does (a sum of
-    // -0 products is +0).
-    float acc = 0.0f;
-    // two register batches (matrix column and G rows) in turn: one batch loads
-    // while the other's chain runs, which hides both the HBM/L2 and the LDS latency
-    auto load_g = [&](float *gg, int32_t r0) {
-        const float *gsrc = Gs + min(r0, Hp - B > 0 ? Hp - B : 0);
+    if (chain_wave) {
+        // The S(Q) sum starts from +0, as NumPy's add reduction does (a sum of
+        // -0 products is +0).
+        float acc = 0.0f;
+        // two register batches (matrix column and G rows) in turn: one batch loads
+        // while the other's chain runs, which hides both the HBM/L2 and the LDS latency
+        auto load_g = [&](float *gg, int32_t r0) {
+            const float *gsrc = Gs + min(r0, Hp - B > 0 ? Hp - B : 0);
 #pragma unroll
-        for (int i4 = 0; i4 < B / 4; i4++) {
-            const float4 gv = *reinterpret_cast<const float4 *>(gsrc + 4 * i4);
-            gg[4 * i4] = gv.x; gg[4 * i4 + 1] = gv.y; gg[4 * i4 + 2] = gv.z; gg[4 * i4 + 3] = gv.w;
-        }
-    };
-    auto chain = [&](const float *m, const float *gg, int32_t r0) {
-        if (r0 + B <= H) {
-            // all products first: the adds then run back to back, each waiting
-            // only on the previous add
+            for (int i4 = 0; i4 < B / 4; i4++) {
+                const float4 gv = *reinterpret_cast<const float4 *>(gsrc + 4 * i4);
+                gg[4 * i4] = gv.x; gg[4 * i4 + 1] = gv.y; gg[4 * i4 + 2] = gv.z; gg[4 * i4 + 3] = gv.w;
+            }
+        };
+        // all products first: the adds then run back to back, each waiting
+        // only on the previous add
+        auto chain_full = [&](const float *m, const float *g) {
             float pr[B];
 #pragma unroll
-            for (int i = 0; i < B; i++) pr[i] = gg[i] * m[i];
+            for (int i = 0; i < B; i++) pr[i] = g[i] * m[i];
             __builtin_amdgcn_sched_barrier(0);  // keep the scheduler from interleaving the two loops
 #pragma unroll
             for (int i = 0; i < B; i++) acc = acc + pr[i];
-        } else {
-            const float *g = Gs + r0;
-            for (int i = 0; i < H - r0; i++) {
-                const float prod = g[i] * m[i];
-                acc = acc + prod;
+        };
+        // a batch that may end past H: rows past H are not added
+        auto chain = [&](const float *m, const float *g, int32_t r0) {
+            float pr[B];
+#pragma unroll
+            for (int i = 0; i < B; i++) pr[i] = g[i] * m[i];
+#pragma unroll
+            for (int i = 0; i < B; i++)
+                if (r0 + i < H) acc = acc + pr[i];
+        };
+        load_g(gg[0], 0);
+        // whole rounds of NBM batches without branches, so that the loads of
+        // the next NBM - 1 batches stay in flight across the adds; ring slot
+        // j holds batch bi + j
+        const int32_t nb = (H + B - 1) / B;  // batches, the last one possibly partial
+        int32_t bi = 0;
+        for (; bi + NBM <= nb - 1; bi += NBM) {
+#pragma unroll
+            for (int j = 0; j < NBM; j++) {
+                load_m(mr[(j + NBM - 1) % NBM], (bi + j + NBM - 1) * B);
+                load_g(gg[(j + 1) & 1], (bi + j + 1) * B);
+                chain_full(mr[j], gg[j & 1]);
             }
         }
-    };
-    load_g(ga, 0);
-    for (int32_t r0 = 0; r0 < H; r0 += 2 * B) {
-        load_m(mb, r0 + B);
-        load_g(gb, r0 + B);
-        chain(ma, ga, r0);
-        if (r0 + B >= H) break;
-        load_m(ma, r0 + 2 * B);
-        load_g(ga, r0 + 2 * B);
-        chain(mb, gb, r0 + B);
-    }
-    if (qok) {
-        float S;
-        if (c.kind == FRMC_RSFC) {
-            S = c.hasSF ? c.sf * acc : acc;
-        } else {
-            S = acc + 1.0f;
-            if (c.hasSF) {
-                float t = S - 1.0f;
-                t = c.sf * t;
-                S = t + 1.0f;
+        // the last (at most NBM) batches: ring slots 0..NBM-2 hold batches
+        // bi..bi+NBM-2; the one after them loads here
+#pragma unroll
+        for (int j = 0; j < NBM; j++) {
+            if (bi + j < nb) {
+                if (bi + j + NBM - 1 < nb) load_m(mr[(j + NBM - 1) % NBM], (bi + j + NBM - 1) * B);
+                load_g(gg[(j + 1) & 1], (bi + j + 1) * B);
+                chain(mr[j], gg[j & 1], (bi + j) * B);
             }
         }
-        const float diff = ex_q - S;
-        float v = diff * diff;
-        if (c.hasW) v = dw_q * v;
-        V.sqbuf[((int64_t)p * V.C + ci) * V.Mmax + q] = v;
+        if (qok) {
+            float S;
+            if (c.kind == FRMC_RSFC) {
+                S = c.hasSF ? c.sf * acc : acc;
+            } else {
+                S = acc + 1.0f;
+                if (c.hasSF) {
+                    float t = S - 1.0f;
+                    t = c.sf * t;
+                    S = t + 1.0f;
+                }
+            }
+            const float diff = ex_q - S;
+            float v = diff * diff;
+            if (c.hasW) v = dw_q * v;
+            if (V.leafmode) scratch[64 * w + lane] = v;
+            else V.sqbuf[((int64_t)p * V.C + ci) * V.Mmax + q] = v;
+        }
     }
+    if (!V.leafmode) return;
+    // the leaf's sum in NumPy's order (the host combines the leaves)
+    __syncthreads();
+    if (tid == 0) publish(V.out_res + (int64_t)p * V.resW + c.leafOff + leaf, leaf_sum(scratch, len), V.seq);
 }
-
 
 // --------------------------------------------------------------- k_eval_b
 
@@ -632,12 +669,12 @@ __device__ __forceinline__ void eval_b_body(const View &V, int32_t *dn_zero, int
         // than 8 terms are summed sequentially from 0
         if (threadIdx.x == 0) {
             tse = tse + chi;
-            publish(V.out_res + (int64_t)p * (V.C + 1) + 1 + ci, chi, seq);
+            publish(V.out_res + (int64_t)p * V.resW + 1 + ci, chi, seq);
         }
     }
     if (threadIdx.x == 0) {
         V.tse_dev[p] = tse;
-        publish(V.out_res + (int64_t)p * (V.C + 1), tse, seq);
+        publish(V.out_res + (int64_t)p * V.resW, tse, seq);
     }
 }
 
@@ -649,13 +686,30 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
 
 // --------------------------------------------------------------- k_eval_a
 
-// grid (nunits, ceil(k/EP)): block (u, g) evaluates unit u for the EP
-// proposals g*EP.. (one S(Q) wave each); U bins per thread per G(r) pass
-template <int U, int EP>
+// grid (nunits, k): block (u, p) evaluates unit u for proposal p; U bins
+// per thread per G(r) pass.  XCD: a one-dimensional grid of
+// ceil(nunits/8)*8 blocks per proposal, so that (workgroups going to the
+// XCDs in turn) unit u always runs on XCD u mod 8 and each XCD's L2 holds
+// the matrix columns of its units only; the padding blocks return at once.
+template <int U, bool XCD>
 __global__ void __launch_bounds__(kEvalThreads) k_eval_a(View V, int32_t k) {
     extern __shared__ float sm[];
-    const int32_t p0 = blockIdx.y * EP, np = min(EP, k - p0);
-    eval_a_body<U, EP>(V, V.dn, p0, np, blockIdx.x, sm, nullptr);
+    int32_t u, p;
+    int64_t b, nblk;
+    if (XCD) {
+        const int32_t nu8 = (V.nunits + 7) & ~7;
+        u = blockIdx.x % nu8;
+        p = blockIdx.x / nu8;
+        b = (int64_t)p * V.nunits + u;
+        nblk = (int64_t)V.nunits * k;
+        if (u >= V.nunits) return;
+    } else {
+        u = blockIdx.x;
+        p = blockIdx.y;
+        b = (int64_t)p * gridDim.x + u;
+        nblk = (int64_t)gridDim.x * gridDim.y;
+    }
+    eval_a_body<U>(V, V.dn, p, u, sm, nullptr, b, nblk);
 }
 
 // ----------------------------------------------------------- batched windows
@@ -899,9 +953,9 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_a_batch(BatchArgs A) {
     const int2 m = S.m;
     if (V.C == 1 && V.c[0].rspace) {
         const Fuse F{W.dn_zero, W.zero_len, W.seq, W.k};
-        eval_a_body<4, 1>(V, W.dn, m.y / V.nunits, 1, m.y % V.nunits, sm, &F);
+        eval_a_body<4>(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, &F, 0, 1);
     } else {
-        eval_a_body<4, 1>(V, W.dn, m.y / V.nunits, 1, m.y % V.nunits, sm, nullptr);
+        eval_a_body<4>(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, nullptr, 0, 1);
     }
 }
 
@@ -1034,28 +1088,33 @@ int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hi
     return FRMC_OK;
 }
 
-size_t eval_lds_bytes(const View &V, int32_t np) {
-    return sizeof(float) * ((size_t)np * ((V.Hmax + 3) & ~3) + V.maxLeaves + kStack + (size_t)2 * V.maxLeaves);
+size_t eval_lds_bytes(const View &V) {
+    // the leaf's 128 squared deviations reuse the pairwise scratch
+    return sizeof(float) * ((size_t)((V.Hmax + 15) & ~15) + std::max<size_t>(V.maxLeaves + kStack, 128) +
+                            (size_t)2 * V.maxLeaves);
 }
 
-template <int U, int EP>
+template <int U, bool XCD>
 static void launch_k_eval_a(const View &V, int32_t k, hipStream_t st) {
-    const size_t shm = eval_lds_bytes(V, EP);
+    const size_t shm = eval_lds_bytes(V);
     if (shm > 48 * 1024)
-        (void)hipFuncSetAttribute((const void *)k_eval_a<U, EP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void *)k_eval_a<U, XCD>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shm);
-    k_eval_a<U, EP><<<dim3((unsigned)V.nunits, (unsigned)((k + EP - 1) / EP)), kEvalThreads, shm, st>>>(V, k);
+    if (XCD)
+        k_eval_a<U, XCD><<<dim3((unsigned)((V.nunits + 7) & ~7) * (unsigned)k), kEvalThreads, shm, st>>>(V, k);
+    else
+        k_eval_a<U, XCD><<<dim3((unsigned)V.nunits, (unsigned)k), kEvalThreads, shm, st>>>(V, k);
 }
 
-// shape s of k_eval_a: bins per thread per G(r) pass U = s / 4 ? 8 : 4,
-// proposals per block EP = 1 << (s % 4) (1, 2 or 4); every shape gives the
-// same results
+// shape s of k_eval_a: bins per thread per G(r) pass U = 8 if s & 4 (and the
+// grid has more than 1,024 bins) else 4; s & 8: XCD-aligned units.  Every
+// shape gives the same results.
 int launch_eval(const View &V, int32_t k, hipStream_t st, int32_t shape) {
-    const bool u8 = shape >= 4 && V.Hmax > 4 * kEvalThreads;
-    switch (shape % 4) {
-        case 0: if (u8) launch_k_eval_a<8, 1>(V, k, st); else launch_k_eval_a<4, 1>(V, k, st); break;
-        case 1: if (u8) launch_k_eval_a<8, 2>(V, k, st); else launch_k_eval_a<4, 2>(V, k, st); break;
-        default: if (u8) launch_k_eval_a<8, 4>(V, k, st); else launch_k_eval_a<4, 4>(V, k, st); break;
+    const bool u8 = (shape & 4) && V.Hmax > 4 * kEvalThreads;
+    if (shape & 8) {
+        if (u8) launch_k_eval_a<8, true>(V, k, st); else launch_k_eval_a<4, true>(V, k, st);
+    } else {
+        if (u8) launch_k_eval_a<8, false>(V, k, st); else launch_k_eval_a<4, false>(V, k, st);
     }
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;

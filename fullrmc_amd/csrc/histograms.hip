@@ -736,6 +736,8 @@ static bool g_forced_on = false;
 static thread_local FullChoice g_last;     // frmc_last_full_schedule (per thread)
 static thread_local int64_t g_last_blocks = 0;
 static thread_local int g_last_tuned = 0;
+static thread_local int64_t g_last_evaluated = -1;  // pairs evaluated by the last call (-1: not measured)
+static thread_local int64_t g_cells_evaluated = -1;  // set by run_cells when timing is on
 static std::mutex g_tune_mu;
 // shape class -> tuned schedule.  Class: device, floor(log2 n), bins
 // (2*nel*nel*H), boundary kind (0 none, 1 triclinic, 2 orthorhombic), EDGE.
@@ -849,6 +851,32 @@ static int run_cells(const FullChoice &C, const float *coords, int32_t n, const 
     k_cell_scan<<<1, 1024, 0, st>>>(cnt, G.ncell, cstart, fill);
     k_cell_scatter<<<blocks_for(n, 256), 256, 0, st>>>(coords, n, mol, elem, cid, fill, sx, se, sid);
     hipError_t le = hipGetLastError();
+    g_cells_evaluated = -1;
+    if (le == hipSuccess && timing_enabled()) {
+        // pairs the items visit: n_c (n_c - 1) / 2 within a cell, n_c n_c' for
+        // each of its 13 visited neighbours (measurement only: synchronises)
+        std::vector<int32_t> hs(nc + 1);
+        le = hipMemcpyAsync(hs.data(), cstart, sizeof(int32_t) * (nc + 1), hipMemcpyDeviceToHost, st);
+        if (le == hipSuccess) le = hipStreamSynchronize(st);
+        if (le == hipSuccess) {
+            int64_t ev = 0;
+            for (int32_t x = 0; x < m[0]; x++)
+                for (int32_t y = 0; y < m[1]; y++)
+                    for (int32_t z = 0; z < m[2]; z++) {
+                        const int64_t c = ((int64_t)x * m[1] + y) * m[2] + z;
+                        const int64_t nc0 = hs[c + 1] - hs[c];
+                        ev += nc0 * (nc0 - 1) / 2;
+                        for (int32_t o = 1; o < 14; o++) {
+                            const int32_t dx = o <= 9 ? 1 : 0, dy = o <= 9 ? (o - 1) / 3 - 1 : (o <= 12 ? 1 : 0),
+                                          dz = o <= 9 ? (o - 1) % 3 - 1 : (o <= 12 ? o - 11 : 1);
+                            const int64_t c2 = ((int64_t)((x + dx + m[0]) % m[0]) * m[1] + (y + dy + m[1]) % m[1]) *
+                                                   m[2] + (z + dz + m[2]) % m[2];
+                            ev += nc0 * (hs[c2 + 1] - hs[c2]);
+                        }
+                    }
+            g_cells_evaluated = ev;
+        }
+    }
     Edge64 none;
     memset(&none, 0, sizeof(none));
     const Edge64 &EE = E ? *E : none;
@@ -1071,6 +1099,7 @@ static int launch_full_timed(const float *coords, int32_t n, const Basis &B, int
     g_last = C;
     g_last_blocks = blocks;
     g_last_tuned = tuned;
+    g_last_evaluated = C.cells ? g_cells_evaluated : (int64_t)n * (n - 1) / 2;
     return rc;
 }
 
@@ -1321,6 +1350,12 @@ int frmc_force_full_schedule(int32_t threads, int32_t rowAtoms, int32_t workgrou
     g_forced.R = g_forced.cells ? 1024 / threads : rowAtoms;
     g_forced.per_cu = workgroupsPerCU;
     g_forced_on = true;
+    return FRMC_OK;
+}
+
+int frmc_last_full_evaluated(int64_t *pairs) {
+    FRMC_REQUIRE(pairs, "NULL pointer argument");
+    *pairs = g_last_evaluated;
     return FRMC_OK;
 }
 

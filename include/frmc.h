@@ -68,6 +68,10 @@ float frmc_last_kernel_ms(void);
 void frmc_set_autotune(int enable);
 int frmc_force_full_schedule(int32_t threads, int32_t rowAtoms, int32_t workgroupsPerCU);
 int frmc_last_full_schedule(int32_t *threads, int32_t *rowAtoms, int64_t *workgroups, int32_t *tuned);
+/* Pairs the last all-pairs call of this thread evaluated: N(N-1)/2, or for
+   the cell-list kernel the pairs in visited cell pairs (counted only while
+   frmc_set_timing is on; -1 otherwise).  Measurement only. */
+int frmc_last_full_evaluated(int64_t *pairs);
 
 /* ------------------------------------------------------------------------
  * pairs_distances  (Extensions/pairs_distances.pyx)
@@ -454,15 +458,17 @@ int frmc_engine_kernel_times(frmc_engine *e, double *ms_out, int64_t *launches_o
    chi-square partials), so the engine state is unchanged.  No reference
    counterpart (measurement only). */
 int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out);
-/* k_eval_a shape tuning.  No reference counterpart (the reference evaluates
-   one move at a time).  A shape is (bins per thread per G(r) pass: 4, or 8
-   when above 1,024 bins; proposals per block, one S(Q) wave each: 1, 2 or
-   4), numbered 0, 1, 2 (4 bins) and 4, 5, 6 (8 bins); every shape gives the
-   same results.  An engine times every shape for each window-size bucket
-   ceil(log2 k) on its first window and keeps the fastest (cached per
-   device, work units, bins and pair types).  frmc_set_eval_shape forces one
-   shape for every engine (-1: tuned); frmc_engine_eval_shapes reports the
-   engine's shape for buckets 0..n-1 (tuning first if needed). */
+/* Window-evaluation shape tuning.  No reference counterpart (the reference
+   evaluates one move at a time).  A shape s is a multiple of 4 in 0..28:
+   bins per thread per G(r) pass 8 (s & 4, above 1,024 bins) or 4; work
+   units aligned to the XCDs (s & 8: unit u on XCD u mod 8); S(Q) units of
+   one pairwise-sum leaf whose sums the host combines (s & 16, no k_eval_b)
+   or of 64 Q points.  Every shape gives the same results.  An engine times
+   every shape for each window-size bucket ceil(log2 k) on its first window
+   and keeps the fastest (cached per device, constraint layout, pair types
+   and largest window).  frmc_set_eval_shape forces one shape for every
+   engine (-1: tuned); frmc_engine_eval_shapes reports the engine's shape
+   for buckets 0..n-1 (tuning first if needed). */
 int frmc_set_eval_shape(int32_t shape);
 int frmc_engine_eval_shapes(frmc_engine *e, int32_t *shapes_out, int32_t n);
 /* window size of frmc_engine_run_translation: adaptive (default; k is the
