@@ -136,6 +136,8 @@ _PROTOS = {
     "frmc_engine_time_eval": [c_vp, c_i32, c_i32, c_f32p],
     "frmc_set_eval_shape": [c_i32],
     "frmc_engine_eval_shapes": [c_vp, c_i32p, c_i32],
+    "frmc_set_delta_cells": [c_i32],
+    "frmc_engine_delta_cells": [c_vp, c_i32p, c_f32p],
     "frmc_engine_kernel_times": [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64)],
     "frmc_engine_set_adaptive": [c_vp, c_int],
 }

@@ -368,6 +368,7 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
     int64_t maxd = 0, maxa = 0;
     for (int32_t r = 0; r < R; r++) {
         use_unit_layout(runs[r].e, false);  // the batched kernels take 64-Q units and k_eval_b_batch
+        drop_cells(runs[r].e);              // k_delta_batch walks all atoms and does not keep the sorted copy
         hv[r] = make_view(runs[r].e, 0);
         if (kBP * hv[r].dnStride <= kBatchLdsInts)
             shm_d = std::max(shm_d, sizeof(int32_t) * kBP * hv[r].dnStride);

@@ -92,6 +92,17 @@ struct View {
     float *tse_dev;                // [kmax] totals in device memory (read by k_commit)
     unsigned long long *out_commit;  // [2] host-mapped: (seq << 32 | proposal committed by k_commit)
     uint32_t seq;
+    // Spatially sorted copy of the atoms (periodic boxes): chunks of kChunkAtoms
+    // atoms that lie close together, and each chunk's box in wrapped
+    // fractional coordinates; k_delta skips a chunk that no pair of a
+    // proposal can reach.  sx null: no sorted copy to read or maintain.
+    float4 *sx;                    // [nchunk*kChunkAtoms] (x, y, z box coordinates, bits of id*64+element; -1 pad)
+    float *cbox;                   // [nchunk][8] lo x,y,z, hi x,y,z (wrapped fractional), 2 unused
+    const int32_t *spos;           // [n] position of each atom in sx
+    int32_t nchunk;
+    int32_t prune;                 // 1: this window's k_delta walks the chunks (single-atom proposals)
+    float hsp[3];                  // lattice plane spacings (A) along the three box axes
+    float wcut;                    // largest binned distance plus a margin for float rounding (A)
 };
 
 // Windows of single-atom moves travel to k_delta in the kernel arguments
@@ -114,9 +125,15 @@ struct Inline {
 struct Pending {
     int32_t valid;
     int32_t atom, el;
+    int32_t slot;  // the atom's position in the sorted copy (-1: none)
     float old[3], nw[3];
     const int32_t *dn;
 };
+
+// atoms per chunk of the sorted copy (one per thread of a k_delta block) and
+// proposals per block of the chunked k_delta
+constexpr int32_t kChunkAtoms = 256;
+constexpr int32_t kPruneP = 32;
 
 // Which proposal of a window k_commit commits: p >= 0 a given one (the
 // caller decided), p < 0 the first proposal that the reference's rule accepts
@@ -215,6 +232,17 @@ struct frmc_engine {
     bool device_decided = false;     // the last window's commit was queued on the device
     bool deferred = false;           // the last window's commit is left to the next k_delta
     frmc::Pending pend{};            // accepted move not yet applied on the device
+    // spatially sorted copy of the atoms for k_delta's chunk walk
+    float4 *sx = nullptr;
+    float *cbox = nullptr;
+    int32_t *spos = nullptr;
+    int32_t nchunk = 0;
+    bool cells = false;              // the engine keeps a sorted copy
+    bool cells_stale = false;        // the copy missed commits (batched windows): rebuild before use
+    int64_t commits_since_cells = 0; // accepted moves since the chunk boxes were last made tight
+    std::vector<int32_t> h_spos;
+    float hsp[3] = {0, 0, 0};
+    float wcut = 0.f;
     const int32_t *win_dn = nullptr; // delta buffer of the last launched window
     int32_t pending_check = -2;      // decision of the last device-decided window (-2: none)
     uint32_t pending_seq = 0;
@@ -256,6 +284,7 @@ int batch_prepare(frmc_engine *e, int32_t k, const int32_t *group_ids, const flo
                   Prop *props);
 int settle_window(frmc_engine *e, int32_t p);
 int finish_windows(frmc_engine *e);
+void drop_cells(frmc_engine *e);  // the sorted copy is not maintained from here on: rebuilt before its next use
 void collect_times(frmc_engine *e);
 
 // bracket one kernel launch with events when profiling is on

@@ -16,6 +16,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -62,8 +63,17 @@ View make_view(frmc_engine *e, int s) {
     V.tse_dev = e->tse_dev;
     V.out_commit = e->h_commit;
     V.seq = e->seq;
+    const bool sorted = e->cells && !e->cells_stale;
+    V.sx = sorted ? e->sx : nullptr;
+    V.cbox = e->cbox;
+    V.spos = e->spos;
+    V.nchunk = e->nchunk;
+    V.prune = sorted ? 1 : 0;
+    for (int d = 0; d < 3; d++) V.hsp[d] = e->hsp[d];
+    V.wcut = e->wcut;
     return V;
 }
+
 
 static hipEvent_t take_event(frmc_engine *e) {
     if (!e->event_pool.empty()) {
@@ -135,6 +145,133 @@ static hipError_t halloc(frmc_engine *e, T **p, size_t count) {
         e->host_allocs.push_back((void *)*p);
     }
     return err;
+}
+
+// ------------------------------------------------------- sorted atom copy
+
+// frmc_set_delta_cells: -1 chunk walk for periodic single-atom engines of at
+// least kCellsMinAtoms atoms, 0 never, 1 for every periodic single-atom engine
+static int g_cells_mode = -1;
+constexpr int32_t kCellsMinAtoms = 4096;
+// box coordinates beyond this magnitude lose the float32 resolution the
+// chunk test's margin assumes (the sorted copy is then not used)
+constexpr float kCellsMaxCoord = 64.0f;
+
+// Sort the atoms by a grid of cells of about kChunkAtoms atoms each (cell
+// index z-major over the wrapped fractional coordinates), so that a chunk of
+// kChunkAtoms consecutive atoms spans one or two cells; upload the copy, the
+// chunk boxes and each atom's position.  The stream is drained first (the
+// kernels of earlier windows read the copy).  A coordinate beyond
+// kCellsMaxCoord turns the copy off for good.
+static float tag_float(int32_t t) {
+    float f;
+    memcpy(&f, &t, sizeof f);
+    return f;
+}
+
+static int build_cells(frmc_engine *e) {
+    const int32_t n = e->n;
+    for (size_t q = 0; q < e->h_box.size(); q++)
+        if (!(fabsf(e->h_box[q]) <= kCellsMaxCoord)) {
+            e->cells = false;
+            return FRMC_OK;
+        }
+    FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
+    // lattice plane spacings h_i = V / |a_j x a_k| (basis rows a_i)
+    double a[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) a[r][c] = e->B.b[3 * r + c];
+    auto cross = [](const double *u, const double *v, double *w) {
+        w[0] = u[1] * v[2] - u[2] * v[1];
+        w[1] = u[2] * v[0] - u[0] * v[2];
+        w[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    double cx[3][3];
+    cross(a[1], a[2], cx[0]);
+    cross(a[2], a[0], cx[1]);
+    cross(a[0], a[1], cx[2]);
+    const double vol = fabs(a[0][0] * cx[0][0] + a[0][1] * cx[0][1] + a[0][2] * cx[0][2]);
+    const double L = cbrt((double)kChunkAtoms * vol / n);  // edge of a cell of about kChunkAtoms atoms
+    int32_t nc[3];
+    double hmax = 0.0;
+    for (int i = 0; i < 3; i++) {
+        const double h = vol / sqrt(cx[i][0] * cx[i][0] + cx[i][1] * cx[i][1] + cx[i][2] * cx[i][2]);
+        e->hsp[i] = (float)h;
+        hmax = std::max(hmax, h);
+        nc[i] = (int32_t)std::min(256.0, std::max(1.0, std::floor(h / L + 0.5)));
+    }
+    float maxD = 0.0f;
+    for (int ci = 0; ci < e->C; ci++) maxD = std::max(maxD, e->c[ci].g.maxD);
+    e->wcut = (float)(maxD * (1.0 + 1e-5) + 0.01 + 1e-4 * hmax);
+    // counting sort by cell
+    const int64_t ncell = (int64_t)nc[0] * nc[1] * nc[2];
+    std::vector<int32_t> cell(n), start(ncell + 1, 0);
+    std::vector<float> wf(3 * (size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        int32_t ci[3];
+        for (int d = 0; d < 3; d++) {
+            const float x = e->h_box[3 * (size_t)i + d];
+            const float f = x - std::floor(x);
+            wf[3 * (size_t)i + d] = f;
+            ci[d] = std::min((int32_t)(f * nc[d]), nc[d] - 1);
+        }
+        cell[i] = (ci[2] * nc[1] + ci[1]) * nc[0] + ci[0];
+        start[cell[i] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; c++) start[c + 1] += start[c];
+    const int32_t nchunk = (n + kChunkAtoms - 1) / kChunkAtoms;
+    std::vector<float4> sx((size_t)nchunk * kChunkAtoms, make_float4(0.f, 0.f, 0.f, tag_float(-1)));
+    std::vector<float> cbox((size_t)nchunk * 8, 0.0f);
+    e->h_spos.assign(n, 0);
+    for (int32_t i = 0; i < n; i++) {
+        const int32_t pos = start[cell[i]]++;
+        e->h_spos[i] = pos;
+        const float *x = &e->h_box[3 * (size_t)i];
+        sx[pos] = make_float4(x[0], x[1], x[2], tag_float(i * 64 + e->h_elem[i]));
+    }
+    for (int32_t c = 0; c < nchunk; c++) {
+        float *bx = &cbox[8 * (size_t)c];
+        for (int d = 0; d < 3; d++) {
+            bx[d] = 2.0f;
+            bx[3 + d] = -1.0f;
+        }
+    }
+    for (int32_t i = 0; i < n; i++) {
+        float *bx = &cbox[8 * (size_t)(e->h_spos[i] / kChunkAtoms)];
+        for (int d = 0; d < 3; d++) {
+            bx[d] = std::min(bx[d], wf[3 * (size_t)i + d]);
+            bx[3 + d] = std::max(bx[3 + d], wf[3 * (size_t)i + d]);
+        }
+    }
+    if (!e->sx) {
+        FRMC_CHECK_HIP(dalloc(e, &e->sx, sx.size()));
+        FRMC_CHECK_HIP(dalloc(e, &e->cbox, cbox.size()));
+        FRMC_CHECK_HIP(dalloc(e, &e->spos, (size_t)n));
+    }
+    e->nchunk = nchunk;
+    FRMC_CHECK_HIP(hipMemcpyAsync(e->sx, sx.data(), sizeof(float4) * sx.size(), hipMemcpyHostToDevice, e->st));
+    FRMC_CHECK_HIP(hipMemcpyAsync(e->cbox, cbox.data(), sizeof(float) * cbox.size(), hipMemcpyHostToDevice, e->st));
+    FRMC_CHECK_HIP(
+        hipMemcpyAsync(e->spos, e->h_spos.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, e->st));
+    FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
+    e->cells_stale = false;
+    e->commits_since_cells = 0;
+    if (e->pend.valid) e->pend.slot = e->h_spos[e->pend.atom];
+    return FRMC_OK;
+}
+
+// chunk boxes only grow as atoms move: made tight again after this many
+// accepted moves
+static int64_t cells_refresh_after(const frmc_engine *e) { return std::max<int64_t>(64, e->n / 4); }
+
+static int update_cells(frmc_engine *e) {
+    if (!e->cells) return FRMC_OK;
+    if (e->cells_stale || e->commits_since_cells > cells_refresh_after(e)) return build_cells(e);
+    return FRMC_OK;
+}
+
+void drop_cells(frmc_engine *e) {
+    if (e->cells) e->cells_stale = true;
 }
 
 static void free_engine(frmc_engine *e) {
@@ -489,6 +626,7 @@ static int stage_host(frmc_engine *e, int32_t k, const int32_t *group_ids, const
 
 static int stage_and_launch(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box,
                             const float *u = nullptr, float tol = 0.0f) {
+    if (int rc0 = update_cells(e)) return rc0;
     int s = 0;
     if (int rc0 = stage_host(e, k, group_ids, new_box, s)) return rc0;
     Stage &S = e->stage[s];
@@ -632,8 +770,13 @@ static void note_commit(frmc_engine *e, int32_t p) {
     const int32_t g0 = e->h_goff[gi], g = e->h_goff[gi + 1] - g0;
     for (int32_t a = 0; a < g; a++) {
         const size_t id = (size_t)e->h_gatoms[g0 + a];
-        for (int d = 0; d < 3; d++) e->h_box[3 * id + d] = S.pnew[3 * (size_t)(a0 + a) + d];
+        for (int d = 0; d < 3; d++) {
+            const float x = S.pnew[3 * (size_t)(a0 + a) + d];
+            e->h_box[3 * id + d] = x;
+            if (!(fabsf(x) <= kCellsMaxCoord)) e->cells_stale = true;  // the rebuild turns the copy off
+        }
     }
+    e->commits_since_cells++;
     e->cur_tse = e->h_tse[p];
     for (int ci = 0; ci < e->C; ci++) e->cur_chi2[ci] = e->h_chi2[(int64_t)p * e->C + ci];
     e->last_k = 0;  // the window now describes a stale state
@@ -684,6 +827,7 @@ int settle_window(frmc_engine *e, int32_t p) {
             e->pend.nw[d] = S.pnew[3 * (size_t)a0 + d];
         }
         e->pend.dn = e->win_dn + (int64_t)p * e->dnStride;
+        e->pend.slot = e->cells ? e->h_spos[id] : -1;
         note_commit(e, p);
         return FRMC_OK;
     }
@@ -909,6 +1053,15 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
         ECHK(dalloc(e, &e->tse_dev, K));
         ECHK(halloc(e, &e->h_commit, 2));
         e->h_commit[0] = e->h_commit[1] = 0xffffffffull;
+        e->pend.slot = -1;
+        // the sorted copy for k_delta's chunk walk: periodic boxes whose groups
+        // are single atoms (atom index and element share one 32-bit tag)
+        e->cells = e->isPBC && e->gmax == 1 && d->n < (1 << 24) && g_cells_mode != 0 &&
+                   (g_cells_mode == 1 || d->n >= kCellsMinAtoms);
+        if (e->cells) {
+            rc = build_cells(e);
+            if (rc) goto fail;
+        }
         ECHK(hipDeviceSynchronize());
     }
 #undef ECHK
@@ -1109,6 +1262,19 @@ int frmc_set_eval_shape(int32_t shape) {
     FRMC_REQUIRE(shape >= -1 && shape <= 28 && (shape == -1 || shape % 4 == 0),
                  "eval shape must be -1 (tuned) or one of 0, 4, 8, ..., 28 (got %d)", shape);
     g_eval_forced = shape;
+    return FRMC_OK;
+}
+
+int frmc_set_delta_cells(int32_t mode) {
+    FRMC_REQUIRE(mode >= -1 && mode <= 1, "delta cells mode must be -1 (by size), 0 (off) or 1 (on) (got %d)", mode);
+    g_cells_mode = mode;
+    return FRMC_OK;
+}
+
+int frmc_engine_delta_cells(frmc_engine *e, int32_t *nchunk_out, float *wcut_out) {
+    FRMC_REQUIRE(e && nchunk_out, "NULL argument");
+    *nchunk_out = e->cells ? e->nchunk : 0;
+    if (wcut_out) *wcut_out = e->cells ? e->wcut : 0.0f;
     return FRMC_OK;
 }
 

@@ -247,6 +247,150 @@ __global__ void __launch_bounds__(256) k_delta(View V, Inline I, Pending P, int3
     delta_body<PBC, ORTHO>(V, S, P, V.dn, k, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, G);
 }
 
+// ------------------------------------------------------- k_delta, chunk walk
+
+// proposals of a staged single-atom window (more proposals than the kernel
+// arguments carry), read from the device copy of the staging buffer
+struct StagedSrc {
+    const View &V;
+    __device__ int32_t atom(int32_t p) const { return V.gatoms[V.goff[V.pgroup[p]]]; }
+    __device__ int32_t el(int32_t p) const { return V.elem[atom(p)]; }
+    __device__ float old(int32_t p, int d) const { return V.pold[3 * (int64_t)V.poff[p] + d]; }
+    __device__ float nw(int32_t p, int d) const { return V.pnew[3 * (int64_t)V.poff[p] + d]; }
+};
+
+// a box coordinate wrapped into [0, 1] (1 only by rounding, the same point
+// as 0 on the unit circle)
+__device__ __forceinline__ float wrap01(float x) { return x - floorf(x); }
+
+// distance on the unit circle from f to the interval [lo, hi] (0 inside it)
+__device__ __forceinline__ float circ_gap(float f, float lo, float hi) {
+    if (f >= lo && f <= hi) return 0.0f;
+    const float direct = f < lo ? lo - f : f - hi;
+    const float around = f < lo ? (f + 1.0f) - hi : (lo + 1.0f) - f;
+    return fminf(direct, around);
+}
+
+// True when every atom of a chunk whose wrapped fractional coordinates lie
+// in the box bx (lo x,y,z, hi x,y,z) is farther than wcut from box position
+// f in the reference's minimum image: the real-space difference r of a
+// fractional difference b satisfies |r| >= |b_i| h_i for each axis i (h_i
+// the spacing of the lattice planes across axis i), and |b_i| is at least
+// the circular gap between f_i and [lo_i, hi_i].  wcut exceeds the largest
+// binned distance by more than the float32 rounding of both sides.
+__device__ __forceinline__ bool far_from(float fx, float fy, float fz, const float *bx, const View &V) {
+    return circ_gap(wrap01(fx), bx[0], bx[3]) * V.hsp[0] > V.wcut ||
+           circ_gap(wrap01(fy), bx[1], bx[4]) * V.hsp[1] > V.wcut ||
+           circ_gap(wrap01(fz), bx[2], bx[5]) * V.hsp[2] > V.wcut;
+}
+
+struct NearProp {
+    int32_t p, id, el;
+    float o[3], w[3];
+};
+
+// Single-atom window of a periodic box over the spatially sorted copy: block
+// (c, row) takes chunk c (one atom per thread) and proposals
+// [row*kPruneP, (row+1)*kPruneP).  A proposal whose old and new positions
+// are both beyond wcut of the chunk's box has no pair in range there and is
+// skipped; the others run as in delta_body (same pairs, same bins; the
+// atomics' order does not change integer sums).  The pending commit is
+// applied as in delta_body: base += its deltas (grid-stride), its slot
+// counts over the chunks near its old or new position (row 0), the moved
+// atom read at its new position; the chunk holding it takes the new
+// position into its box, and row 0 of that chunk writes the new position
+// and box back (the other rows read either version and substitute the same
+// values).
+template <bool ORTHO, class Src>
+__device__ __forceinline__ void delta_chunk_body(const View &V, const Src &S, const Pending &P, int32_t k) {
+    __shared__ NearProp nearp[kPruneP];
+    __shared__ float bx[6];
+    __shared__ int32_t nnear, cnear;
+    const int tid = threadIdx.x;
+    const int32_t c = blockIdx.x, row = blockIdx.y;
+    const int32_t p0 = row * kPruneP, np = min(kPruneP, k - p0);
+    if (P.valid) {
+        const int64_t gt = ((int64_t)row * gridDim.x + c) * blockDim.x + tid;
+        const int64_t gs = (int64_t)gridDim.x * gridDim.y * blockDim.x;
+        for (int32_t ci = 0; ci < V.C; ci++) {
+            const DevC &dc = V.c[ci];
+            for (int64_t s = gt; s < (int64_t)V.T * dc.H; s += gs) dc.base[s] += P.dn[dc.dnOff + s];
+        }
+    }
+    if (tid < 6) bx[tid] = load_dev(V.cbox + 8 * (int64_t)c + tid);
+    if (tid == 0) {
+        nnear = 0;
+        cnear = 0;
+    }
+    __syncthreads();
+    const bool own = P.valid && P.slot / kChunkAtoms == c;
+    if (own && tid < 3) {
+        const float f = wrap01(P.nw[tid]);
+        bx[tid] = fminf(bx[tid], f);
+        bx[3 + tid] = fmaxf(bx[3 + tid], f);
+    }
+    __syncthreads();
+    if (tid < np) {
+        const int32_t p = p0 + tid;
+        NearProp q;
+        q.p = p;
+        for (int d = 0; d < 3; d++) {
+            q.o[d] = S.old(p, d);
+            q.w[d] = S.nw(p, d);
+        }
+        if (!(far_from(q.o[0], q.o[1], q.o[2], bx, V) && far_from(q.w[0], q.w[1], q.w[2], bx, V))) {
+            q.id = S.atom(p);
+            q.el = S.el(p);
+            nearp[atomicAdd(&nnear, 1)] = q;
+        }
+    }
+    if (tid == kPruneP && row == 0 && P.valid &&
+        !(far_from(P.old[0], P.old[1], P.old[2], bx, V) && far_from(P.nw[0], P.nw[1], P.nw[2], bx, V)))
+        cnear = 1;
+    __syncthreads();
+    const int32_t nn = nnear;
+    const bool cm = cnear != 0;
+    if (own && row == 0 && tid == 0) {
+        V.sx[P.slot] = make_float4(P.nw[0], P.nw[1], P.nw[2], __int_as_float(P.atom * 64 + P.el));
+        for (int d = 0; d < 6; d++) V.cbox[8 * (int64_t)c + d] = bx[d];
+    }
+    if (P.valid && c == 0 && row == 0 && tid == 0)
+        for (int d = 0; d < 3; d++) V.xyz[3 * (int64_t)P.atom + d] = P.nw[d];
+    if (nn == 0 && !cm) return;
+    const float4 a = V.sx[(int64_t)c * kChunkAtoms + tid];
+    const int32_t tag = __float_as_int(a.w);
+    if (tag < 0) return;  // padding past the last atom
+    const int32_t j = tag >> 6, ej = tag & 63;
+    float cx = a.x, cy = a.y, cz = a.z;
+    if (P.valid && j == P.atom) {
+        cx = P.nw[0];
+        cy = P.nw[1];
+        cz = P.nw[2];
+    }
+    if (cm && j != P.atom) {
+        const float so = pair_s<true, ORTHO>(P.old[0], P.old[1], P.old[2], cx, cy, cz, V.B);
+        const float sn = pair_s<true, ORTHO>(P.nw[0], P.nw[1], P.nw[2], cx, cy, cz, V.B);
+        const bool intra = V.mol[P.atom] == V.mol[j];
+        commit_pair<true, ORTHO>(V, so, P.el, ej, intra, -1);
+        commit_pair<true, ORTHO>(V, sn, P.el, ej, intra, +1);
+    }
+    for (int32_t i = 0; i < nn; i++) {
+        const NearProp &q = nearp[i];
+        if (j == q.id) continue;
+        const float so = pair_s<true, ORTHO>(q.o[0], q.o[1], q.o[2], cx, cy, cz, V.B);
+        const float sn = pair_s<true, ORTHO>(q.w[0], q.w[1], q.w[2], cx, cy, cz, V.B);
+        delta_pair<true, ORTHO>(V, V.dn + (int64_t)q.p * V.dnStride, so, sn, q.el, ej);
+    }
+}
+
+// grid (nchunk, ceil(k/kPruneP)); proposals in the kernel arguments (INL) or
+// the staging buffer
+template <bool ORTHO, bool INL>
+__global__ void __launch_bounds__(256) k_delta_chunks(View V, Inline I, Pending P, int32_t k) {
+    if (INL) delta_chunk_body<ORTHO>(V, InlineSrc{I}, P, k);
+    else delta_chunk_body<ORTHO>(V, StagedSrc{V}, P, k);
+}
+
 // ------------------------------------------------------ pairwise summation
 
 // NumPy's pairwise float32 sum (np.add.reduce over a contiguous 1-D array):
@@ -1048,6 +1192,20 @@ __global__ void __launch_bounds__(256) k_commit(View V, Inline I, CommitCtl C) {
         }
         for (int32_t a = tid; a < g; a += blockDim.x)
             for (int d = 0; d < 3; d++) V.xyz[3 * (int64_t)G.id[a] + d] = G.w[a][d];
+        if (V.sx && tid == 0) {
+            // the sorted copy (kept for single-atom groups): the new position,
+            // and its chunk's box grown to hold it
+            for (int32_t a = 0; a < g; a++) {
+                const int32_t pos = V.spos[G.id[a]];
+                V.sx[pos] = make_float4(G.w[a][0], G.w[a][1], G.w[a][2], __int_as_float(G.id[a] * 64 + G.el[a]));
+                float *bx = V.cbox + 8 * (int64_t)(pos / kChunkAtoms);
+                for (int d = 0; d < 3; d++) {
+                    const float f = wrap01(G.w[a][d]);
+                    bx[d] = fminf(load_dev(bx + d), f);
+                    bx[3 + d] = fmaxf(load_dev(bx + 3 + d), f);
+                }
+            }
+        }
     }
 }
 
@@ -1077,6 +1235,19 @@ __global__ void k_fill(int32_t *__restrict__ p, int64_t n) {
 // ------------------------------------------------------------------ launch
 
 int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hipStream_t st) {
+    if (V.prune) {
+        // single-atom proposals in a periodic box: the chunk walk
+        const dim3 grid((unsigned)V.nchunk, (unsigned)((k + kPruneP - 1) / kPruneP));
+        if (V.ortho) {
+            if (I.k > 0) k_delta_chunks<true, true><<<grid, 256, 0, st>>>(V, I, P, k);
+            else k_delta_chunks<true, false><<<grid, 256, 0, st>>>(V, I, P, k);
+        } else {
+            if (I.k > 0) k_delta_chunks<false, true><<<grid, 256, 0, st>>>(V, I, P, k);
+            else k_delta_chunks<false, false><<<grid, 256, 0, st>>>(V, I, P, k);
+        }
+        FRMC_CHECK_LAUNCH();
+        return FRMC_OK;
+    }
     const dim3 grid(blocks_for(V.n, 256), (unsigned)((k + kDP - 1) / kDP));
     if (V.isPBC) {
         if (V.ortho) k_delta<true, true><<<grid, 256, 0, st>>>(V, I, P, k);

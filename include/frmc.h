@@ -471,6 +471,19 @@ int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out
    for buckets 0..n-1 (tuning first if needed). */
 int frmc_set_eval_shape(int32_t shape);
 int frmc_engine_eval_shapes(frmc_engine *e, int32_t *shapes_out, int32_t n);
+/* No counterpart in the reference (an engine-internal schedule).  In a
+   periodic box whose groups are single atoms, the engine keeps a copy of the
+   atoms sorted into compact chunks of 256 with each chunk's bounding box in
+   wrapped fractional coordinates, and k_delta skips a (chunk, proposal)
+   whose old and new positions are both farther than the largest binned
+   distance (plus a rounding margin) from the chunk's box along some lattice
+   axis: those pairs are out of range in the minimum image, so the deltas are
+   the same.  frmc_set_delta_cells: -1 (default) for engines of at least
+   4,096 atoms, 1 for every such engine, 0 never (engines created after the
+   call).  frmc_engine_delta_cells reports the chunks (0: no chunk walk) and
+   the cut-off distance used. */
+int frmc_set_delta_cells(int32_t mode);
+int frmc_engine_delta_cells(frmc_engine *e, int32_t *nchunk_out, float *wcut_out);
 /* window size of frmc_engine_run_translation: adaptive (default; k is the
    maximum, the size follows the acceptance rate) or always k */
 int frmc_engine_set_adaptive(frmc_engine *e, int enable);
