@@ -40,6 +40,7 @@ struct DevC {
     float sf;
     const float *w, *D, *shellV, *pref, *expv, *dataW;
     const float *sqm;       // S(Q) matrix transposed: [M][hpb], column q of M[r][q] zero-padded
+    const float *sqi;       // the same matrix interleaved by four rows: [hpb/4][M][4], zero-padded rows
     int32_t hpb;            // H rounded up to a multiple of 32
     const int32_t *leaves;  // (start,len) pairs of NumPy's pairwise-sum leaves of length M
     const int32_t *prog;    // postfix combine program over the leaves

@@ -431,7 +431,7 @@ static void set_units(frmc_engine *e, bool leaf) {
 // engine stream, twice, and the fastest is kept per shape class (device,
 // constraint layout, pair types, largest window), so engines of one class
 // tune once.
-static const int32_t kEvalShapes[8] = {0, 8, 16, 24, 4, 12, 20, 28};
+static const int32_t kEvalShapes[16] = {0, 8, 16, 24, 32, 40, 48, 56, 4, 12, 20, 28, 36, 44, 52, 60};
 static std::mutex g_eval_mu;
 static std::map<std::tuple<int, int32_t, int32_t, int32_t, int32_t, int32_t>, std::vector<int8_t>> g_eval_tuned;
 static int g_eval_forced = -1;  // frmc_set_eval_shape
@@ -459,7 +459,7 @@ static int launch_eval_shape(frmc_engine *e, int32_t k, int32_t sh) {
 static int tune_eval(frmc_engine *e) {
     const bool was_leaf = e->leaf_units;
     const int32_t nb = k_bucket(e->kmax) + 1;
-    const int32_t ns = e->Hmax > 4 * kEvalThreads ? 8 : 4;  // U = 8 differs from 4 only above 1,024 bins
+    const int32_t ns = e->Hmax > 4 * kEvalThreads ? 16 : 8;  // U = 8 differs from 4 only above 1,024 bins
     constexpr int reps = 5, rounds = 2;
     std::vector<hipEvent_t> ev(2 * rounds * nb * ns, nullptr);
     int rc = FRMC_OK;
@@ -950,7 +950,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             if (!cd.pairWeights || !cd.pairDensity || !cd.shellVolumes || !cd.prefactor || !cd.experimental)
                 EFAIL(FRMC_EINVAL, "constraint %d: NULL array", ci);
             if (!dc.rspace && !cd.sqMatrix) EFAIL(FRMC_EINVAL, "constraint %d: S(Q) needs sqMatrix", ci);
-            float *wp, *Dp, *sv, *pf, *ex, *dw = nullptr, *sm = nullptr;
+            float *wp, *Dp, *sv, *pf, *ex, *dw = nullptr, *sm = nullptr, *si = nullptr;
             ECHK(dupload(e, &wp, cd.pairWeights, (size_t)e->T));
             ECHK(dupload(e, &Dp, cd.pairDensity, (size_t)e->T));
             ECHK(dupload(e, &sv, cd.shellVolumes, (size_t)dc.H));
@@ -965,6 +965,14 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
                 for (int32_t r = 0; r < dc.H; r++)
                     for (int32_t q = 0; q < dc.M; q++) mt[(size_t)q * dc.hpb + r] = cd.sqMatrix[(size_t)r * dc.M + q];
                 ECHK(dupload(e, &sm, mt.data(), mt.size()));
+                // and interleaved by groups of four rows, [r/4][q][r%4]: a wave's
+                // 64 lanes read four rows of 64 consecutive Q points as one
+                // contiguous kilobyte
+                std::vector<float> mi((size_t)dc.M * dc.hpb, 0.0f);
+                for (int32_t r = 0; r < dc.H; r++)
+                    for (int32_t q = 0; q < dc.M; q++)
+                        mi[((size_t)(r / 4) * dc.M + q) * 4 + r % 4] = cd.sqMatrix[(size_t)r * dc.M + q];
+                ECHK(dupload(e, &si, mi.data(), mi.size()));
             }
             dc.w = wp;
             dc.D = Dp;
@@ -973,6 +981,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             dc.expv = ex;
             dc.dataW = dw;
             dc.sqm = sm;
+            dc.sqi = si;
             std::vector<int32_t> lv, pg;
             pairwise_plan(dc.M, lv, pg);
             int32_t *lvp, *pgp;
@@ -1259,8 +1268,8 @@ int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out
 }
 
 int frmc_set_eval_shape(int32_t shape) {
-    FRMC_REQUIRE(shape >= -1 && shape <= 28 && (shape == -1 || shape % 4 == 0),
-                 "eval shape must be -1 (tuned) or one of 0, 4, 8, ..., 28 (got %d)", shape);
+    FRMC_REQUIRE(shape >= -1 && shape <= 60 && (shape == -1 || shape % 4 == 0),
+                 "eval shape must be -1 (tuned) or one of 0, 4, 8, ..., 60 (got %d)", shape);
     g_eval_forced = shape;
     return FRMC_OK;
 }

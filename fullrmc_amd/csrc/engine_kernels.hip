@@ -502,7 +502,7 @@ struct Fuse {
 // otherwise the squared deviations go to sqbuf for k_eval_b.  F (batched
 // windows of one r-space constraint): also k_eval_b's work for that
 // constraint.
-template <int U>
+template <int U, bool IL>
 __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, int32_t p, int32_t u, float *sm,
                                             const Fuse *F, int64_t b, int64_t nblk) {
     const int32_t Hp = (V.Hmax + 15) & ~15;      // whole batches of 16 rows (float4 reads aligned)
@@ -536,17 +536,22 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
     const bool chain_wave = !rsp && w < 2 && 64 * w < len;
     const int32_t q = q0 + 64 * w + lane;
     const bool qok = chain_wave && 64 * w + lane < len;
-    const float *mcol = c.sqm + (int64_t)min(q, M - 1) * c.hpb;  // M[r][q] at mcol[r], 16-byte aligned
+    const int32_t qc = min(q, M - 1);
+    const float *mcol = c.sqm + (int64_t)qc * c.hpb;  // transposed: M[r][q] at mcol[r], 16-byte aligned
+    const float *mil = c.sqi + 4 * (int64_t)qc;         // interleaved: M[r][q] at mil[(r/4)*4M + r%4]
     // the S(Q) sum streams the matrix column in batches of B rows, NBM of
     // them in flight (16 x 4 measured 1.6x faster than 32 x 2 on 2,000 rows)
     constexpr int B = 16, NBM = 4;               // hpb is a multiple of 32 >= B
     float mr[NBM][B], gg[2][B];
     auto load_m = [&](float *m, int32_t r0) {
         // a batch past the end (a prefetch that is never used) rereads the last one
-        const float *src = mcol + min(r0, c.hpb - B);
+        const int32_t rc = min(r0, c.hpb - B);
 #pragma unroll
         for (int i4 = 0; i4 < B / 4; i4++) {
-            const float4 v = *reinterpret_cast<const float4 *>(src + 4 * i4);
+            // IL: the wave's lanes read one contiguous kilobyte per four rows;
+            // transposed: each lane reads its own column
+            const float *src = IL ? mil + (int64_t)(rc / 4 + i4) * 4 * M : mcol + rc + 4 * i4;
+            const float4 v = *reinterpret_cast<const float4 *>(src);
             m[4 * i4] = v.x; m[4 * i4 + 1] = v.y; m[4 * i4 + 2] = v.z; m[4 * i4 + 3] = v.w;
         }
     };
@@ -835,7 +840,11 @@ __global__ void __launch_bounds__(256) k_eval_b(View V, int32_t k) {
 // ceil(nunits/8)*8 blocks per proposal, so that (workgroups going to the
 // XCDs in turn) unit u always runs on XCD u mod 8 and each XCD's L2 holds
 // the matrix columns of its units only; the padding blocks return at once.
-template <int U, bool XCD>
+// IL: the S(Q) matrix read in its interleaved copy (one contiguous kilobyte
+// per wave and four rows: fewer cache lines per load than the transposed
+// copy's 64, which two chain waves of a CU contend for, and no L2 thrash
+// when many proposals stream it), else the transposed one.
+template <int U, bool XCD, bool IL>
 __global__ void __launch_bounds__(kEvalThreads) k_eval_a(View V, int32_t k) {
     extern __shared__ float sm[];
     int32_t u, p;
@@ -853,7 +862,7 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_a(View V, int32_t k) {
         b = (int64_t)p * gridDim.x + u;
         nblk = (int64_t)gridDim.x * gridDim.y;
     }
-    eval_a_body<U>(V, V.dn, p, u, sm, nullptr, b, nblk);
+    eval_a_body<U, IL>(V, V.dn, p, u, sm, nullptr, b, nblk);
 }
 
 // ----------------------------------------------------------- batched windows
@@ -1097,9 +1106,9 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_a_batch(BatchArgs A) {
     const int2 m = S.m;
     if (V.C == 1 && V.c[0].rspace) {
         const Fuse F{W.dn_zero, W.zero_len, W.seq, W.k};
-        eval_a_body<4>(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, &F, 0, 1);
+        eval_a_body<4, false>(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, &F, 0, 1);
     } else {
-        eval_a_body<4>(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, nullptr, 0, 1);
+        eval_a_body<4, false>(V, W.dn, m.y / V.nunits, m.y % V.nunits, sm, nullptr, 0, 1);
     }
 }
 
@@ -1265,28 +1274,35 @@ size_t eval_lds_bytes(const View &V) {
                             (size_t)2 * V.maxLeaves);
 }
 
-template <int U, bool XCD>
+template <int U, bool XCD, bool IL>
 static void launch_k_eval_a(const View &V, int32_t k, hipStream_t st) {
     const size_t shm = eval_lds_bytes(V);
     if (shm > 48 * 1024)
-        (void)hipFuncSetAttribute((const void *)k_eval_a<U, XCD>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void *)k_eval_a<U, XCD, IL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shm);
     if (XCD)
-        k_eval_a<U, XCD><<<dim3((unsigned)((V.nunits + 7) & ~7) * (unsigned)k), kEvalThreads, shm, st>>>(V, k);
+        k_eval_a<U, XCD, IL><<<dim3((unsigned)((V.nunits + 7) & ~7) * (unsigned)k), kEvalThreads, shm, st>>>(V, k);
     else
-        k_eval_a<U, XCD><<<dim3((unsigned)V.nunits, (unsigned)k), kEvalThreads, shm, st>>>(V, k);
+        k_eval_a<U, XCD, IL><<<dim3((unsigned)V.nunits, (unsigned)k), kEvalThreads, shm, st>>>(V, k);
+}
+
+template <bool IL>
+static void launch_k_eval_a_il(const View &V, int32_t k, hipStream_t st, bool u8, bool xcd) {
+    if (xcd) {
+        if (u8) launch_k_eval_a<8, true, IL>(V, k, st); else launch_k_eval_a<4, true, IL>(V, k, st);
+    } else {
+        if (u8) launch_k_eval_a<8, false, IL>(V, k, st); else launch_k_eval_a<4, false, IL>(V, k, st);
+    }
 }
 
 // shape s of k_eval_a: bins per thread per G(r) pass U = 8 if s & 4 (and the
-// grid has more than 1,024 bins) else 4; s & 8: XCD-aligned units.  Every
-// shape gives the same results.
+// grid has more than 1,024 bins) else 4; s & 8: XCD-aligned units; s & 32:
+// the interleaved S(Q) matrix (s & 16, the unit layout, is the caller's).
+// Every shape gives the same results.
 int launch_eval(const View &V, int32_t k, hipStream_t st, int32_t shape) {
     const bool u8 = (shape & 4) && V.Hmax > 4 * kEvalThreads;
-    if (shape & 8) {
-        if (u8) launch_k_eval_a<8, true>(V, k, st); else launch_k_eval_a<4, true>(V, k, st);
-    } else {
-        if (u8) launch_k_eval_a<8, false>(V, k, st); else launch_k_eval_a<4, false>(V, k, st);
-    }
+    if (shape & 32) launch_k_eval_a_il<true>(V, k, st, u8, (shape & 8) != 0);
+    else launch_k_eval_a_il<false>(V, k, st, u8, (shape & 8) != 0);
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;
 }
