@@ -285,7 +285,7 @@ __device__ __forceinline__ bool far_from(float fx, float fy, float fz, const flo
 }
 
 struct NearProp {
-    int32_t p, id, el;
+    int32_t id, el, near;
     float o[3], w[3];
 };
 
@@ -300,15 +300,28 @@ struct NearProp {
 // atom read at its new position; the chunk holding it takes the new
 // position into its box, and row 0 of that chunk writes the new position
 // and box back (the other rows read either version and substitute the same
-// values).
+// values).  The chunk's atom, its box and the proposals load together at
+// the start: one memory round trip before the pair work.
 template <bool ORTHO, class Src>
 __device__ __forceinline__ void delta_chunk_body(const View &V, const Src &S, const Pending &P, int32_t k) {
-    __shared__ NearProp nearp[kPruneP];
-    __shared__ float bx[6];
-    __shared__ int32_t nnear, cnear;
+    __shared__ NearProp prop[kPruneP];
     const int tid = threadIdx.x;
     const int32_t c = blockIdx.x, row = blockIdx.y;
     const int32_t p0 = row * kPruneP, np = min(kPruneP, k - p0);
+    const float4 a = V.sx[(int64_t)c * kChunkAtoms + tid];
+    float bx[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) bx[d] = load_dev(V.cbox + 8 * (int64_t)c + d);
+    NearProp q;
+    if (tid < np) {
+        const int32_t p = p0 + tid;
+        q.id = S.atom(p);
+        q.el = S.el(p);
+        for (int d = 0; d < 3; d++) {
+            q.o[d] = S.old(p, d);
+            q.w[d] = S.nw(p, d);
+        }
+    }
     if (P.valid) {
         const int64_t gt = ((int64_t)row * gridDim.x + c) * blockDim.x + tid;
         const int64_t gs = (int64_t)gridDim.x * gridDim.y * blockDim.x;
@@ -317,47 +330,28 @@ __device__ __forceinline__ void delta_chunk_body(const View &V, const Src &S, co
             for (int64_t s = gt; s < (int64_t)V.T * dc.H; s += gs) dc.base[s] += P.dn[dc.dnOff + s];
         }
     }
-    if (tid < 6) bx[tid] = load_dev(V.cbox + 8 * (int64_t)c + tid);
-    if (tid == 0) {
-        nnear = 0;
-        cnear = 0;
-    }
-    __syncthreads();
     const bool own = P.valid && P.slot / kChunkAtoms == c;
-    if (own && tid < 3) {
-        const float f = wrap01(P.nw[tid]);
-        bx[tid] = fminf(bx[tid], f);
-        bx[3 + tid] = fmaxf(bx[3 + tid], f);
-    }
-    __syncthreads();
-    if (tid < np) {
-        const int32_t p = p0 + tid;
-        NearProp q;
-        q.p = p;
+    if (own) {
+#pragma unroll
         for (int d = 0; d < 3; d++) {
-            q.o[d] = S.old(p, d);
-            q.w[d] = S.nw(p, d);
+            const float f = wrap01(P.nw[d]);
+            bx[d] = fminf(bx[d], f);
+            bx[3 + d] = fmaxf(bx[3 + d], f);
         }
-        if (!(far_from(q.o[0], q.o[1], q.o[2], bx, V) && far_from(q.w[0], q.w[1], q.w[2], bx, V))) {
-            q.id = S.atom(p);
-            q.el = S.el(p);
-            nearp[atomicAdd(&nnear, 1)] = q;
+        if (row == 0 && tid == 0) {
+            V.sx[P.slot] = make_float4(P.nw[0], P.nw[1], P.nw[2], __int_as_float(P.atom * 64 + P.el));
+            for (int d = 0; d < 6; d++) V.cbox[8 * (int64_t)c + d] = bx[d];
         }
-    }
-    if (tid == kPruneP && row == 0 && P.valid &&
-        !(far_from(P.old[0], P.old[1], P.old[2], bx, V) && far_from(P.nw[0], P.nw[1], P.nw[2], bx, V)))
-        cnear = 1;
-    __syncthreads();
-    const int32_t nn = nnear;
-    const bool cm = cnear != 0;
-    if (own && row == 0 && tid == 0) {
-        V.sx[P.slot] = make_float4(P.nw[0], P.nw[1], P.nw[2], __int_as_float(P.atom * 64 + P.el));
-        for (int d = 0; d < 6; d++) V.cbox[8 * (int64_t)c + d] = bx[d];
     }
     if (P.valid && c == 0 && row == 0 && tid == 0)
         for (int d = 0; d < 3; d++) V.xyz[3 * (int64_t)P.atom + d] = P.nw[d];
-    if (nn == 0 && !cm) return;
-    const float4 a = V.sx[(int64_t)c * kChunkAtoms + tid];
+    if (tid < np) {
+        q.near = !(far_from(q.o[0], q.o[1], q.o[2], bx, V) && far_from(q.w[0], q.w[1], q.w[2], bx, V));
+        prop[tid] = q;
+    }
+    const bool cm = row == 0 && P.valid &&
+                    !(far_from(P.old[0], P.old[1], P.old[2], bx, V) && far_from(P.nw[0], P.nw[1], P.nw[2], bx, V));
+    __syncthreads();
     const int32_t tag = __float_as_int(a.w);
     if (tag < 0) return;  // padding past the last atom
     const int32_t j = tag >> 6, ej = tag & 63;
@@ -374,12 +368,12 @@ __device__ __forceinline__ void delta_chunk_body(const View &V, const Src &S, co
         commit_pair<true, ORTHO>(V, so, P.el, ej, intra, -1);
         commit_pair<true, ORTHO>(V, sn, P.el, ej, intra, +1);
     }
-    for (int32_t i = 0; i < nn; i++) {
-        const NearProp &q = nearp[i];
-        if (j == q.id) continue;
-        const float so = pair_s<true, ORTHO>(q.o[0], q.o[1], q.o[2], cx, cy, cz, V.B);
-        const float sn = pair_s<true, ORTHO>(q.w[0], q.w[1], q.w[2], cx, cy, cz, V.B);
-        delta_pair<true, ORTHO>(V, V.dn + (int64_t)q.p * V.dnStride, so, sn, q.el, ej);
+    for (int32_t i = 0; i < np; i++) {
+        if (!prop[i].near || j == prop[i].id) continue;
+        const NearProp &r = prop[i];
+        const float so = pair_s<true, ORTHO>(r.o[0], r.o[1], r.o[2], cx, cy, cz, V.B);
+        const float sn = pair_s<true, ORTHO>(r.w[0], r.w[1], r.w[2], cx, cy, cz, V.B);
+        delta_pair<true, ORTHO>(V, V.dn + (int64_t)(p0 + i) * V.dnStride, so, sn, r.el, ej);
     }
 }
 
