@@ -269,14 +269,17 @@ def kernel_roofline(name, avg_s, n, props_per_launch, shape=None, traffic=None, 
             "proposals_per_launch": round(props_per_launch, 2)}
 
 
-def committed_traffic(profile, kernel):
-    """HBM bytes per launch of kernel from a committed PMC summary (JSON
-    beside the .md), with its source, or (None, None)."""
+def committed_traffic(profile, kernel, proposals):
+    """HBM bytes per launch of kernel at `proposals` proposals per launch,
+    from a committed PMC summary (JSON beside the .md: measured bytes per
+    proposal, scaled here), with its source, or (None, None)."""
     try:
         with open(os.path.join(ROOT, "profiles", profile)) as f:
             rec = json.load(f)["kernels"][kernel]
-        return int(rec["traffic_bytes_per_launch"]), "profiles/%s (%s)" % (profile, rec["how"])
-    except (OSError, KeyError, ValueError):
+        per = float(rec["traffic_bytes_per_proposal"])
+        return int(round(per * proposals)), "profiles/%s (%s; %.0f B per proposal x %d)" % (
+            profile, rec["how"], per, proposals)
+    except (OSError, KeyError, ValueError, TypeError):
         return None, None
 
 
@@ -581,7 +584,9 @@ def engine_timed_run(args, E, n, shape, traffic_profile, steps, moves, k, log_fi
     # two HIP events on the engine stream: launch gaps and event markers
     # amortised, the figure a kernel trace's average duration compares with
     b2b = ctypes.c_float(0.0)
+    marker()  # the third and fourth markers bracket these launches in a kernel trace
     _lib.check(L.frmc_engine_time_eval(h, kwin, 200, ctypes.byref(b2b)))
+    marker()
     avg_s = b2b.value * 1e-3  # ms per launch
     # the tuned k_eval_a shape per window-size bucket, and every shape's time
     # at the mean window size
@@ -596,7 +601,7 @@ def engine_timed_run(args, E, n, shape, traffic_profile, steps, moves, k, log_fi
             per_shape[str(sh)] = round(1e3 * t.value, 2)
     finally:
         _lib.check(L.frmc_set_eval_shape(-1))
-    traffic, tsrc = committed_traffic(traffic_profile, "k_eval_a")
+    traffic, tsrc = committed_traffic(traffic_profile, "k_eval_a", kwin)
     roof = kernel_roofline("k_eval_a", avg_s, n, kwin, shape, traffic, tsrc)
     if roof is not None:
         roof["duration_source"] = ("HIP events around 200 back-to-back launches at the run's mean window size "

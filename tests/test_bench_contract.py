@@ -34,18 +34,18 @@ def test_kernel_roofline_record():
     assert bench.kernel_roofline("k_commit", 1e-5, 13500, 1.0) is None
 
 
-def test_committed_traffic_reads_per_launch_bytes():
+def test_committed_traffic_scales_the_per_proposal_bytes():
     for prof in ("hbm_traffic_engine_sio2_200k.json", "hbm_traffic_engine_b2_13500.json"):
         path = os.path.join(ROOT, "profiles", prof)
         if not os.path.exists(path):
             continue
         with open(path) as f:
             rec = json.load(f)["kernels"]["k_eval_a"]
-        if "traffic_bytes_per_launch" not in rec:
+        if "traffic_bytes_per_proposal" not in rec or "how" not in rec:
             continue
-        t, src = bench.committed_traffic(prof, "k_eval_a")
-        assert t == int(rec["traffic_bytes_per_launch"]) and prof in src
-    assert bench.committed_traffic("no_such_profile.json", "k_eval_a") == (None, None)
+        t, src = bench.committed_traffic(prof, "k_eval_a", 12)
+        assert t == int(round(float(rec["traffic_bytes_per_proposal"]) * 12)) and prof in src
+    assert bench.committed_traffic("no_such_profile.json", "k_eval_a", 12) == (None, None)
 
 
 def test_more_than_one_gpu_is_refused():
