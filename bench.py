@@ -600,7 +600,7 @@ def engine_timed_run(args, E, n, shape, traffic_profile, steps, moves, k, log_fi
             _lib.check(L.frmc_engine_time_eval(h, kwin, 50, ctypes.byref(t)))
             per_shape[str(sh)] = round(1e3 * t.value, 2)
     finally:
-        _lib.check(L.frmc_set_eval_shape(-1))
+        _lib.check(L.frmc_set_eval_shape(args.eval_shape))
     traffic, tsrc = committed_traffic(traffic_profile, "k_eval_a", kwin)
     roof = kernel_roofline("k_eval_a", avg_s, n, kwin, shape, traffic, tsrc)
     if roof is not None:
@@ -747,6 +747,8 @@ def main():
                     help="also time the replicas with one stream per engine")
     ap.add_argument("--molecular-moves", type=int, default=1000,
                     help="moves of the 6,500-atom molecular triclinic box (0: skip)")
+    ap.add_argument("--eval-shape", type=int, default=-1,
+                    help="force one window-evaluation shape (0..60 step 4; -1: tuned per window size)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -767,7 +769,7 @@ def main():
     from fullrmc_amd import _lib
     from fullrmc_amd._device import require_gpu
     require_gpu()
-    _lib.lib()
+    _lib.check(_lib.lib().frmc_set_eval_shape(args.eval_shape))
 
     if args.copy_calibration:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.int32, device="cuda").fill_(1)

@@ -131,10 +131,12 @@ struct Pending {
     const int32_t *dn;
 };
 
-// atoms per chunk of the sorted copy (one per thread of a k_delta block) and
-// proposals per block of the chunked k_delta
+// atoms per chunk of the sorted copy (one per thread of a k_delta block), the
+// most proposals per block of the chunked k_delta, and the blocks its launch
+// aims for (proposals per block = k * chunks / kDeltaBlocks, at least 1)
 constexpr int32_t kChunkAtoms = 256;
 constexpr int32_t kPruneP = 32;
+constexpr int32_t kDeltaBlocks = 1024;
 
 // Which proposal of a window k_commit commits: p >= 0 a given one (the
 // caller decided), p < 0 the first proposal that the reference's rule accepts

@@ -291,7 +291,7 @@ struct NearProp {
 
 // Single-atom window of a periodic box over the spatially sorted copy: block
 // (c, row) takes chunk c (one atom per thread) and proposals
-// [row*kPruneP, (row+1)*kPruneP).  A proposal whose old and new positions
+// [row*pp, (row+1)*pp), pp <= kPruneP.  A proposal whose old and new positions
 // are both beyond wcut of the chunk's box has no pair in range there and is
 // skipped; the others run as in delta_body (same pairs, same bins; the
 // atomics' order does not change integer sums).  The pending commit is
@@ -303,11 +303,12 @@ struct NearProp {
 // values).  The chunk's atom, its box and the proposals load together at
 // the start: one memory round trip before the pair work.
 template <bool ORTHO, class Src>
-__device__ __forceinline__ void delta_chunk_body(const View &V, const Src &S, const Pending &P, int32_t k) {
+__device__ __forceinline__ void delta_chunk_body(const View &V, const Src &S, const Pending &P, int32_t k,
+                                                 int32_t pp) {
     __shared__ NearProp prop[kPruneP];
     const int tid = threadIdx.x;
     const int32_t c = blockIdx.x, row = blockIdx.y;
-    const int32_t p0 = row * kPruneP, np = min(kPruneP, k - p0);
+    const int32_t p0 = row * pp, np = min(pp, k - p0);
     const float4 a = V.sx[(int64_t)c * kChunkAtoms + tid];
     float bx[6];
 #pragma unroll
@@ -377,12 +378,12 @@ __device__ __forceinline__ void delta_chunk_body(const View &V, const Src &S, co
     }
 }
 
-// grid (nchunk, ceil(k/kPruneP)); proposals in the kernel arguments (INL) or
-// the staging buffer
+// grid (nchunk, ceil(k/pp)); proposals in the kernel arguments (INL) or the
+// staging buffer
 template <bool ORTHO, bool INL>
-__global__ void __launch_bounds__(256) k_delta_chunks(View V, Inline I, Pending P, int32_t k) {
-    if (INL) delta_chunk_body<ORTHO>(V, InlineSrc{I}, P, k);
-    else delta_chunk_body<ORTHO>(V, StagedSrc{V}, P, k);
+__global__ void __launch_bounds__(256) k_delta_chunks(View V, Inline I, Pending P, int32_t k, int32_t pp) {
+    if (INL) delta_chunk_body<ORTHO>(V, InlineSrc{I}, P, k, pp);
+    else delta_chunk_body<ORTHO>(V, StagedSrc{V}, P, k, pp);
 }
 
 // ------------------------------------------------------ pairwise summation
@@ -1240,13 +1241,17 @@ __global__ void k_fill(int32_t *__restrict__ p, int64_t n) {
 int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hipStream_t st) {
     if (V.prune) {
         // single-atom proposals in a periodic box: the chunk walk
-        const dim3 grid((unsigned)V.nchunk, (unsigned)((k + kPruneP - 1) / kPruneP));
+        // proposals per block: enough rows of chunks for about kDeltaBlocks
+        // blocks (a small box has few chunks, each near most proposals)
+        const int64_t want = ((int64_t)k * V.nchunk + kDeltaBlocks - 1) / kDeltaBlocks;
+        const int32_t pp = (int32_t)std::max<int64_t>(1, std::min<int64_t>(kPruneP, want));
+        const dim3 grid((unsigned)V.nchunk, (unsigned)((k + pp - 1) / pp));
         if (V.ortho) {
-            if (I.k > 0) k_delta_chunks<true, true><<<grid, 256, 0, st>>>(V, I, P, k);
-            else k_delta_chunks<true, false><<<grid, 256, 0, st>>>(V, I, P, k);
+            if (I.k > 0) k_delta_chunks<true, true><<<grid, 256, 0, st>>>(V, I, P, k, pp);
+            else k_delta_chunks<true, false><<<grid, 256, 0, st>>>(V, I, P, k, pp);
         } else {
-            if (I.k > 0) k_delta_chunks<false, true><<<grid, 256, 0, st>>>(V, I, P, k);
-            else k_delta_chunks<false, false><<<grid, 256, 0, st>>>(V, I, P, k);
+            if (I.k > 0) k_delta_chunks<false, true><<<grid, 256, 0, st>>>(V, I, P, k, pp);
+            else k_delta_chunks<false, false><<<grid, 256, 0, st>>>(V, I, P, k, pp);
         }
         FRMC_CHECK_LAUNCH();
         return FRMC_OK;
