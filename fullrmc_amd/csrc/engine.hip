@@ -967,7 +967,9 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
                 ECHK(dupload(e, &sm, mt.data(), mt.size()));
                 // and interleaved by groups of four rows, [r/4][q][r%4]: a wave's
                 // 64 lanes read four rows of 64 consecutive Q points as one
-                // contiguous kilobyte
+                // contiguous kilobyte (k_eval_a indexes it with 32-bit offsets)
+                if ((int64_t)dc.hpb * dc.M >= ((int64_t)1 << 31))
+                    EFAIL(FRMC_EINVAL, "constraint %d: S(Q) matrix of %d x %d too large", ci, dc.hpb, dc.M);
                 std::vector<float> mi((size_t)dc.M * dc.hpb, 0.0f);
                 for (int32_t r = 0; r < dc.H; r++)
                     for (int32_t q = 0; q < dc.M; q++)

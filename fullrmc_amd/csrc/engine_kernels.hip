@@ -544,8 +544,11 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
 #pragma unroll
         for (int i4 = 0; i4 < B / 4; i4++) {
             // IL: the wave's lanes read one contiguous kilobyte per four rows;
-            // transposed: each lane reads its own column
-            const float *src = IL ? mil + (int64_t)(rc / 4 + i4) * 4 * M : mcol + rc + 4 * i4;
+            // transposed: each lane reads its own column.  The interleaved
+            // offset is 32-bit (hpb*M < 2^31, checked at engine creation): a
+            // 64-bit multiply per load measured 7 us slower per launch on
+            // 2,000 rows
+            const float *src = IL ? mil + (uint32_t)((rc / 4 + i4) * 4 * M) : mcol + rc + 4 * i4;
             const float4 v = *reinterpret_cast<const float4 *>(src);
             m[4 * i4] = v.x; m[4 * i4 + 1] = v.y; m[4 * i4 + 2] = v.z; m[4 * i4 + 3] = v.w;
         }
