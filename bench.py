@@ -230,6 +230,11 @@ def cpu_full_histogram_sample(start, elem, dr, H, rows, threads):
 # operation order forbids contraction): one operation per lane per cycle,
 # 256 CUs x 64 lanes x 2.4 GHz
 ATTAINABLE_FP32_TOPS = 39.3
+# a wave's chain of dependent v_add_f32 issues one add per 6.94 cycles
+# (measured on MI355X: 2,000 dependent register adds per wave, clock64 around
+# the loop), at the 2.4 GHz engine clock
+DEPENDENT_ADD_CYCLES = 6.94
+CLOCK_GHZ = 2.4
 
 
 def eval_flops_per_proposal(H_pdc=2000, H_sfc=2000, nQ=1000, T=3):
@@ -258,6 +263,12 @@ def kernel_roofline(name, avg_s, n, props_per_launch, shape=None, traffic=None, 
     if not avg_s > 0:
         return None
     tf = flops / avg_s / 1e12
+    extra = {}
+    if name == "k_eval_a":
+        # the sequential floor: each Q point's S(Q) sum is H dependent adds
+        floor_us = shape[1] * DEPENDENT_ADD_CYCLES / CLOCK_GHZ * 1e-3
+        extra = {"sequential_floor_us": round(floor_us, 2),
+                 "frac_of_sequential_floor": round(floor_us / (avg_s * 1e6), 4)}
     return {"bound": "valu", "limited_by": limit,
             "compute": "fp32 VALU, no matrix instructions (bit-exact float32 forbids FMA and packed FP32)",
             "achieved": round(tf, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -266,7 +277,7 @@ def kernel_roofline(name, avg_s, n, props_per_launch, shape=None, traffic=None, 
             "frac_of_attainable": round(tf / ATTAINABLE_FP32_TOPS, 6),
             "traffic": traffic, "traffic_source": traffic_source,
             "kernel": name, "avg_launch_us": round(avg_s * 1e6, 2), "flops_per_launch": flops,
-            "proposals_per_launch": round(props_per_launch, 2)}
+            "proposals_per_launch": round(props_per_launch, 2), **extra}
 
 
 def committed_traffic(profile, kernel, proposals):

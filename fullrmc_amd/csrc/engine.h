@@ -187,6 +187,13 @@ struct BatchArgs {
     const int2 *map_d, *map_a, *map_b;  // per block of each kernel: (window, block within it)
 };
 
+// Kernel arguments travel by value.  HIP allows 4 KiB per launch; the largest
+// launches (k_commit: View + Inline + CommitCtl; k_delta: View + Inline +
+// Pending + two ints) keep a margin of more than 512 bytes for the runtime's
+// hidden arguments.
+static_assert(sizeof(View) + sizeof(Inline) + sizeof(CommitCtl) <= 4096 - 512, "k_commit arguments too large");
+static_assert(sizeof(View) + sizeof(Inline) + sizeof(Pending) + 8 <= 4096 - 512, "k_delta arguments too large");
+
 // launchers (engine_kernels.hip)
 int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hipStream_t st);
 int launch_eval(const View &V, int32_t k, hipStream_t st, int32_t shape);
