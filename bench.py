@@ -605,7 +605,7 @@ def engine_timed_run(args, E, n, shape, traffic_profile, steps, moves, k, log_fi
     _lib.check(L.frmc_engine_eval_shapes(h, shapes, 8))
     per_shape = {}
     try:
-        for sh in range(0, 64, 4):
+        for sh in list(range(0, 64, 4)) + [80, 84, 88, 92]:
             _lib.check(L.frmc_set_eval_shape(sh))
             t = ctypes.c_float(0.0)
             _lib.check(L.frmc_engine_time_eval(h, kwin, 50, ctypes.byref(t)))

@@ -207,6 +207,7 @@ int launch_batch(const BatchArgs &A, int32_t nd, int32_t na, int32_t nb, size_t 
                  hipStream_t st);
 void pairwise_plan(int32_t n, std::vector<int32_t> &leaves, std::vector<int32_t> &prog);
 size_t eval_lds_bytes(const View &V);
+size_t pc_lds_bytes(const View &V);  // k_eval_pc (the split-chain shapes)
 size_t eval_b_lds_bytes(const View &V);
 
 }  // namespace frmc
@@ -267,6 +268,7 @@ struct frmc_engine {
     bool have_real = false;
     bool adaptive = true;  // window size follows the acceptance rate
     bool eval_tuned = false;     // eval_shape holds the tuned k_eval_a shapes
+    int8_t pc_fit = -1;          // the split-chain shapes' LDS fits a workgroup (-1: not yet known)
     int8_t eval_shape[16] = {0};  // k_eval_a shape per window-size bucket ceil(log2 k)
     std::vector<void *> allocs, host_allocs;
     // optional per-kernel timing with HIP events on the engine stream

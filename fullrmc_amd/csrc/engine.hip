@@ -431,7 +431,25 @@ static void set_units(frmc_engine *e, bool leaf) {
 // engine stream, twice, and the fastest is kept per shape class (device,
 // constraint layout, pair types, largest window), so engines of one class
 // tune once.
+// the shapes the tuner times: U = 4 first, then U = 8 (above 1,024 bins),
+// then the split-chain shapes (when their LDS fits): U = 2, then U = 4
+// (above 2,048 bins)
 static const int32_t kEvalShapes[16] = {0, 8, 16, 24, 32, 40, 48, 56, 4, 12, 20, 28, 36, 44, 52, 60};
+static const int32_t kPcShapes[4] = {80, 88, 84, 92};
+constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS per workgroup on gfx950
+
+static bool pc_fits(frmc_engine *e) {
+    if (e->pc_fit < 0) e->pc_fit = pc_lds_bytes(make_view(e, e->cur_stage)) <= kMaxLdsBytes ? 1 : 0;
+    return e->pc_fit == 1;
+}
+
+// shape i of the tuner's list for this engine, -1 past its end
+static int32_t tuner_shape(frmc_engine *e, int32_t i) {
+    const int32_t nbase = e->Hmax > 4 * kEvalThreads ? 16 : 8;
+    if (i < nbase) return kEvalShapes[i];
+    const int32_t npc = pc_fits(e) ? (e->Hmax > 2048 ? 4 : 2) : 0;
+    return i - nbase < npc ? kPcShapes[i - nbase] : -1;
+}
 static std::mutex g_eval_mu;
 static std::map<std::tuple<int, int32_t, int32_t, int32_t, int32_t, int32_t>, std::vector<int8_t>> g_eval_tuned;
 static int g_eval_forced = -1;  // frmc_set_eval_shape
@@ -459,7 +477,8 @@ static int launch_eval_shape(frmc_engine *e, int32_t k, int32_t sh) {
 static int tune_eval(frmc_engine *e) {
     const bool was_leaf = e->leaf_units;
     const int32_t nb = k_bucket(e->kmax) + 1;
-    const int32_t ns = e->Hmax > 4 * kEvalThreads ? 16 : 8;  // U = 8 differs from 4 only above 1,024 bins
+    int32_t ns = 0;
+    while (tuner_shape(e, ns) >= 0) ns++;
     constexpr int reps = 5, rounds = 2;
     std::vector<hipEvent_t> ev(2 * rounds * nb * ns, nullptr);
     int rc = FRMC_OK;
@@ -470,9 +489,9 @@ static int tune_eval(frmc_engine *e) {
         for (int round = 0; round < rounds && rc == FRMC_OK; round++)
             for (int32_t i = 0; i < ns && rc == FRMC_OK; i++) {
                 const size_t q = 2 * (((size_t)b * rounds + round) * ns + i);
-                rc = launch_eval_shape(e, k, kEvalShapes[i]);  // warm
+                rc = launch_eval_shape(e, k, tuner_shape(e, i));  // warm
                 if (rc == FRMC_OK && hipEventRecord(ev[q], e->st) != hipSuccess) rc = FRMC_EHIP;
-                for (int r = 0; r < reps && rc == FRMC_OK; r++) rc = launch_eval_shape(e, k, kEvalShapes[i]);
+                for (int r = 0; r < reps && rc == FRMC_OK; r++) rc = launch_eval_shape(e, k, tuner_shape(e, i));
                 if (rc == FRMC_OK && hipEventRecord(ev[q + 1], e->st) != hipSuccess) rc = FRMC_EHIP;
             }
     }
@@ -490,7 +509,7 @@ static int tune_eval(frmc_engine *e) {
                 }
                 if ((i == 0 && round == 0) || ms < best) {
                     best = ms;
-                    pick = kEvalShapes[i];
+                    pick = tuner_shape(e, i);
                 }
             }
         e->eval_shape[b] = (int8_t)pick;
@@ -542,7 +561,12 @@ static int launch_window(frmc_engine *e, int s, int32_t k, const Inline *I, cons
     // a device-decided commit reads the totals k_eval_b leaves on the device,
     // so those windows take 64-Q units
     int32_t shape = eval_shape(e, k);
-    if (C) shape &= ~16;
+    if (C) shape &= ~(16 | 64);  // the 64-Q layout, with k_eval_b
+    if ((shape & 64) && !pc_fits(e)) {
+        set_error("eval shape %d needs %zu bytes of LDS per workgroup (at most %zu)", shape,
+                  pc_lds_bytes(make_view(e, s)), kMaxLdsBytes);
+        return FRMC_EINVAL;
+    }
     set_units(e, (shape & 16) != 0);
     e->seq++;
     View V = make_view(e, s);
@@ -1270,8 +1294,9 @@ int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out
 }
 
 int frmc_set_eval_shape(int32_t shape) {
-    FRMC_REQUIRE(shape >= -1 && shape <= 60 && (shape == -1 || shape % 4 == 0),
-                 "eval shape must be -1 (tuned) or one of 0, 4, 8, ..., 60 (got %d)", shape);
+    FRMC_REQUIRE(shape == -1 || (shape >= 0 && shape <= 60 && shape % 4 == 0) || shape == 80 || shape == 84 ||
+                     shape == 88 || shape == 92,
+                 "eval shape must be -1 (tuned), one of 0, 4, 8, ..., 60, or 80, 84, 88, 92 (got %d)", shape);
     g_eval_forced = shape;
     return FRMC_OK;
 }

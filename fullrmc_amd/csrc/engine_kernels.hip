@@ -486,6 +486,127 @@ struct Fuse {
     int32_t nb;  // blocks of the window
 };
 
+// G(r) of proposal p (dn its delta block) into Gs[0, H), or for an r-space
+// constraint the squared deviations of its model, by NT threads with U bins
+// each per pass, in the reference's float32 order
+// (PairDistributionConstraints.py:854-895).
+template <int U, int NT>
+__device__ __forceinline__ void build_g(const View &V, const DevC &c, const int32_t *dn, float *Gs) {
+    const int tid = threadIdx.x;
+    const int32_t H = c.H, T = V.T;
+    const bool rsp = c.rspace;
+    const int32_t *base = c.base;
+    // U bins per lane per pass; every load is unconditional (clamped index,
+    // valid pointer): a load under a divergent branch is followed by its
+    // own wait, which serialises the batch
+    constexpr int TMAX = 3;  // pair types (two elements) whose counts load in one round
+    const float *expv = rsp ? c.expv : c.shellV;                 // S(Q): unused (expv has M entries)
+    const float *dataW = (rsp && c.hasW) ? c.dataW : c.shellV;   // unused unless hasW
+    for (int32_t r0 = 0; r0 < H; r0 += NT * U) {
+        int32_t rr[U];
+#pragma unroll
+        for (int i = 0; i < U; i++) rr[i] = min(r0 + tid + NT * i, H - 1);
+        float Gr[U];
+        float sv[U], pf[U], ex[U], dw[U];
+#pragma unroll
+        for (int i = 0; i < U; i++) Gr[i] = 0.0f;
+        if (T <= TMAX) {
+            // every type's counts in one round of loads (types past T
+            // reload the last one and are not summed)
+            int32_t bv[TMAX][U], dv[TMAX][U];
+#pragma unroll
+            for (int t = 0; t < TMAX; t++) {
+                const int32_t tc = min(t, T - 1);
+#pragma unroll
+                for (int i = 0; i < U; i++) {
+                    bv[t][i] = base[tc * H + rr[i]];
+                    dv[t][i] = dn[tc * H + rr[i]];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < U; i++) {
+                sv[i] = c.shellV[rr[i]];
+                pf[i] = c.pref[rr[i]];
+                ex[i] = expv[rr[i]];
+                dw[i] = dataW[rr[i]];
+            }
+            // pair types in order: Gr += wij*nij/Dij with nij integer-valued (exact in float32)
+#pragma unroll
+            for (int t = 0; t < TMAX; t++) {
+                if (t >= T) break;
+                const float wt = c.w[t], Dt = c.D[t];
+#pragma unroll
+                for (int i = 0; i < U; i++) {
+                    const float nij = (float)(bv[t][i] + dv[t][i]);
+                    float term = wt * nij;
+                    term = term / Dt;
+                    Gr[i] = Gr[i] + term;
+                }
+            }
+        } else {
+            int32_t bv[U], dv[U];
+#pragma unroll
+            for (int i = 0; i < U; i++) {
+                bv[i] = base[rr[i]];
+                dv[i] = dn[rr[i]];
+            }
+            // the per-bin constants load with the first counts
+#pragma unroll
+            for (int i = 0; i < U; i++) {
+                sv[i] = c.shellV[rr[i]];
+                pf[i] = c.pref[rr[i]];
+                ex[i] = expv[rr[i]];
+                dw[i] = dataW[rr[i]];
+            }
+            // pair types in order; the next type's counts load while this one is summed
+            for (int32_t t = 0; t < T; t++) {
+                const int32_t tn = min(t + 1, T - 1);
+                int32_t nb[U], nd[U];
+#pragma unroll
+                for (int i = 0; i < U; i++) {
+                    nb[i] = base[tn * H + rr[i]];
+                    nd[i] = dn[tn * H + rr[i]];
+                }
+                const float wt = c.w[t], Dt = c.D[t];
+#pragma unroll
+                for (int i = 0; i < U; i++) {
+                    // Gr += wij*nij/Dij with nij integer-valued (exact in float32)
+                    const float nij = (float)(bv[i] + dv[i]);
+                    float term = wt * nij;
+                    term = term / Dt;
+                    Gr[i] = Gr[i] + term;
+                    bv[i] = nb[i];
+                    dv[i] = nd[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < U; i++) {
+            float g = Gr[i] / sv[i];           // Gr /= shellVolumes
+            float G = pf[i] * (g - 1.0f);      // (4*PI*r*rho0)*(Gr-1)
+            float out = G;
+            if (rsp) {
+                float model;
+                if (c.kind == FRMC_PCF) {
+                    // g(r); a scale factor acts on G(r): g = 1 + (sf*G)/(4*PI*r*rho0)
+                    model = g;
+                    if (c.hasSF) {
+                        const float Gsf = G * c.sf;
+                        model = 1.0f + Gsf / pf[i];
+                    }
+                } else {
+                    model = c.hasSF ? G * c.sf : G;
+                }
+                const float diff = ex[i] - model;
+                float v = diff * diff;
+                if (c.hasW) v = dw[i] * v;
+                out = v;
+            }
+            if (r0 + tid + NT * i < H) Gs[r0 + tid + NT * i] = out;
+        }
+    }
+}
+
 // Work unit u of proposal p.  A unit is one r-space constraint (PDC, PCF) or
 // one leaf of NumPy's pairwise sum over an S(Q) constraint's Q points (at
 // most 128: waves 0 and 1 run the S(Q) sums of its first and second 64).
@@ -511,10 +632,9 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
     while (ci + 1 < V.C && u >= V.c[ci + 1].uOff) ci++;
     // constraint fields in registers: keeps every load below unconditional
     const DevC c = V.c[ci];
-    const int32_t H = c.H, M = c.M, T = V.T;
+    const int32_t H = c.H, M = c.M;
     const bool rsp = c.rspace;
     const int32_t *dn = dnw + (int64_t)p * V.dnStride + c.dnOff;
-    const int32_t *base = c.base;
     // S(Q) unit: leaf l (Q points [q0, q0 + len), lane q = q0 + 64 w + lane,
     // waves 0 and 1), or 64 Q points from q0 (wave 0)
     int32_t leaf = 0, q0 = 0, len = 0;
@@ -562,118 +682,7 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
         ex_q = c.expv[min(q, M - 1)];
         if (c.hasW) dw_q = c.dataW[min(q, M - 1)];
     }
-    {
-        // U bins per lane per pass; every load is unconditional (clamped index,
-        // valid pointer): a load under a divergent branch is followed by its
-        // own wait, which serialises the batch
-        constexpr int NT = kEvalThreads;
-        constexpr int TMAX = 3;  // pair types (two elements) whose counts load in one round
-        const float *expv = rsp ? c.expv : c.shellV;                 // S(Q): unused (expv has M entries)
-        const float *dataW = (rsp && c.hasW) ? c.dataW : c.shellV;   // unused unless hasW
-        for (int32_t r0 = 0; r0 < H; r0 += NT * U) {
-            int32_t rr[U];
-#pragma unroll
-            for (int i = 0; i < U; i++) rr[i] = min(r0 + tid + NT * i, H - 1);
-            float Gr[U];
-            float sv[U], pf[U], ex[U], dw[U];
-#pragma unroll
-            for (int i = 0; i < U; i++) Gr[i] = 0.0f;
-            if (T <= TMAX) {
-                // every type's counts in one round of loads (types past T
-                // reload the last one and are not summed)
-                int32_t bv[TMAX][U], dv[TMAX][U];
-#pragma unroll
-                for (int t = 0; t < TMAX; t++) {
-                    const int32_t tc = min(t, T - 1);
-#pragma unroll
-                    for (int i = 0; i < U; i++) {
-                        bv[t][i] = base[tc * H + rr[i]];
-                        dv[t][i] = dn[tc * H + rr[i]];
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < U; i++) {
-                    sv[i] = c.shellV[rr[i]];
-                    pf[i] = c.pref[rr[i]];
-                    ex[i] = expv[rr[i]];
-                    dw[i] = dataW[rr[i]];
-                }
-                // pair types in order: Gr += wij*nij/Dij with nij integer-valued (exact in float32)
-#pragma unroll
-                for (int t = 0; t < TMAX; t++) {
-                    if (t >= T) break;
-                    const float wt = c.w[t], Dt = c.D[t];
-#pragma unroll
-                    for (int i = 0; i < U; i++) {
-                        const float nij = (float)(bv[t][i] + dv[t][i]);
-                        float term = wt * nij;
-                        term = term / Dt;
-                        Gr[i] = Gr[i] + term;
-                    }
-                }
-            } else {
-                int32_t bv[U], dv[U];
-#pragma unroll
-                for (int i = 0; i < U; i++) {
-                    bv[i] = base[rr[i]];
-                    dv[i] = dn[rr[i]];
-                }
-                // the per-bin constants load with the first counts
-#pragma unroll
-                for (int i = 0; i < U; i++) {
-                    sv[i] = c.shellV[rr[i]];
-                    pf[i] = c.pref[rr[i]];
-                    ex[i] = expv[rr[i]];
-                    dw[i] = dataW[rr[i]];
-                }
-                // pair types in order; the next type's counts load while this one is summed
-                for (int32_t t = 0; t < T; t++) {
-                    const int32_t tn = min(t + 1, T - 1);
-                    int32_t nb[U], nd[U];
-#pragma unroll
-                    for (int i = 0; i < U; i++) {
-                        nb[i] = base[tn * H + rr[i]];
-                        nd[i] = dn[tn * H + rr[i]];
-                    }
-                    const float wt = c.w[t], Dt = c.D[t];
-#pragma unroll
-                    for (int i = 0; i < U; i++) {
-                        // Gr += wij*nij/Dij with nij integer-valued (exact in float32)
-                        const float nij = (float)(bv[i] + dv[i]);
-                        float term = wt * nij;
-                        term = term / Dt;
-                        Gr[i] = Gr[i] + term;
-                        bv[i] = nb[i];
-                        dv[i] = nd[i];
-                    }
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < U; i++) {
-                float g = Gr[i] / sv[i];           // Gr /= shellVolumes
-                float G = pf[i] * (g - 1.0f);      // (4*PI*r*rho0)*(Gr-1)
-                float out = G;
-                if (rsp) {
-                    float model;
-                    if (c.kind == FRMC_PCF) {
-                        // g(r); a scale factor acts on G(r): g = 1 + (sf*G)/(4*PI*r*rho0)
-                        model = g;
-                        if (c.hasSF) {
-                            const float Gsf = G * c.sf;
-                            model = 1.0f + Gsf / pf[i];
-                        }
-                    } else {
-                        model = c.hasSF ? G * c.sf : G;
-                    }
-                    const float diff = ex[i] - model;
-                    float v = diff * diff;
-                    if (c.hasW) v = dw[i] * v;
-                    out = v;
-                }
-                if (r0 + tid + NT * i < H) Gs[r0 + tid + NT * i] = out;
-            }
-        }
-    }
+    build_g<U, kEvalThreads>(V, c, dn, Gs);
     __syncthreads();
     if (rsp) {
         const float chi = block_pairwise(Gs, c, scratch, lprog);
@@ -861,6 +870,193 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_a(View V, int32_t k) {
         nblk = (int64_t)gridDim.x * gridDim.y;
     }
     eval_a_body<U, IL>(V, V.dn, p, u, sm, nullptr, b, nblk);
+}
+
+// ------------------------------------------------- k_eval_a, split chain
+
+// The leaf layout's S(Q) sums with the products computed apart from the
+// adds: in a block of kPcThreads threads, waves 0 and 1 (the chain waves,
+// Q points [q0, q0 + 64) and [q0 + 64, q0 + len) of the leaf) only add,
+// while the other fourteen waves (seven per chain wave) load the
+// interleaved matrix, multiply by G from LDS and store the products to an
+// LDS ring of two slots of kPcRows rows; one barrier per slot hands a slot
+// over.  A chain wave then issues one LDS read per four rows and the adds,
+// so its sum runs close to the latency of the dependent adds.  Every sum and
+// product is the one eval_a_body computes, in the same order.
+constexpr int kPcThreads = 1024;
+constexpr int kPcRows = 128;               // rows per ring slot
+constexpr int kPcGroups = kPcRows / 4;     // 4-row groups per slot
+constexpr int kPcProd = 7;                 // producer waves per chain wave
+constexpr int kPcPerProd = (kPcGroups + kPcProd - 1) / kPcProd;  // row groups per producer per slot
+constexpr int kPcAhead = 2;                // slots of matrix rows a producer has in flight
+constexpr int kPcRead = 8;                 // product groups a chain wave has in flight
+
+// LDS of k_eval_pc: G(r) (Hp floats), the ring (2 slots x 2 halves x
+// kPcGroups x 64 float4), then the leaf and pairwise scratch of eval_a_body
+size_t pc_lds_bytes(const View &V) {
+    const size_t hp = (size_t)((V.Hmax + 15) & ~15);
+    const size_t ring = (size_t)2 * 2 * kPcGroups * 64 * 4;
+    return sizeof(float) * (hp + ring + std::max<size_t>(V.maxLeaves + kStack, 128) + (size_t)2 * V.maxLeaves);
+}
+
+template <int U>
+__device__ __forceinline__ void eval_pc_body(const View &V, int32_t p, int32_t u, float *sm, int64_t b, int64_t nblk) {
+    const int32_t Hp = (V.Hmax + 15) & ~15;
+    float *Gs = sm;
+    float4 *ring = reinterpret_cast<float4 *>(sm + Hp);  // Hp is a multiple of 16: 16-byte aligned
+    float *scratch = sm + Hp + 2 * 2 * kPcGroups * 64 * 4;
+    int32_t *lprog = (int32_t *)(scratch + V.maxLeaves + kStack);
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    for (int64_t s = b * blockDim.x + tid; s < V.zero_len; s += nblk * blockDim.x) V.dn_zero[s] = 0;
+    int32_t ci = 0;
+    while (ci + 1 < V.C && u >= V.c[ci + 1].uOff) ci++;
+    const DevC c = V.c[ci];
+    const int32_t H = c.H, M = c.M;
+    const int32_t *dn = V.dn + (int64_t)p * V.dnStride + c.dnOff;
+    if (c.rspace) {
+        build_g<U, kPcThreads>(V, c, dn, Gs);
+        __syncthreads();
+        const float chi = block_pairwise(Gs, c, scratch, lprog);
+        if (tid == 0) {
+            V.chi2d[(int64_t)p * V.C + ci] = chi;
+            publish(V.out_res + (int64_t)p * V.resW + 1 + ci, chi, V.seq);
+        }
+        return;
+    }
+    const int32_t leaf = u - c.uOff;
+    const int32_t q0 = c.leaves[2 * leaf], len = c.leaves[2 * leaf + 1];
+    const bool chain = w < 2;
+    const int32_t half = chain ? w : (w - 2) / kPcProd;  // the chain wave a wave works for
+    const int32_t pw = chain ? 0 : (w - 2) % kPcProd;     // a producer's rank among its seven
+    const int32_t q = q0 + 64 * half + lane;
+    const int32_t qc = min(q, M - 1);
+    const float *mil = c.sqi + 4 * (int64_t)qc;           // M[r][qc] at mil[(r/4)*4M + r%4]
+    const int32_t ngroups = c.hpb / 4;
+    const int32_t nslot = (H + kPcRows - 1) / kPcRows;
+    float4 mreg[kPcAhead + 1][kPcPerProd];
+    auto load_slot = [&](float4 *mr, int32_t s) {
+#pragma unroll
+        for (int j = 0; j < kPcPerProd; j++) {
+            const int32_t rg = min(pw + kPcProd * j, kPcGroups - 1);
+            const int32_t grow = min(s * kPcGroups + rg, ngroups - 1);
+            mr[j] = *reinterpret_cast<const float4 *>(mil + (uint32_t)(grow * 4 * M));
+        }
+    };
+    if (!chain) {
+#pragma unroll
+        for (int s = 0; s < kPcAhead; s++) load_slot(mreg[s], s);
+    }
+    float ex_q = 0.0f, dw_q = 1.0f;
+    if (chain) {
+        ex_q = c.expv[qc];
+        if (c.hasW) dw_q = c.dataW[qc];
+    }
+    build_g<U, kPcThreads>(V, c, dn, Gs);
+    __syncthreads();
+    if (chain) {
+        // S(Q) from +0, as NumPy's add reduction does
+        float acc = 0.0f;
+        for (int32_t s = 0; s < nslot; s++) {
+            __syncthreads();  // slot s written
+            const float4 *src = ring + ((s & 1) * 2 + half) * kPcGroups * 64;
+            float4 v[kPcRead];
+#pragma unroll
+            for (int i = 0; i < kPcRead; i++) v[i] = src[i * 64 + lane];
+            const int32_t rows = min(kPcRows, H - s * kPcRows);
+            if (rows == kPcRows) {
+#pragma unroll
+                for (int g = 0; g < kPcGroups; g++) {
+                    const float4 x = v[g % kPcRead];
+                    if (g + kPcRead < kPcGroups) v[g % kPcRead] = src[(g + kPcRead) * 64 + lane];
+                    acc = acc + x.x;
+                    acc = acc + x.y;
+                    acc = acc + x.z;
+                    acc = acc + x.w;
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < kPcGroups; g++) {
+                    const float4 x = v[g % kPcRead];
+                    if (g + kPcRead < kPcGroups) v[g % kPcRead] = src[(g + kPcRead) * 64 + lane];
+                    if (4 * g + 0 < rows) acc = acc + x.x;
+                    if (4 * g + 1 < rows) acc = acc + x.y;
+                    if (4 * g + 2 < rows) acc = acc + x.z;
+                    if (4 * g + 3 < rows) acc = acc + x.w;
+                }
+            }
+        }
+        if (64 * half + lane < len) {
+            float S;
+            if (c.kind == FRMC_RSFC) {
+                S = c.hasSF ? c.sf * acc : acc;
+            } else {
+                S = acc + 1.0f;
+                if (c.hasSF) {
+                    float t = S - 1.0f;
+                    t = c.sf * t;
+                    S = t + 1.0f;
+                }
+            }
+            const float diff = ex_q - S;
+            float v = diff * diff;
+            if (c.hasW) v = dw_q * v;
+            scratch[64 * half + lane] = v;
+        }
+    } else {
+        // slot s in round s, the matrix rows of slot s + kPcAhead loading
+        for (int32_t s0 = 0; s0 < nslot; s0 += kPcAhead + 1) {
+#pragma unroll
+            for (int i = 0; i < kPcAhead + 1; i++) {
+                const int32_t s = s0 + i;
+                if (s < nslot) {
+                    load_slot(mreg[(i + kPcAhead) % (kPcAhead + 1)], s + kPcAhead);
+                    float4 *dst = ring + ((s & 1) * 2 + half) * kPcGroups * 64;
+#pragma unroll
+                    for (int j = 0; j < kPcPerProd; j++) {
+                        const int32_t rg = pw + kPcProd * j;
+                        if (rg < kPcGroups) {
+                            const int32_t r = min(s * kPcRows + 4 * rg, Hp - 4);
+                            const float4 gv = *reinterpret_cast<const float4 *>(Gs + r);
+                            const float4 mv = mreg[i][j];
+                            float4 pr;
+                            pr.x = gv.x * mv.x;
+                            pr.y = gv.y * mv.y;
+                            pr.z = gv.z * mv.z;
+                            pr.w = gv.w * mv.w;
+                            dst[rg * 64 + lane] = pr;
+                        }
+                    }
+                    __syncthreads();  // hands slot s to the chain waves
+                }
+            }
+        }
+    }
+    // the leaf's sum in NumPy's order (the host combines the leaves)
+    __syncthreads();
+    if (tid == 0) publish(V.out_res + (int64_t)p * V.resW + c.leafOff + leaf, leaf_sum(scratch, len), V.seq);
+}
+
+// grid as k_eval_a's (units x proposals, or XCD-aligned), blocks of
+// kPcThreads; leaf-layout windows only
+template <int U, bool XCD>
+__global__ void __launch_bounds__(kPcThreads) k_eval_pc(View V, int32_t k) {
+    extern __shared__ float sm[];
+    int32_t u, p;
+    int64_t b, nblk;
+    if (XCD) {
+        const int32_t nu8 = (V.nunits + 7) & ~7;
+        u = blockIdx.x % nu8;
+        p = blockIdx.x / nu8;
+        b = (int64_t)p * V.nunits + u;
+        nblk = (int64_t)V.nunits * k;
+        if (u >= V.nunits) return;
+    } else {
+        u = blockIdx.x;
+        p = blockIdx.y;
+        b = (int64_t)p * gridDim.x + u;
+        nblk = (int64_t)gridDim.x * gridDim.y;
+    }
+    eval_pc_body<U>(V, p, u, sm, b, nblk);
 }
 
 // ----------------------------------------------------------- batched windows
@@ -1297,11 +1493,33 @@ static void launch_k_eval_a_il(const View &V, int32_t k, hipStream_t st, bool u8
     }
 }
 
+template <int U, bool XCD>
+static void launch_k_eval_pc(const View &V, int32_t k, hipStream_t st) {
+    const size_t shm = pc_lds_bytes(V);
+    (void)hipFuncSetAttribute((const void *)k_eval_pc<U, XCD>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (XCD)
+        k_eval_pc<U, XCD><<<dim3((unsigned)((V.nunits + 7) & ~7) * (unsigned)k), kPcThreads, shm, st>>>(V, k);
+    else
+        k_eval_pc<U, XCD><<<dim3((unsigned)V.nunits, (unsigned)k), kPcThreads, shm, st>>>(V, k);
+}
+
 // shape s of k_eval_a: bins per thread per G(r) pass U = 8 if s & 4 (and the
 // grid has more than 1,024 bins) else 4; s & 8: XCD-aligned units; s & 32:
-// the interleaved S(Q) matrix (s & 16, the unit layout, is the caller's).
-// Every shape gives the same results.
+// the interleaved S(Q) matrix (s & 16, the unit layout, is the caller's);
+// s & 64: the split chain of k_eval_pc (leaf layout, interleaved matrix;
+// U = 2 or 4 bins per pass for its 1,024 threads).  Every shape gives the
+// same results.
 int launch_eval(const View &V, int32_t k, hipStream_t st, int32_t shape) {
+    if (shape & 64) {
+        const bool u4 = (shape & 4) && V.Hmax > 2 * kPcThreads;
+        if (shape & 8) {
+            if (u4) launch_k_eval_pc<4, true>(V, k, st); else launch_k_eval_pc<2, true>(V, k, st);
+        } else {
+            if (u4) launch_k_eval_pc<4, false>(V, k, st); else launch_k_eval_pc<2, false>(V, k, st);
+        }
+        FRMC_CHECK_LAUNCH();
+        return FRMC_OK;
+    }
     const bool u8 = (shape & 4) && V.Hmax > 4 * kEvalThreads;
     if (shape & 32) launch_k_eval_a_il<true>(V, k, st, u8, (shape & 8) != 0);
     else launch_k_eval_a_il<false>(V, k, st, u8, (shape & 8) != 0);

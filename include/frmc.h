@@ -459,13 +459,15 @@ int frmc_engine_kernel_times(frmc_engine *e, double *ms_out, int64_t *launches_o
    counterpart (measurement only). */
 int frmc_engine_time_eval(frmc_engine *e, int32_t k, int32_t reps, float *ms_out);
 /* Window-evaluation shape tuning.  No reference counterpart (the reference
-   evaluates one move at a time).  A shape s is a multiple of 4 in 0..60:
+   evaluates one move at a time).  A shape s is a multiple of 4 in 0..60,
+   or 80, 84, 88, 92 (s & 64: the split chain, leaf layout only):
    bins per thread per G(r) pass 8 (s & 4, above 1,024 bins) or 4; work
    units aligned to the XCDs (s & 8: unit u on XCD u mod 8); S(Q) units of
    one pairwise-sum leaf whose sums the host combines (s & 16, no k_eval_b)
    or of 64 Q points; the S(Q) matrix read from its copy interleaved by
-   four rows (s & 32) or its transposed copy.  Every shape gives the same
-   results.  An engine times
+   four rows (s & 32) or its transposed copy; with s & 64, 1,024-thread
+   workgroups in which two waves add the S(Q) products that fourteen others
+   compute into LDS.  Every shape gives the same results.  An engine times
    every shape for each window-size bucket ceil(log2 k) on its first window
    and keeps the fastest (cached per device, constraint layout, pair types
    and largest window).  frmc_set_eval_shape forces one shape for every

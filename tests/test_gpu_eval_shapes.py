@@ -1,7 +1,8 @@
 """Every window-evaluation shape (bins per thread 4 or 8; work units spread
 over the XCDs or aligned to them; S(Q) units of 64 Q points with k_eval_b,
 or pairwise-sum leaves whose sums the host combines; the S(Q) matrix in its
-transposed or its four-row interleaved copy) evaluates a window to
+transposed or its four-row interleaved copy; the split chain, products in
+LDS from producer waves) evaluates a window to
 the same bits: forced in turn, each gives the oracle's trajectory; the tuned
 engine reports a shape per window-size bucket and also matches."""
 import ctypes
@@ -38,7 +39,7 @@ def case():
     return s, pdc, sfc, oe, olog, pys, nps
 
 
-@pytest.mark.parametrize("shape", list(range(0, 64, 4)) + [-1])
+@pytest.mark.parametrize("shape", list(range(0, 64, 4)) + [80, 84, 88, 92, -1])
 def test_every_eval_shape_gives_the_oracle_trajectory(case, shape):
     s, pdc, sfc, oe, olog, pys, nps = case
     L = _lib.lib()
@@ -54,7 +55,7 @@ def test_every_eval_shape_gives_the_oracle_trajectory(case, shape):
         assert_same_state(E, oe, dlog, olog, (pys, nps))
         got = (ctypes.c_int32 * 6)()
         _lib.check(L.frmc_engine_eval_shapes(E.device_handle(), got, 6))
-        assert all(g in range(0, 64, 4) for g in got)
+        assert all(g in list(range(0, 64, 4)) + [80, 84, 88, 92] for g in got)
         if shape >= 0:
             assert list(got) == [shape] * 6
         E._drop_device()
@@ -64,7 +65,7 @@ def test_every_eval_shape_gives_the_oracle_trajectory(case, shape):
 
 def test_eval_shape_setter_rejects_unknown_shapes():
     L = _lib.lib()
-    for bad in (1, 2, 3, 5, 6, 7, -2, 9, 30, 62, 64):
+    for bad in (1, 2, 3, 5, 6, 7, -2, 9, 30, 62, 64, 68, 76, 96, 112):
         with pytest.raises(ValueError):
             _lib.check(L.frmc_set_eval_shape(bad))
 
@@ -81,7 +82,7 @@ def small_box():
     return systems.random_box(300, 15.0, 1.6, seed=91, species=("Si", "O"), fractions=(1 / 3, 2 / 3))
 
 
-@pytest.mark.parametrize("shape", [0, 16, 32, 48])
+@pytest.mark.parametrize("shape", [0, 16, 32, 48, 80])
 @pytest.mark.parametrize("rows", SQ_ROWS)
 def test_sq_sum_over_every_row_tail(small_box, rows, shape):
     s = small_box
