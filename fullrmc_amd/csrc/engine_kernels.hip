@@ -877,8 +877,9 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_a(View V, int32_t k) {
 // The leaf layout's S(Q) sums with the products computed apart from the
 // adds: in a block of kPcThreads threads, waves 0 and 1 (the chain waves,
 // Q points [q0, q0 + 64) and [q0 + 64, q0 + len) of the leaf) only add,
-// while the other fourteen waves (seven per chain wave) load the
-// interleaved matrix, multiply by G from LDS and store the products to an
+// while the other fourteen waves load the interleaved matrix for both
+// halves, multiply by G from LDS (one read per row group for both) and
+// store the products to an
 // LDS ring of two slots of kPcRows rows; one barrier per slot hands a slot
 // over.  A chain wave then issues one LDS read per four rows and the adds,
 // so its sum runs close to the latency of the dependent adds.  Every sum and
@@ -886,7 +887,7 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_a(View V, int32_t k) {
 constexpr int kPcThreads = 1024;
 constexpr int kPcRows = 128;               // rows per ring slot
 constexpr int kPcGroups = kPcRows / 4;     // 4-row groups per slot
-constexpr int kPcProd = 7;                 // producer waves per chain wave
+constexpr int kPcProd = 14;                // producer waves (each serves both chain waves)
 constexpr int kPcPerProd = (kPcGroups + kPcProd - 1) / kPcProd;  // row groups per producer per slot
 constexpr int kPcAhead = 2;                // slots of matrix rows a producer has in flight
 constexpr int kPcRead = 8;                 // product groups a chain wave has in flight
@@ -926,20 +927,23 @@ __device__ __forceinline__ void eval_pc_body(const View &V, int32_t p, int32_t u
     const int32_t leaf = u - c.uOff;
     const int32_t q0 = c.leaves[2 * leaf], len = c.leaves[2 * leaf + 1];
     const bool chain = w < 2;
-    const int32_t half = chain ? w : (w - 2) / kPcProd;  // the chain wave a wave works for
-    const int32_t pw = chain ? 0 : (w - 2) % kPcProd;     // a producer's rank among its seven
+    const int32_t half = chain ? w : 0;                   // a chain wave's half of the leaf
+    const int32_t pw = chain ? 0 : w - 2;                 // a producer's rank
     const int32_t q = q0 + 64 * half + lane;
     const int32_t qc = min(q, M - 1);
     const float *mil = c.sqi + 4 * (int64_t)qc;           // M[r][qc] at mil[(r/4)*4M + r%4]
+    // a producer serves both halves: lane l computes Q points q0 + l and q0 + 64 + l
+    const float *mil1 = c.sqi + 4 * (int64_t)min(q0 + 64 + lane, M - 1);
     const int32_t ngroups = c.hpb / 4;
     const int32_t nslot = (H + kPcRows - 1) / kPcRows;
-    float4 mreg[kPcAhead + 1][kPcPerProd];
+    float4 mreg[kPcAhead + 1][2 * kPcPerProd];
     auto load_slot = [&](float4 *mr, int32_t s) {
 #pragma unroll
         for (int j = 0; j < kPcPerProd; j++) {
             const int32_t rg = min(pw + kPcProd * j, kPcGroups - 1);
-            const int32_t grow = min(s * kPcGroups + rg, ngroups - 1);
-            mr[j] = *reinterpret_cast<const float4 *>(mil + (uint32_t)(grow * 4 * M));
+            const uint32_t off = (uint32_t)(min(s * kPcGroups + rg, ngroups - 1) * 4 * M);
+            mr[2 * j] = *reinterpret_cast<const float4 *>(mil + off);
+            mr[2 * j + 1] = *reinterpret_cast<const float4 *>(mil1 + off);
         }
     };
     if (!chain) {
@@ -1010,20 +1014,23 @@ __device__ __forceinline__ void eval_pc_body(const View &V, int32_t p, int32_t u
                 const int32_t s = s0 + i;
                 if (s < nslot) {
                     load_slot(mreg[(i + kPcAhead) % (kPcAhead + 1)], s + kPcAhead);
-                    float4 *dst = ring + ((s & 1) * 2 + half) * kPcGroups * 64;
+                    float4 *dst = ring + (s & 1) * 2 * kPcGroups * 64;
 #pragma unroll
                     for (int j = 0; j < kPcPerProd; j++) {
                         const int32_t rg = pw + kPcProd * j;
                         if (rg < kPcGroups) {
                             const int32_t r = min(s * kPcRows + 4 * rg, Hp - 4);
                             const float4 gv = *reinterpret_cast<const float4 *>(Gs + r);
-                            const float4 mv = mreg[i][j];
-                            float4 pr;
-                            pr.x = gv.x * mv.x;
-                            pr.y = gv.y * mv.y;
-                            pr.z = gv.z * mv.z;
-                            pr.w = gv.w * mv.w;
-                            dst[rg * 64 + lane] = pr;
+#pragma unroll
+                            for (int hh = 0; hh < 2; hh++) {
+                                const float4 mv = mreg[i][2 * j + hh];
+                                float4 pr;
+                                pr.x = gv.x * mv.x;
+                                pr.y = gv.y * mv.y;
+                                pr.z = gv.z * mv.z;
+                                pr.w = gv.w * mv.w;
+                                dst[(hh * kPcGroups + rg) * 64 + lane] = pr;
+                            }
                         }
                     }
                     __syncthreads();  // hands slot s to the chain waves
