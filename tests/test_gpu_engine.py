@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from fullrmc_amd import systems
-from fullrmc_amd.Constraints import (InterMolecularDistanceConstraint, PairCorrelationConstraint,
+from fullrmc_amd.Constraints import (InterMolecularDistanceConstraint, IntraMolecularDistanceConstraint,
+                                     PairCorrelationConstraint,
                                      PairDistributionConstraint, ReducedStructureFactorConstraint,
                                      StructureFactorConstraint)
 from fullrmc_amd.Engine import Engine
@@ -314,6 +315,38 @@ def test_intermolecular_distance_constraint_vetoes_in_dropin_loop(small_triclini
     assert oc.sd > 0
     with pytest.raises(NotImplementedError):
         E.run_device(10)
+
+
+@pytest.mark.parametrize("flexible", [True, False])
+def test_intramolecular_distance_constraint_vetoes_in_dropin_loop(small_triclinic, flexible):
+    """IntraMolecularDistanceConstraint (rigid, element types) with a PDC,
+    through the per-move drop-in loop: single-atom moves inside the rigid
+    rings bring H-H and C-H pairs of one molecule under their limits, and the
+    constraint vetoes exactly the moves the oracle vetoes (reference
+    DistanceConstraints.py:1093-1210)."""
+    s = small_triclinic
+    exp = pdc_target(s, 0.1, 33, 0.025, 0.05, 200)
+    n = len(s["real"])
+    limits = {("H", "H"): 1.7, ("C", "H"): 1.05}
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], oeng.Translation(0.4), _pdc_cons_of({"exp": exp}),
+                                    120, 19,
+                                    rigid_fn=lambda o: [oeng.IMD(o, 1.0, limits, flexible=flexible, inter=False)])
+    random.seed(19)
+    np.random.seed(19)
+    E = device_engine(s, [PairDistributionConstraint(exp),
+                          IntraMolecularDistanceConstraint(1.0, typeDefinition="element",
+                                                           pairsDistanceDefinition=[("H", "H", 1.7),
+                                                                                    ("C", "H", 1.05)],
+                                                           flexible=flexible)])
+    E.set_move_generator(TranslationGenerator(0.4))
+    dlog = []
+    E.run(120, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    c, oc = E.rigidConstraints[0], oe.rigid[0]
+    assert c.data["number"].tobytes() == oc.number.tobytes()
+    assert c.data["distanceSum"].tobytes() == oc.dsum.tobytes()
+    assert np.float32(c.standardError).tobytes() == np.float32(oc.sd).tobytes()
+    assert E.tried < E.generated  # some moves were vetoed before being tried
 
 
 def test_swap_moves_in_dropin_loop():
