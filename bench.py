@@ -619,6 +619,9 @@ def engine_timed_run(args, E, n, shape, traffic_profile, steps, moves, k, log_fi
                                    "(engine stream)")
         roof["in_run_us_with_events"] = kernels["k_eval_a"]["us_avg_with_events"]
         roof["tuned_shape_per_bucket"] = list(shapes)
+        sh = shapes[min(7, max(0, (kwin - 1).bit_length()))]
+        roof["shape"] = sh
+        roof["kernel_instance"] = "k_eval_pc" if sh & 64 else "k_eval_a"
         roof["us_per_shape_at_mean_k"] = per_shape
     idel = _lib.KERNEL_NAMES.index("k_delta")
     roof_delta = kernel_roofline("k_delta", ms[idel] * 1e-3 / max(1, launches[idel]), n, kmean, shape)
