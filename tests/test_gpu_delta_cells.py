@@ -158,3 +158,41 @@ def test_chunk_walk_off_and_by_size(cubic, cells_mode):
     assert chunks_of(E)[0] == 0
     with pytest.raises(ValueError):
         _lib.check(_lib.lib().frmc_set_delta_cells(2))
+
+
+def test_chunk_walk_survives_batched_replica_runs(cubic, cells_mode):
+    """Native windows, then a batched replica run (whose kernels do not keep
+    the sorted copy), then native windows again: the engine rebuilds the copy
+    and ends in exactly the state of a twin engine without the chunk walk
+    (whose path the other engine tests check against the oracle)."""
+    from fullrmc_amd.Engine import run_replicas
+    s = cubic
+    small = systems.random_box(400, 20.0, 1.8, seed=71, species=("Si", "O"), fractions=(1 / 3, 2 / 3))
+    small["exp"] = pdc_target(small, 0.1, 72, 0.025, 0.05, 120)
+
+    def engine(box, mode):
+        cells_mode(mode)
+        E = device_engine(box, [PairDistributionConstraint(box["exp"])])
+        E.set_move_generator(TranslationGenerator(0.2))
+        return E
+
+    finals = []
+    for mode in (1, 0):
+        E = engine(s, mode)
+        F = engine(small, mode)
+        random.seed(73)
+        np.random.seed(73)
+        E.run_device(300, k=16)
+        run_replicas([E, F], 200, k=16, seeds=[74, 75])
+        random.seed(76)
+        np.random.seed(76)
+        E.run_device(300, k=16)
+        assert (chunks_of(E)[0] > 0) == (mode == 1)
+        finals.append((E.realCoordinates.tobytes(), E.boxCoordinates.tobytes(),
+                       np.float32(E.totalStandardError).tobytes(),
+                       [(c.data["intra"].tobytes(), c.data["inter"].tobytes()) for c in E.constraints],
+                       E.accepted))
+        E._drop_device()
+        F._drop_device()
+    assert finals[0] == finals[1]
+    assert finals[0][4] > 100
