@@ -196,3 +196,25 @@ def test_chunk_walk_survives_batched_replica_runs(cubic, cells_mode):
         F._drop_device()
     assert finals[0] == finals[1]
     assert finals[0][4] > 100
+
+
+def test_far_unwrapped_coordinates_turn_the_chunk_walk_off(cells_mode):
+    """Box coordinates beyond 64 box lengths would need a larger rounding
+    margin than the chunk test allows: the engine keeps no sorted copy and
+    still gives the oracle's trajectory."""
+    s = systems.random_box(600, 18.0, 1.8, seed=81, species=("Si", "O"), fractions=(1 / 3, 2 / 3))
+    s["exp"] = pdc_target(s, 0.1, 82, 0.025, 0.05, 120)
+    s = dict(s)
+    s["real"] = (s["real"] + np.float32(18.0 * 100)).astype(F32)  # 100 cells away along every axis
+    n = len(s["real"])
+    cons = lambda osys, w: [oeng.PDC(osys, s["exp"], w)]
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], oeng.Translation(0.2), cons, 300, 83)
+    cells_mode(1)
+    random.seed(83)
+    np.random.seed(83)
+    E = device_engine(s, [PairDistributionConstraint(s["exp"])])
+    E.set_move_generator(TranslationGenerator(0.2))
+    dlog = []
+    E.run_device(300, k=16, log=dlog)
+    assert chunks_of(E)[0] == 0
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
