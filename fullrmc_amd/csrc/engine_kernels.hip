@@ -1272,7 +1272,10 @@ __device__ __forceinline__ void delta_chunk_any(const View &V, const BatchWin &W
     else delta_chunk<PBC, ORTHO, false>(V, W, props, pb, ch, nch, bins);
 }
 
-__global__ void __launch_bounds__(256) k_delta_batch(BatchArgs A) {
+// at most 128 VGPRs (four waves per SIMD): the unconstrained 130 allowed
+// three, and 256 replicas ran 8 % faster with four (3.57M against 3.30M
+// moves/s; five waves need a register spill)
+__global__ void __launch_bounds__(256, 4) k_delta_batch(BatchArgs A) {
     extern __shared__ int32_t bins[];  // kBP delta blocks (the largest that fits kBatchLdsInts)
     __shared__ BatchLds S;
     __shared__ Prop sp[kBP];
