@@ -763,6 +763,8 @@ def main():
                     help="moves of the 6,500-atom molecular triclinic box (0: skip)")
     ap.add_argument("--eval-shape", type=int, default=-1,
                     help="force one window-evaluation shape (0..60 step 4; -1: tuned per window size)")
+    ap.add_argument("--window-loop", type=int, default=-1, choices=(-1, 0, 1),
+                    help="device window loop for single-engine runs: -1 default (on), 0 off (host walks and draws), 1 on")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -784,6 +786,7 @@ def main():
     from fullrmc_amd._device import require_gpu
     require_gpu()
     _lib.check(_lib.lib().frmc_set_eval_shape(args.eval_shape))
+    _lib.check(_lib.lib().frmc_set_window_loop(args.window_loop))
 
     if args.copy_calibration:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.int32, device="cuda").fill_(1)

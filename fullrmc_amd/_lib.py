@@ -138,6 +138,8 @@ _PROTOS = {
     "frmc_engine_eval_shapes": [c_vp, c_i32p, c_i32],
     "frmc_set_delta_cells": [c_i32],
     "frmc_engine_delta_cells": [c_vp, c_i32p, c_f32p],
+    "frmc_set_window_loop": [c_i32],
+    "frmc_engine_loop_windows": [c_vp, ctypes.POINTER(c_i64)],
     "frmc_engine_kernel_times": [c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64)],
     "frmc_engine_set_adaptive": [c_vp, c_int],
 }

@@ -21,6 +21,7 @@
 #include <thread>
 #include <vector>
 
+#include "draws.h"
 #include "engine.h"
 
 namespace frmc {
@@ -31,30 +32,21 @@ struct Stream {
     frmc_mt19937 *s;
     int32_t used;
     inline uint32_t next() {
-        if (s->pos >= 624) {
-            for (int i = 0; i < 624; i++) {
-                uint32_t y = (s->mt[i] & 0x80000000u) | (s->mt[(i + 1) % 624] & 0x7fffffffu);
-                s->mt[i] = s->mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
-            }
+        if (s->pos >= kMtN) {
+            for (int i = 0; i < kMtN; i++) s->mt[i] = s->mt[(i + 397) % kMtN] ^ mt_mix(s->mt[i], s->mt[(i + 1) % kMtN]);
             s->pos = 0;
         }
-        uint32_t y = s->mt[s->pos++];
-        y ^= (y >> 11);
-        y ^= (y << 7) & 0x9d2c5680u;
-        y ^= (y << 15) & 0xefc60000u;
-        y ^= (y >> 18);
         used++;
-        return y;
+        return mt_temper(s->mt[s->pos++]);
     }
     // random.random() and RandomState.random_sample(): 53-bit double
     inline double uniform() {
-        uint32_t a = next() >> 5, b = next() >> 6;
-        return (a * 67108864.0 + b) * (1.0 / 9007199254740992.0);
+        const uint32_t a = next(), b = next();
+        return mt_uniform(a, b);
     }
     // random.randint(0, n-1) == randrange(0, n) -> _randbelow_with_getrandbits(n)
     inline int32_t randbelow(int32_t n) {
-        int k = 0;
-        while (k < 32 && ((uint32_t)1 << k) <= (uint32_t)n) k++;  // n.bit_length()
+        const int k = bit_length(n);
         uint32_t r;
         do {
             r = next() >> (32 - k);
@@ -63,39 +55,14 @@ struct Stream {
     }
 };
 
-// np.linalg.norm of a float32 3-vector: x.dot(x) goes through BLAS sdot,
-// which rounds each product to float32 and accumulates in double; NumPy
-// casts the sum to float32 and takes the float32 sqrt.
-static inline float np_norm3(const float v[3]) {
-    const float p0 = v[0] * v[0], p1 = v[1] * v[1], p2 = v[2] * v[2];
-    double acc = 0.0;
-    acc += (double)p0;
-    acc += (double)p1;
-    acc += (double)p2;
-    return sqrtf((float)acc);
-}
-
 // generate_random_vector(minAmp, maxAmp) (Core/Collection.py:674-701)
 static inline void random_vector(Stream &py, Stream &np_, float minAmp, float maxAmp, float v[3]) {
     float norm;
+    double u[3];
     do {
-        for (int d = 0; d < 3; d++) v[d] = (float)(1.0 - 2.0 * np_.uniform());
-        norm = np_norm3(v);
-    } while (norm == 0.0f);
-    for (int d = 0; d < 3; d++) v[d] = v[d] / norm;
-    float base[3];
-    for (int d = 0; d < 3; d++) base[d] = v[d] * minAmp;
-    const float amp = maxAmp - minAmp;
-    const float scale = (float)py.uniform() * amp;  // Python float * float32 -> float32
-    for (int d = 0; d < 3; d++) v[d] = v[d] * scale;
-    for (int d = 0; d < 3; d++) v[d] = v[d] + base[d];
-}
-
-// transform_coordinates (boundary_conditions_collection.pyx:105-108)
-static inline void to_box(const float M[9], const float r[3], float b[3]) {
-    b[0] = r[0] * M[0] + r[1] * M[3] + r[2] * M[6];
-    b[1] = r[0] * M[1] + r[1] * M[4] + r[2] * M[7];
-    b[2] = r[0] * M[2] + r[1] * M[5] + r[2] * M[8];
+        for (int d = 0; d < 3; d++) u[d] = np_.uniform();
+    } while (!random_direction(u, v, norm));
+    scale_direction(v, norm, py.uniform(), minAmp, maxAmp);
 }
 
 constexpr int32_t kReplicaThreads = 16;  // host threads driving independent replicas
@@ -204,6 +171,46 @@ struct TranslationRun {
         }
         aoff[kw] = off;
         for (int32_t p = 0; p < kw; p++) uf[p] = (float)uacc[p];
+    }
+
+    // Batches of windows on the device loop while a whole window is left:
+    // k is fixed per batch (from the acceptance rate so far), and batches
+    // start small so that it adapts early.  The host loop runs what is left.
+    int run_loop() {
+        if (!loop_usable(e, k)) return FRMC_OK;
+        if (int rc = loop_begin(e, *py_state, *np_state)) return rc;
+        int32_t ramp = 16;
+        while (!done()) {
+            const int32_t kk = e->adaptive ? window_size(gen0, acc0, k) : k;
+            const int64_t fit = (nsteps - step) / kk;
+            int32_t M = 0;
+            if (int rc = loop_plan(e, maxAmp, (int32_t)std::min<int64_t>(std::min(ramp, kLoopWindows), fit), M))
+                return rc;
+            if (M < 1) break;
+            if (int rc = loop_batch(e, kk, M, minAmp, maxAmp, tolerance)) return rc;
+            const LoopState &L = *e->h_loop;
+            const int64_t walked = L.h.step;
+            stats->windows += M;
+            stats->evaluated += (int64_t)M * kk;
+            stats->generated += walked;
+            stats->tried += walked;
+            stats->accepted += L.h.nacc;
+            stats->tolerated += L.h.ntol;
+            gen0 += walked;
+            acc0 += L.h.nacc;
+            for (int64_t i = 0; i < walked; i++) {
+                if (group_log) group_log[step + i] = e->h_loop_glog[i];
+                if (accept_log) accept_log[step + i] = e->h_loop_alog[i];
+                if (tse_log) tse_log[step + i] = e->h_loop_tlog[i];
+            }
+            step += walked;
+            memcpy(py_state->mt, L.py.mt, sizeof(py_state->mt));
+            py_state->pos = L.py.pos;
+            memcpy(np_state->mt, L.np.mt, sizeof(np_state->mt));
+            np_state->pos = L.np.pos;
+            ramp = std::min(2 * ramp, kLoopWindows);
+        }
+        return FRMC_OK;
     }
 
     int finish() {
@@ -576,6 +583,7 @@ extern "C" int frmc_engine_run_translation(frmc_engine *e, int64_t nsteps, int32
     run.tse_log = tse_log;
     int rc = run.init();
     if (rc) return rc;
+    if ((rc = run.run_loop())) return rc;
     while (!run.done()) {
         if ((rc = run.launch())) return rc;
         if ((rc = run.finish())) return rc;

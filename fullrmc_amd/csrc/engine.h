@@ -67,6 +67,13 @@ struct Stage {
     float *d_pold = nullptr, *d_pnew = nullptr;
 };
 
+struct Prop {  // one single-atom proposal: atom, element, old and new box coordinates
+    int32_t atom, el;
+    float old[3], nw[3];
+};
+
+struct Pending;
+
 struct View {
     int32_t n, nel, T, C, isPBC, ortho;
     Basis B;
@@ -104,6 +111,11 @@ struct View {
     int32_t prune;                 // 1: this window's k_delta walks the chunks (single-atom proposals)
     float hsp[3];                  // lattice plane spacings (A) along the three box axes
     float wcut;                    // largest binned distance plus a margin for float rounding (A)
+    // device window loop: the pending commit is read from here (written by
+    // k_loop_step) instead of the kernel argument, and the window's
+    // single-atom proposals from the table props; null otherwise
+    const Pending *pend_dev;
+    const Prop *props;
 };
 
 // Windows of single-atom moves travel to k_delta in the kernel arguments
@@ -130,6 +142,88 @@ struct Pending {
     float old[3], nw[3];
     const int32_t *dn;
 };
+
+// Device window loop (window_loop.hip): the walk of a window's totals, the
+// rewind of both random streams and the next window's draws run on the
+// device (k_loop_step, one workgroup), queued with the windows' k_delta and
+// k_eval launches, so the host is off the per-window critical path.
+constexpr int32_t kLoopMaxK = 128;     // proposals per window
+constexpr int32_t kLoopWindows = 256;  // windows per batch (the host syncs once per batch)
+struct LoopMt {
+    uint32_t mt[624];
+    int32_t pos;
+};
+struct LoopAcc {  // an accepted move of the batch (host mirror)
+    int32_t atom;
+    float box[3], real[3];
+};
+
+// The device loop's state: both streams at the start of the window in
+// flight (or of the next one), the accepted state, the batch's counters and
+// the window in flight's proposals as its walk needs them.
+struct LoopHead {
+    float cur_tse;
+    float cur_chi2[kMaxC];
+    int32_t pad;
+    int64_t step;        // proposals walked in the batch (log index)
+    int32_t nacc, ntol;  // accepted and tolerated (accepted worse) moves of the batch
+    int32_t error;       // kLoopErr*: a draw the device loop does not make
+    int32_t kw;          // proposals of the window in flight (0: none)
+    Pending pend;        // accepted move the next k_delta commits
+};
+static_assert(sizeof(Pending) == 48 && sizeof(LoopHead) % 4 == 0 && sizeof(LoopHead) <= 4 * 64,
+              "the loop header is read as one word per lane of a wave");
+struct LoopState {
+    LoopMt py, np;
+    LoopHead h;
+    // per proposal of the window in flight (atom, element and box
+    // coordinates are in the proposal table k_delta reads)
+    int32_t gid[kLoopMaxK], slot[kLoopMaxK];
+    int32_t py_pre[kLoopMaxK], py_post[kLoopMaxK];  // Python-stream words used before / after the acceptance draw
+    float uacc[kLoopMaxK];                          // acceptance draws rounded to float32
+    float nreal[kLoopMaxK][3];
+};
+constexpr int32_t kLoopNodes = 96;   // S(Q) leaf-combine tree nodes (all constraints) k_loop_step stages
+constexpr int32_t kLoopLevels = 16;  // levels of those trees
+constexpr int32_t kLoopResW = 64;    // result words per proposal
+
+// k_loop_step's arguments: the walk of the window in flight (its totals in
+// the result words), the draw of the next window, the batch's logs
+struct LoopArgs {
+    LoopState *L;
+    int32_t reset;  // first step of a batch: the batch counters start at 0
+    // walk (wk proposals; 0: no window in flight)
+    int32_t wk, leafmode, resW, C, dnStride;
+    float tol;
+    const unsigned long long *res;
+    const int32_t *win_dn;  // the walked window's delta buffer
+    // NumPy's pairwise combine of the S(Q) leaf sums as a tree, nodes by
+    // level: node i adds row words x and y into row word z of a proposal's
+    // row (its rs words: the result words, then one per node); constraint
+    // c's chi-square is row word root[c]
+    const int4 *nodes;
+    int32_t nnodes, nlev, rs;
+    int32_t lev_off[kLoopLevels + 1];
+    int32_t rspace[kMaxC], root[kMaxC];
+    // draw (dk proposals; 0: none)
+    int32_t dk, G, nbits;
+    int32_t ident;  // group g is atom g (no group table lookups)
+    float minAmp, maxAmp;
+    float recip[9];
+    const int32_t *goff, *gatoms, *elem, *spos;
+    const float *xyz;
+    float *real;
+    Prop *props;  // the window's proposal table (read by the walk, rewritten by the draw)
+    // logs of the batch (index = proposal walked in the batch) and its accepted moves
+    int32_t *glog;
+    uint8_t *alog;
+    float *tlog;
+    LoopAcc *acc;
+};
+int launch_loop_step(const LoopArgs &A, hipStream_t st);
+
+constexpr int32_t kLoopErrDirection = 1;  // a zero-length direction draw (the reference would redraw)
+constexpr int32_t kLoopErrWords = 2;      // a window's draws past four state blocks of the Python stream
 
 // atoms per chunk of the sorted copy (one per thread of a k_delta block), the
 // most proposals per block of the chunked k_delta, and the blocks its launch
@@ -170,10 +264,6 @@ constexpr int32_t kChunk = 2048;
 // leave the latency-bound kernel short of waves)
 constexpr int32_t kBP = 1;
 constexpr int32_t kBatchLdsInts = 8192;  // largest kBP delta blocks counted in LDS there (32 KB)
-struct Prop {  // one single-atom proposal: atom, element, old and new box coordinates
-    int32_t atom, el;
-    float old[3], nw[3];
-};
 struct BatchWin {
     int32_t rep;             // replica (index into the views)
     int32_t k, prop;         // proposals, first of them in the proposal table
@@ -257,6 +347,24 @@ struct frmc_engine {
     float hsp[3] = {0, 0, 0};
     float wcut = 0.f;
     const int32_t *win_dn = nullptr; // delta buffer of the last launched window
+    // device window loop (allocated on first use): state, batch logs and
+    // accepted moves (device, with pinned host copies), real coordinates and
+    // the windows' results on the device
+    frmc::LoopState *loop = nullptr, *h_loop = nullptr;
+    int32_t *loop_glog = nullptr, *h_loop_glog = nullptr;
+    uint8_t *loop_alog = nullptr, *h_loop_alog = nullptr;
+    float *loop_tlog = nullptr, *h_loop_tlog = nullptr;
+    frmc::LoopAcc *loop_acc = nullptr, *h_loop_acc = nullptr;
+    frmc::Prop *loop_props = nullptr;  // the loop window's proposals
+    int4 *loop_nodes = nullptr;         // S(Q) leaf-combine trees, nodes by level
+    int32_t loop_nnodes = -1, loop_nlev = 0, loop_rs = 0;  // (-1: trees not built)
+    int32_t loop_lev_off[frmc::kLoopLevels + 1] = {0};
+    int32_t loop_root[frmc::kMaxC] = {0};
+    int8_t groups_ident = -1;          // group g is atom g for every group (-1: not yet known)
+    float *d_real = nullptr;
+    unsigned long long *d_res = nullptr;
+    int64_t loop_windows = 0;        // windows run by the device loop (all runs)
+    float loop_maxabs = 0.f;         // largest box coordinate magnitude seen by the loop (bound)
     int32_t pending_check = -2;      // decision of the last device-decided window (-2: none)
     uint32_t pending_seq = 0;
     frmc::Inline last_inline;        // proposals of the last evaluated window (k = 0: staged)
@@ -300,6 +408,11 @@ int settle_window(frmc_engine *e, int32_t p);
 int finish_windows(frmc_engine *e);
 void drop_cells(frmc_engine *e);  // the sorted copy is not maintained from here on: rebuilt before its next use
 void collect_times(frmc_engine *e);
+// device window loop (single-atom translation runs)
+bool loop_usable(frmc_engine *e, int32_t kmax_run);
+int loop_begin(frmc_engine *e, const frmc_mt19937 &py, const frmc_mt19937 &np);
+int loop_plan(frmc_engine *e, float maxAmp, int32_t want, int32_t &M);
+int loop_batch(frmc_engine *e, int32_t k, int32_t M, float minAmp, float maxAmp, float tol);
 
 // bracket one kernel launch with events when profiling is on
 struct KTimer {

@@ -239,12 +239,30 @@ struct PropSrc {
     __device__ float nw(int32_t p, int d) const { return pp[p].nw[d]; }
 };
 
-// grid (ceil(n/256), ceil(k/kDP))
-template <bool PBC, bool ORTHO>
-__global__ void __launch_bounds__(256) k_delta(View V, Inline I, Pending P, int32_t k) {
+// The pending commit of a window: the kernel argument, or (device window
+// loop) the one k_loop_step left in device memory.
+// Its 12 words are read one per lane (vector loads, all in flight together)
+// and broadcast to the wave.
+__device__ __forceinline__ Pending pending_of(const View &V, const Pending &arg) {
+    if (!V.pend_dev) return arg;
+    constexpr int kWords = (int)(sizeof(Pending) / 4);
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = lane < kWords ? ((const uint32_t *)V.pend_dev)[lane] : 0u;
+    Pending P;
+    uint32_t *dst = (uint32_t *)&P;
+#pragma unroll
+    for (int i = 0; i < kWords; i++) dst[i] = (uint32_t)__builtin_amdgcn_readlane((int)w, i);
+    return P;
+}
+
+// grid (ceil(n/256), ceil(k/kDP)); PROPS: the proposals come from the
+// device table V.props (device window loop)
+template <bool PBC, bool ORTHO, bool PROPS>
+__global__ void __launch_bounds__(256) k_delta(View V, Inline I, Pending Parg, int32_t k) {
     __shared__ GroupLds G[kDP];
-    const InlineSrc S{I};
-    delta_body<PBC, ORTHO>(V, S, P, V.dn, k, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, G);
+    const Pending P = pending_of(V, Parg);
+    if (PROPS) delta_body<PBC, ORTHO>(V, PropSrc{V.props}, P, V.dn, k, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, G);
+    else delta_body<PBC, ORTHO>(V, InlineSrc{I}, P, V.dn, k, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, G);
 }
 
 // ------------------------------------------------------- k_delta, chunk walk
@@ -378,12 +396,14 @@ __device__ __forceinline__ void delta_chunk_body(const View &V, const Src &S, co
     }
 }
 
-// grid (nchunk, ceil(k/pp)); proposals in the kernel arguments (INL) or the
-// staging buffer
-template <bool ORTHO, bool INL>
-__global__ void __launch_bounds__(256) k_delta_chunks(View V, Inline I, Pending P, int32_t k, int32_t pp) {
-    if (INL) delta_chunk_body<ORTHO>(V, InlineSrc{I}, P, k, pp);
-    else delta_chunk_body<ORTHO>(V, StagedSrc{V}, P, k, pp);
+// grid (nchunk, ceil(k/pp)); proposals in the kernel arguments (SRC 0), the
+// staging buffer (1) or the device window loop's table (2)
+template <bool ORTHO, int SRC>
+__global__ void __launch_bounds__(256) k_delta_chunks(View V, Inline I, Pending Parg, int32_t k, int32_t pp) {
+    const Pending P = pending_of(V, Parg);
+    if (SRC == 0) delta_chunk_body<ORTHO>(V, InlineSrc{I}, P, k, pp);
+    else if (SRC == 1) delta_chunk_body<ORTHO>(V, StagedSrc{V}, P, k, pp);
+    else delta_chunk_body<ORTHO>(V, PropSrc{V.props}, P, k, pp);
 }
 
 // ------------------------------------------------------ pairwise summation
@@ -1455,22 +1475,29 @@ int launch_delta(const View &V, const Inline &I, const Pending &P, int32_t k, hi
         const int64_t want = ((int64_t)k * V.nchunk + kDeltaBlocks - 1) / kDeltaBlocks;
         const int32_t pp = (int32_t)std::max<int64_t>(1, std::min<int64_t>(kPruneP, want));
         const dim3 grid((unsigned)V.nchunk, (unsigned)((k + pp - 1) / pp));
+        const int src = V.props ? 2 : I.k > 0 ? 0 : 1;
         if (V.ortho) {
-            if (I.k > 0) k_delta_chunks<true, true><<<grid, 256, 0, st>>>(V, I, P, k, pp);
-            else k_delta_chunks<true, false><<<grid, 256, 0, st>>>(V, I, P, k, pp);
+            if (src == 0) k_delta_chunks<true, 0><<<grid, 256, 0, st>>>(V, I, P, k, pp);
+            else if (src == 1) k_delta_chunks<true, 1><<<grid, 256, 0, st>>>(V, I, P, k, pp);
+            else k_delta_chunks<true, 2><<<grid, 256, 0, st>>>(V, I, P, k, pp);
         } else {
-            if (I.k > 0) k_delta_chunks<false, true><<<grid, 256, 0, st>>>(V, I, P, k, pp);
-            else k_delta_chunks<false, false><<<grid, 256, 0, st>>>(V, I, P, k, pp);
+            if (src == 0) k_delta_chunks<false, 0><<<grid, 256, 0, st>>>(V, I, P, k, pp);
+            else if (src == 1) k_delta_chunks<false, 1><<<grid, 256, 0, st>>>(V, I, P, k, pp);
+            else k_delta_chunks<false, 2><<<grid, 256, 0, st>>>(V, I, P, k, pp);
         }
         FRMC_CHECK_LAUNCH();
         return FRMC_OK;
     }
     const dim3 grid(blocks_for(V.n, 256), (unsigned)((k + kDP - 1) / kDP));
-    if (V.isPBC) {
-        if (V.ortho) k_delta<true, true><<<grid, 256, 0, st>>>(V, I, P, k);
-        else k_delta<true, false><<<grid, 256, 0, st>>>(V, I, P, k);
+    if (V.props) {
+        if (!V.isPBC) k_delta<false, false, true><<<grid, 256, 0, st>>>(V, I, P, k);
+        else if (V.ortho) k_delta<true, true, true><<<grid, 256, 0, st>>>(V, I, P, k);
+        else k_delta<true, false, true><<<grid, 256, 0, st>>>(V, I, P, k);
+    } else if (V.isPBC) {
+        if (V.ortho) k_delta<true, true, false><<<grid, 256, 0, st>>>(V, I, P, k);
+        else k_delta<true, false, false><<<grid, 256, 0, st>>>(V, I, P, k);
     } else {
-        k_delta<false, false><<<grid, 256, 0, st>>>(V, I, P, k);
+        k_delta<false, false, false><<<grid, 256, 0, st>>>(V, I, P, k);
     }
     FRMC_CHECK_LAUNCH();
     return FRMC_OK;

@@ -488,6 +488,23 @@ int frmc_engine_eval_shapes(frmc_engine *e, int32_t *shapes_out, int32_t n);
    the cut-off distance used. */
 int frmc_set_delta_cells(int32_t mode);
 int frmc_engine_delta_cells(frmc_engine *e, int32_t *nchunk_out, float *wcut_out);
+/* Device window loop of frmc_engine_run_translation (no reference
+   counterpart: the per-move loop of Engine.run, Engine.py:3231-3344, with
+   its draws, Core/Collection.py:674-701 and Selectors/RandomSelectors.py:
+   87-94, and its accept rule, Engine.py:3308-3317, as one device step per
+   window).  For single-atom groups and k <= 128 the walk of each window's
+   totals, the rewind of both random streams and the next window's draws run
+   on the GPU, queued behind the window's evaluation in batches of up to 256
+   windows of one size; the host synchronises once per batch and runs the
+   last windows (fewer steps left than a window holds) itself.  Trajectory,
+   logs, statistics and stream states are those of the host loop.
+   frmc_set_window_loop: 1 on, 0 or -1 (default) off: the host walks and
+   draws every window (for a single engine the device step costs as much
+   as the host's round trip between windows, about 7 us, and measured runs
+   were 2-6 % slower with it).  frmc_engine_loop_windows: windows the
+   device loop has run on the engine. */
+int frmc_set_window_loop(int32_t mode);
+int frmc_engine_loop_windows(frmc_engine *e, int64_t *windows_out);
 /* window size of frmc_engine_run_translation: adaptive (default; k is the
    maximum, the size follows the acceptance rate) or always k */
 int frmc_engine_set_adaptive(frmc_engine *e, int enable);
