@@ -549,6 +549,228 @@ static int run_batched(std::vector<TranslationRun> &runs, int32_t T) {
     return FRMC_OK;
 }
 
+// Batched replicas on the device window loop: per step one
+// k_loop_step_batch (every replica's walk of its previous window and draw of
+// its next, one workgroup each) and the batched delta and evaluation
+// kernels, queued for a batch of M steps with window sizes fixed per replica
+// for the batch; the host uploads the batch's tables once and synchronises
+// once per batch, so it is off the per-step critical path.  A replica's last
+// windows draw only the moves it has left (as the host loop's last window
+// does), and a replica that has finished walks empty windows until the
+// batch ends, so the batches run every replica to its end.  The replicas'
+// real coordinates and result words live in the run's own device block.
+static int run_batched_loop(std::vector<TranslationRun> &runs) {
+    const int32_t R = (int32_t)runs.size();
+    for (auto &r : runs)
+        if (!loop_batch_usable(r.e, r.k)) return FRMC_OK;
+    hipStream_t st = runs[0].e->st;
+    // the engines' own streams may still hold work on their buffers; a
+    // pending commit goes to the loop state (the first step's k_delta_batch
+    // applies it)
+    for (auto &r : runs) FRMC_CHECK_HIP(hipStreamSynchronize(r.e->st));
+    std::vector<View> hv(R);
+    std::vector<size_t> o_real(R), o_res(R);
+    size_t shm_a = 0, shm_b = 0, shm_d = 0, shm_l = 4, n_real = 0, n_res = 0;
+    int32_t kmax = 1;
+    int64_t maxd = 0, maxa = 0, maxb = 0;
+    for (int32_t r = 0; r < R; r++) {
+        frmc_engine *e = runs[r].e;
+        use_unit_layout(e, false);  // 64-Q units, totals from k_eval_b_batch (or k_eval_a_batch)
+        drop_cells(e);
+        if (e->groups_ident < 0) {
+            bool id = e->G == e->n;
+            for (int32_t g = 0; g < e->G && id; g++) id = e->h_goff[g] == g && e->h_gatoms[g] == g;
+            e->groups_ident = id ? 1 : 0;
+        }
+        hv[r] = make_view(e, 0);
+        o_real[r] = n_real;
+        n_real += 3 * (size_t)e->n;
+        o_res[r] = n_res;
+        n_res += (size_t)runs[r].k * e->resW;
+        if (kBP * hv[r].dnStride <= kBatchLdsInts) shm_d = std::max(shm_d, sizeof(int32_t) * kBP * hv[r].dnStride);
+        shm_a = std::max(shm_a, eval_lds_bytes(hv[r]));
+        shm_b = std::max(shm_b, eval_b_lds_bytes(hv[r]));
+        shm_l = std::max(shm_l, sizeof(float) * (size_t)runs[r].k * e->resW);
+        kmax = std::max(kmax, runs[r].k);
+        maxd += (int64_t)((hv[r].n + kChunk - 1) / kChunk) * ((runs[r].k + kBP - 1) / kBP);
+        maxa += (int64_t)hv[r].nunits * runs[r].k;
+        maxb += runs[r].k;
+    }
+    FRMC_REQUIRE(maxd + maxa + maxb < ((int64_t)1 << 30), "batched grid too large");
+    // one device block and its pinned mirror: loop states, arguments,
+    // accepted moves, proposals, the three window tables, the block maps,
+    // the views, the real coordinates, the result words
+    const size_t b_states = align16(sizeof(LoopState) * R), b_args = align16(sizeof(LoopArgs) * R),
+                 b_acc = align16(sizeof(LoopAcc) * (size_t)R * kLoopWindows),
+                 b_props = align16(sizeof(Prop) * ((size_t)R * kmax + kLoopMaxK)),
+                 b_wins = align16(sizeof(BatchWin) * 3 * (size_t)R),
+                 b_maps = align16(sizeof(int2) * (size_t)(maxd + maxa + maxb)), b_views = align16(sizeof(View) * R),
+                 b_real = align16(sizeof(float) * n_real), b_res = align16(sizeof(unsigned long long) * n_res);
+    const size_t o_args = b_states, o_acc = o_args + b_args, o_props = o_acc + b_acc, o_wins = o_props + b_props,
+                 o_maps = o_wins + b_wins, o_views = o_maps + b_maps, o_reals = o_views + b_views,
+                 o_ress = o_reals + b_real, bytes = o_ress + b_res;
+    char *d = nullptr, *h = nullptr;
+    struct Free {
+        hipStream_t st;
+        char *&d, *&h;
+        ~Free() {
+            (void)hipStreamSynchronize(st);
+            if (d) (void)hipFree(d);
+            if (h) (void)hipHostFree(h);
+        }
+    } guard{st, d, h};
+    FRMC_CHECK_HIP(hipMalloc((void **)&d, bytes));
+    FRMC_CHECK_HIP(hipHostMalloc((void **)&h, bytes, hipHostMallocDefault));
+    LoopState *d_states = (LoopState *)d, *h_states = (LoopState *)h;
+    LoopArgs *d_args = (LoopArgs *)(d + o_args), *h_args = (LoopArgs *)(h + o_args);
+    LoopAcc *d_acc = (LoopAcc *)(d + o_acc), *h_acc = (LoopAcc *)(h + o_acc);
+    Prop *d_props = (Prop *)(d + o_props);
+    BatchWin *d_wins = (BatchWin *)(d + o_wins), *h_wins = (BatchWin *)(h + o_wins);
+    int2 *d_maps = (int2 *)(d + o_maps), *h_maps = (int2 *)(h + o_maps);
+    View *d_views = (View *)(d + o_views), *h_views = (View *)(h + o_views);
+    float *d_real = (float *)(d + o_reals), *h_real = (float *)(h + o_reals);
+    unsigned long long *d_res = (unsigned long long *)(d + o_ress);
+    for (int32_t r = 0; r < R; r++) {
+        frmc_engine *e = runs[r].e;
+        loop_init_state(e, h_states[r], *runs[r].py_state, *runs[r].np_state);
+        hv[r].pend_dev = &d_states[r].h.pend;
+        hv[r].out_res = d_res + o_res[r];
+        h_views[r] = hv[r];
+        memcpy(h_real + o_real[r], e->h_real.data(), sizeof(float) * 3 * (size_t)e->n);
+    }
+    FRMC_CHECK_HIP(hipMemcpyAsync(d, h, b_states, hipMemcpyHostToDevice, st));
+    FRMC_CHECK_HIP(hipMemcpyAsync(d_views, h_views, sizeof(View) * R, hipMemcpyHostToDevice, st));
+    FRMC_CHECK_HIP(hipMemcpyAsync(d_real, h_real, sizeof(float) * n_real, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> kr(R), poff(R), c0(R);
+    int32_t ramp = 16;
+    for (;;) {
+        // the batch: window sizes from each replica's acceptance so far, and
+        // about as many steps as the replica furthest from its end needs
+        int64_t need = 0;
+        int32_t off = 0;
+        for (int32_t r = 0; r < R; r++) {
+            const TranslationRun &run = runs[r];
+            kr[r] = run.e->adaptive ? window_size(run.gen0, run.acc0, run.k) : run.k;
+            const double p = (run.acc0 + 1.0) / (run.gen0 + 2.0);
+            const int64_t left = run.nsteps - run.step;
+            if (left > 0)
+                need = std::max<int64_t>(need, std::min<int64_t>(left, (int64_t)std::ceil(left * p * 1.2) + 2));
+            poff[r] = off;
+            off += kr[r];
+            c0[r] = run.e->cur_dn;
+        }
+        if (need == 0) break;
+        const int32_t M = (int32_t)std::min<int64_t>({(int64_t)ramp, (int64_t)kLoopWindows, need});
+        // arguments, window tables (first step, odd steps, even steps), block maps
+        FRMC_CHECK_HIP(hipStreamSynchronize(st));  // the previous batch no longer reads the pinned block
+        int32_t nd = 0, na = 0, nb = 0;
+        for (int32_t r = 0; r < R; r++) {
+            frmc_engine *e = runs[r].e;
+            LoopArgs &A = h_args[r];
+            A = LoopArgs{};
+            A.L = d_states + r;
+            A.resW = e->resW;
+            A.rs = e->resW;
+            A.C = e->C;
+            A.dnStride = e->dnStride;
+            A.tol = runs[r].tolerance;
+            A.res = d_res + o_res[r];
+            A.G = e->G;
+            A.nbits = bit_length(e->G);
+            A.ident = e->groups_ident;
+            A.minAmp = runs[r].minAmp;
+            A.maxAmp = runs[r].maxAmp;
+            for (int q = 0; q < 9; q++) A.recip[q] = e->recip[q];
+            A.goff = e->goff;
+            A.gatoms = e->gatoms;
+            A.elem = e->elem;
+            A.spos = nullptr;
+            A.xyz = e->xyz;
+            A.real = d_real + o_real[r];
+            A.props = d_props + poff[r];
+            A.acc = d_acc + (size_t)r * kLoopWindows;
+            A.kr = kr[r];
+            A.c0 = c0[r];
+            A.dnb[0] = e->dnbuf[0];
+            A.dnb[1] = e->dnbuf[1];
+            A.left = runs[r].nsteps - runs[r].step;
+            for (int t = 0; t < 3; t++) {
+                BatchWin &W = h_wins[(size_t)t * R + r];
+                const int32_t s = t;  // a step of the table's kind: the first, odd, even
+                W.rep = r;
+                W.k = kr[r];
+                W.prop = poff[r];
+                W.seq = 0;
+                W.dn = e->dnbuf[(c0[r] + s) & 1];
+                W.dn_zero = e->dnbuf[(c0[r] + s + 1) & 1];
+                W.zero_len = (int64_t)(s == 0 ? e->dn_used[(c0[r] + 1) & 1] : kr[r]) * e->dnStride;
+                W.P = Pending{};
+                W.P.valid = 0;
+            }
+        }
+        for (int32_t r = 0; r < R; r++) {
+            const int32_t bd = ((hv[r].n + kChunk - 1) / kChunk) * ((kr[r] + kBP - 1) / kBP);
+            for (int32_t q = 0; q < bd; q++) h_maps[nd++] = make_int2(r, q);
+        }
+        for (int32_t r = 0; r < R; r++)
+            for (int32_t q = 0; q < hv[r].nunits * kr[r]; q++) h_maps[nd + na++] = make_int2(r, q);
+        for (int32_t r = 0; r < R; r++) {
+            if (hv[r].C == 1 && hv[r].c[0].rspace) continue;  // k_eval_a_batch publishes the total
+            for (int32_t q = 0; q < kr[r]; q++) h_maps[nd + na + nb++] = make_int2(r, q);
+        }
+        FRMC_CHECK_HIP(hipMemcpyAsync(d_args, h_args, sizeof(LoopArgs) * R, hipMemcpyHostToDevice, st));
+        FRMC_CHECK_HIP(hipMemcpyAsync(d_wins, h_wins, sizeof(BatchWin) * 3 * R, hipMemcpyHostToDevice, st));
+        FRMC_CHECK_HIP(hipMemcpyAsync(d_maps, h_maps, sizeof(int2) * (nd + na + nb), hipMemcpyHostToDevice, st));
+        for (int32_t s = 0; s <= M; s++) {
+            if (int rc = launch_loop_step_batch(d_args, R, s, M, shm_l, st)) return rc;
+            if (s == M) break;
+            BatchArgs BA{};
+            BA.views = d_views;
+            BA.win = d_wins + (size_t)(s == 0 ? 0 : (s & 1) ? 1 : 2) * R;
+            BA.props = d_props;
+            BA.map_d = d_maps;
+            BA.map_a = d_maps + nd;
+            BA.map_b = d_maps + nd + na;
+            if (int rc = launch_batch(BA, nd, na, nb, shm_d, shm_a, shm_b, st)) return rc;
+        }
+        // each replica's streams and header, and its accepted moves
+        FRMC_CHECK_HIP(hipMemcpy2DAsync(h_states, sizeof(LoopState), d_states, sizeof(LoopState),
+                                        offsetof(LoopState, gid), R, hipMemcpyDeviceToHost, st));
+        FRMC_CHECK_HIP(hipMemcpy2DAsync(h_acc, sizeof(LoopAcc) * kLoopWindows, d_acc, sizeof(LoopAcc) * kLoopWindows,
+                                        sizeof(LoopAcc) * M, R, hipMemcpyDeviceToHost, st));
+        FRMC_CHECK_HIP(hipStreamSynchronize(st));
+        g_batch_steps += M;
+        for (int32_t r = 0; r < R; r++) {
+            TranslationRun &run = runs[r];
+            frmc_engine *e = run.e;
+            const LoopState &L = h_states[r];
+            if (int rc = loop_apply(e, L, h_acc + (size_t)r * kLoopWindows, M, (int64_t)M * kr[r])) return rc;
+            const int64_t walked = L.h.step;
+            FRMC_REQUIRE(walked <= run.nsteps - run.step, "replica %d walked past its end", r);
+            run.stats->windows += L.h.nwin;
+            run.stats->evaluated += L.h.neval;
+            run.stats->generated += walked;
+            run.stats->tried += walked;
+            run.stats->accepted += L.h.nacc;
+            run.stats->tolerated += L.h.ntol;
+            run.gen0 += walked;
+            run.acc0 += L.h.nacc;
+            run.step += walked;
+            memcpy(run.py_state->mt, L.py.mt, sizeof(run.py_state->mt));
+            run.py_state->pos = L.py.pos;
+            memcpy(run.np_state->mt, L.np.mt, sizeof(run.np_state->mt));
+            run.np_state->pos = L.np.pos;
+            // window s wrote buffer (c0 + s) & 1 and zeroed the other
+            e->dn_used[(c0[r] + M - 1) & 1] = kr[r];
+            e->dn_used[(c0[r] + M) & 1] = 0;
+            e->cur_dn = (int)((c0[r] + M) & 1);
+            e->loop_windows += L.h.nwin;
+        }
+        ramp = std::min(2 * ramp, kLoopWindows);
+    }
+    return FRMC_OK;
+}
+
 }  // namespace frmc
 
 using namespace frmc;
@@ -622,7 +844,20 @@ extern "C" int frmc_engines_run_translation(frmc_engine **engines, int32_t nengi
         FRMC_REQUIRE(engines[r]->device == engines[0]->device, "replicas must share one device");
     bool batchable = g_batch_replicas != 0;
     for (int32_t r = 0; r < nengines; r++) batchable &= engines[r]->gmax == 1;
-    if (batchable) return run_batched(runs, T);
+    if (batchable) {
+        if (int rc = run_batched_loop(runs)) return rc;
+        bool all_done = true;
+        for (auto &r : runs) all_done &= r.done();
+        if (!all_done) return run_batched(runs, T);
+        // the accepted moves still pending, on one stream
+        hipStream_t st = engines[0]->st;
+        for (auto &r : runs)
+            if (int rc = flush_commit(r.e, st)) return rc;
+        FRMC_CHECK_HIP(hipStreamSynchronize(st));
+        for (auto &r : runs)
+            if (int rc = finish_windows(r.e)) return rc;
+        return FRMC_OK;
+    }
     std::vector<int> rcs(T, FRMC_OK);
     std::vector<std::string> errs(T);
     auto work = [&](int32_t t) {

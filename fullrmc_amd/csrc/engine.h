@@ -169,6 +169,7 @@ struct LoopHead {
     int32_t nacc, ntol;  // accepted and tolerated (accepted worse) moves of the batch
     int32_t error;       // kLoopErr*: a draw the device loop does not make
     int32_t kw;          // proposals of the window in flight (0: none)
+    int32_t nwin, neval; // windows drawn and proposals drawn in the batch
     Pending pend;        // accepted move the next k_delta commits
 };
 static_assert(sizeof(Pending) == 48 && sizeof(LoopHead) % 4 == 0 && sizeof(LoopHead) <= 4 * 64,
@@ -214,13 +215,23 @@ struct LoopArgs {
     const float *xyz;
     float *real;
     Prop *props;  // the window's proposal table (read by the walk, rewritten by the draw)
-    // logs of the batch (index = proposal walked in the batch) and its accepted moves
+    // logs of the batch (index = proposal walked in the batch; null: none)
+    // and its accepted moves
     int32_t *glog;
     uint8_t *alog;
     float *tlog;
     LoopAcc *acc;
+    // batched replicas (k_loop_step_batch): window size, the two delta
+    // buffers and the one the batch's first window uses
+    int32_t kr, c0;
+    const int32_t *dnb[2];
+    // moves left at the batch's start: a window draws at most the moves
+    // left after the walks before it (the host loop's last window too)
+    int64_t left;
 };
 int launch_loop_step(const LoopArgs &A, hipStream_t st);
+// step s of a batch of M windows of R replicas (args: R device entries)
+int launch_loop_step_batch(const LoopArgs *args, int32_t R, int32_t s, int32_t M, size_t shm, hipStream_t st);
 
 constexpr int32_t kLoopErrDirection = 1;  // a zero-length direction draw (the reference would redraw)
 constexpr int32_t kLoopErrWords = 2;      // a window's draws past four state blocks of the Python stream
@@ -410,6 +421,10 @@ void drop_cells(frmc_engine *e);  // the sorted copy is not maintained from here
 void collect_times(frmc_engine *e);
 // device window loop (single-atom translation runs)
 bool loop_usable(frmc_engine *e, int32_t kmax_run);
+bool loop_batch_usable(frmc_engine *e, int32_t kmax_run);  // batched replicas (64-Q units, no chunk walk)
+int loop_alloc(frmc_engine *e, bool single);
+void loop_init_state(frmc_engine *e, LoopState &H, const frmc_mt19937 &py, const frmc_mt19937 &np);
+int loop_apply(frmc_engine *e, const LoopState &H, const LoopAcc *acc, int32_t M, int64_t nlog);
 int loop_begin(frmc_engine *e, const frmc_mt19937 &py, const frmc_mt19937 &np);
 int loop_plan(frmc_engine *e, float maxAmp, int32_t want, int32_t &M);
 int loop_batch(frmc_engine *e, int32_t k, int32_t M, float minAmp, float maxAmp, float tol);

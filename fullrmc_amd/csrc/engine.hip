@@ -951,10 +951,22 @@ bool loop_usable(frmc_engine *e, int32_t kmax_run) {
     return loop_trees(e);
 }
 
-// Allocate the loop's buffers (first use) and load its state: both streams,
-// the totals, the device copy of the real coordinates.
-int loop_begin(frmc_engine *e, const frmc_mt19937 &py, const frmc_mt19937 &np) {
-    if (!e->loop) {
+// Batched replicas take the loop by default: there the host's walks and
+// draws of all replicas, not the GPU, bound a step.
+bool loop_batch_usable(frmc_engine *e, int32_t kmax_run) {
+    return g_loop_mode != 0 && e->gmax == 1 && e->have_real && kmax_run <= kLoopMaxK && kmax_run <= e->kmax &&
+           e->resW <= kLoopResW;
+}
+
+// The loop's buffers (first use; single: also the single-engine state, logs
+// and proposal table) and the device copy of the real coordinates (queued
+// on the engine stream).
+int loop_alloc(frmc_engine *e, bool single) {
+    if (!e->d_real) {
+        FRMC_CHECK_HIP(dalloc(e, &e->d_real, 3 * (size_t)e->n));
+        FRMC_CHECK_HIP(dalloc(e, &e->d_res, (size_t)e->kmax * e->resW));
+    }
+    if (single && !e->loop) {
         const size_t cap = (size_t)kLoopWindows * kLoopMaxK;
         FRMC_CHECK_HIP(dalloc(e, &e->loop, 1));
         FRMC_CHECK_HIP(halloc(e, &e->h_loop, 1));
@@ -966,8 +978,6 @@ int loop_begin(frmc_engine *e, const frmc_mt19937 &py, const frmc_mt19937 &np) {
         FRMC_CHECK_HIP(halloc(e, &e->h_loop_tlog, cap));
         FRMC_CHECK_HIP(dalloc(e, &e->loop_acc, (size_t)kLoopWindows));
         FRMC_CHECK_HIP(halloc(e, &e->h_loop_acc, (size_t)kLoopWindows));
-        FRMC_CHECK_HIP(dalloc(e, &e->d_real, 3 * (size_t)e->n));
-        FRMC_CHECK_HIP(dalloc(e, &e->d_res, (size_t)e->kmax * e->resW));
         FRMC_CHECK_HIP(dalloc(e, &e->loop_props, (size_t)kLoopMaxK));
     }
     if (e->groups_ident < 0) {
@@ -975,24 +985,67 @@ int loop_begin(frmc_engine *e, const frmc_mt19937 &py, const frmc_mt19937 &np) {
         for (int32_t g = 0; g < e->G && id; g++) id = e->h_goff[g] == g && e->h_gatoms[g] == g;
         e->groups_ident = id ? 1 : 0;
     }
-    // nothing queued may still read the pinned block
-    FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
-    LoopState &H = *e->h_loop;
+    FRMC_CHECK_HIP(hipMemcpyAsync(e->d_real, e->h_real.data(), sizeof(float) * 3 * (size_t)e->n,
+                                  hipMemcpyHostToDevice, e->st));
+    return FRMC_OK;
+}
+
+// a run's starting loop state: both streams, the totals, the pending commit
+void loop_init_state(frmc_engine *e, LoopState &H, const frmc_mt19937 &py, const frmc_mt19937 &np) {
     memcpy(H.py.mt, py.mt, sizeof(H.py.mt));
     H.py.pos = py.pos;
     memcpy(H.np.mt, np.mt, sizeof(H.np.mt));
     H.np.pos = np.pos;
     H.h.cur_tse = e->cur_tse;
     for (int ci = 0; ci < kMaxC; ci++) H.h.cur_chi2[ci] = e->cur_chi2[ci];
+    H.h.pad = 0;
     H.h.step = 0;
     H.h.nacc = H.h.ntol = H.h.error = H.h.kw = 0;
     H.h.pend = e->pend;
+}
+
+// Allocate the loop's buffers (first use) and load its state: both streams,
+// the totals, the device copy of the real coordinates.
+int loop_begin(frmc_engine *e, const frmc_mt19937 &py, const frmc_mt19937 &np) {
+    // nothing queued may still read the pinned block
+    FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
+    if (int rc = loop_alloc(e, true)) return rc;
+    loop_init_state(e, *e->h_loop, py, np);
     FRMC_CHECK_HIP(hipMemcpyAsync(e->loop, e->h_loop, offsetof(LoopState, gid), hipMemcpyHostToDevice, e->st));
-    FRMC_CHECK_HIP(hipMemcpyAsync(e->d_real, e->h_real.data(), sizeof(float) * 3 * (size_t)e->n,
-                                  hipMemcpyHostToDevice, e->st));
     e->loop_maxabs = 0.0f;
     for (float x : e->h_box) e->loop_maxabs = std::max(e->loop_maxabs, fabsf(x));
     FRMC_CHECK_HIP(hipStreamSynchronize(e->st));  // the pinned block is rewritten per batch
+    return FRMC_OK;
+}
+
+// The host mirror after a batch of the loop: the accepted moves' box and
+// real coordinates, the totals, the pending commit; an error the step
+// reported.
+int loop_apply(frmc_engine *e, const LoopState &H, const LoopAcc *acc, int32_t M, int64_t nlog) {
+    if (H.h.error) {
+        set_error(H.h.error == kLoopErrDirection ? "a direction draw of length 0 (the device window loop does not redraw)"
+                                                 : "a window's draws ran past four MT19937 state blocks");
+        return FRMC_ESTATE;
+    }
+    FRMC_REQUIRE(H.h.nacc >= 0 && H.h.nacc <= M && H.h.step >= 0 && H.h.step <= nlog, "bad loop batch state");
+    for (int32_t i = 0; i < H.h.nacc; i++) {
+        const LoopAcc &a = acc[i];
+        FRMC_REQUIRE(a.atom >= 0 && a.atom < e->n, "bad accepted move");
+        for (int d = 0; d < 3; d++) {
+            e->h_box[3 * (size_t)a.atom + d] = a.box[d];
+            e->h_real[3 * (size_t)a.atom + d] = a.real[d];
+            e->loop_maxabs = std::max(e->loop_maxabs, fabsf(a.box[d]));
+            if (!(fabsf(a.box[d]) <= kCellsMaxCoord)) e->cells_stale = true;
+        }
+    }
+    e->commits_since_cells += H.h.nacc;
+    e->cur_tse = H.h.cur_tse;
+    for (int ci = 0; ci < e->C; ci++) e->cur_chi2[ci] = H.h.cur_chi2[ci];
+    e->pend = H.h.pend;
+    e->win_dn = H.h.pend.dn;
+    e->deferred = false;
+    e->device_decided = false;
+    e->last_k = 0;
     return FRMC_OK;
 }
 
@@ -1069,6 +1122,7 @@ int loop_batch(frmc_engine *e, int32_t k, int32_t M, float minAmp, float maxAmp,
     A.alog = e->loop_alog;
     A.tlog = e->loop_tlog;
     A.acc = e->loop_acc;
+    A.left = (int64_t)1 << 62;  // the caller's batches fit the moves left
     Inline I0;
     I0.k = 0;
     Pending P0{};
@@ -1116,31 +1170,7 @@ int loop_batch(frmc_engine *e, int32_t k, int32_t M, float minAmp, float maxAmp,
     FRMC_CHECK_HIP(hipStreamSynchronize(e->st));
     if (e->profile) collect_times(e);
     e->loop_windows += M;
-    if (H.h.error) {
-        set_error(H.h.error == kLoopErrDirection ? "a direction draw of length 0 (the device window loop does not redraw)"
-                                               : "a window's draws ran past four MT19937 state blocks");
-        return FRMC_ESTATE;
-    }
-    FRMC_REQUIRE(H.h.nacc >= 0 && H.h.nacc <= M && H.h.step >= 0 && H.h.step <= (int64_t)nlog, "bad loop batch state");
-    // the host mirror of the accepted moves
-    for (int32_t i = 0; i < H.h.nacc; i++) {
-        const LoopAcc &a = e->h_loop_acc[i];
-        for (int d = 0; d < 3; d++) {
-            e->h_box[3 * (size_t)a.atom + d] = a.box[d];
-            e->h_real[3 * (size_t)a.atom + d] = a.real[d];
-            e->loop_maxabs = std::max(e->loop_maxabs, fabsf(a.box[d]));
-            if (!(fabsf(a.box[d]) <= kCellsMaxCoord)) e->cells_stale = true;
-        }
-    }
-    e->commits_since_cells += H.h.nacc;
-    e->cur_tse = H.h.cur_tse;
-    for (int ci = 0; ci < e->C; ci++) e->cur_chi2[ci] = H.h.cur_chi2[ci];
-    e->pend = H.h.pend;
-    e->win_dn = H.h.pend.dn;
-    e->deferred = false;
-    e->device_decided = false;
-    e->last_k = 0;
-    return FRMC_OK;
+    return loop_apply(e, H, e->h_loop_acc, M, (int64_t)nlog);
 }
 
 }  // namespace frmc

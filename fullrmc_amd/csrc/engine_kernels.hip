@@ -1212,7 +1212,7 @@ __device__ __forceinline__ void delta_chunk(const View &V, const BatchWin &W, co
     const int32_t dnStride = V.dnStride, T = V.T, k = W.k;
     const int32_t p0 = pb * kBP, np = min(kBP, k - p0);
     int32_t *out = W.dn + (int64_t)p0 * dnStride;
-    const Pending P = W.P;
+    const Pending P = pending_of(V, W.P);  // (device window loop: the replica's loop state)
     int32_t qa[kBP], qe[kBP];
     float ox[kBP], oy[kBP], oz[kBP], nx[kBP], ny[kBP], nz[kBP];
 #pragma unroll

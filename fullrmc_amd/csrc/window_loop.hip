@@ -56,7 +56,7 @@ __device__ __forceinline__ float res_value(const unsigned long long *w) {
 // combine programs) and once more for the drawn atoms (their coordinates):
 // the step sits between two windows on the stream, so its latency is what
 // it costs.
-__global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
+__device__ void loop_step_body(const LoopArgs &A) {
     __shared__ uint32_t ring[2][kLoopRing][kMtN];
     extern __shared__ float s_res[];  // [wk][rs]: result words, then the combine tree's nodes
     __shared__ int4 s_node[kLoopNodes];
@@ -141,12 +141,13 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
             for (int d = 0; d < 3; d++) s_nreal[tid][d] = nr[d];
         }
     }
-    if (tid == 0) {
-        s_first = A.wk;
-        s_preal_set = 0;
-    }
+    if (tid == 0) s_preal_set = 0;
     __syncthreads();
+    // proposals of the window in flight (fewer than its size when the run's
+    // last moves did not fill it)
+    const int32_t kw = A.wk > 0 ? min(s_head.kw, A.wk) : 0;
     if (tid == 0) {
+        s_first = kw;
         s_pv = s_head.pend.valid;
         s_patom = s_head.pend.atom;
         for (int d = 0; d < 3; d++) s_pbox[d] = s_head.pend.nw[d];
@@ -171,7 +172,7 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
             // the leaf combines, one tree level at a time over all proposals
             for (int32_t lv = 1; lv <= A.nlev; lv++) {
                 const int32_t n0 = A.lev_off[lv - 1], nn = A.lev_off[lv] - n0;
-                for (int32_t t = tid; t < A.wk * nn; t += kLoopThreads) {
+                for (int32_t t = tid; t < kw * nn; t += kLoopThreads) {
                     const int32_t p = t / nn;
                     const int4 nd = s_node[n0 + t % nn];
                     float *row = s_res + p * A.rs;
@@ -180,7 +181,7 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
                 __syncthreads();
             }
         }
-        if (tid < A.wk) {
+        if (tid < kw) {
             const int32_t p = tid;
             const float *rv = s_res + p * A.rs;
             float tse = 0.0f;
@@ -200,15 +201,15 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
             if (!worse || !(s_u[p] > A.tol)) atomicMin(&s_first, p);
         }
         __syncthreads();
-        a = s_first < A.wk ? s_first : -1;
-        walked = a >= 0 ? a + 1 : A.wk;
+        a = s_first < kw ? s_first : -1;
+        walked = a >= 0 ? a + 1 : kw;
         const bool worse_a = a >= 0 && s_tse[a] > cur;
         if (a >= 0) {
             used[0] = worse_a ? s_post[a] : s_pre[a];
             used[1] = 6 * (a + 1);
         } else {
-            used[0] = s_post[A.wk - 1];
-            used[1] = 6 * A.wk;
+            used[0] = kw > 0 ? s_post[kw - 1] : 0;
+            used[1] = 6 * kw;
         }
         // the accepted move replaces its atom's stored coordinates in the draws
         if (tid == 0) {
@@ -250,9 +251,10 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
         return mt_temper(ring[s][(sb[s] + r / kMtN) & 3][r % kMtN]);
     };
     bool words_err = false;  // the draws need more Python-stream words than the ring holds
-    if (A.dk > 0) {
+    const int32_t dk = A.dk > 0 ? (int32_t)min((int64_t)A.dk, max((int64_t)0, A.left - step)) : 0;
+    if (dk > 0) {
         // NumPy stream: 6 words (three uniforms) per proposal
-        const int32_t last_np = sb[1] + (pos[1] + 6 * A.dk - 1) / kMtN;
+        const int32_t last_np = sb[1] + (pos[1] + 6 * dk - 1) / kMtN;
         while (hi[1] <= last_np) {
             twist_block(ring[1][(hi[1] - 1) & 3], ring[1][hi[1] & 3]);
             hi[1]++;
@@ -271,13 +273,13 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
                 // one ballot marks the usable words of a 64-word tile; the
                 // proposals in it are then found with scalar bit operations
                 int32_t p = s_scan_p, j = s_scan_j;
-                while (p < A.dk && j + 68 <= avail) {
+                while (p < dk && j + 68 <= avail) {
                     const uint32_t w = word(0, j + tid);
                     const unsigned long long m = __ballot((w >> (32 - A.nbits)) < (uint32_t)A.G);
                     const int32_t base = j;
                     j = base + 64;
                     int32_t off = 0;
-                    while (p < A.dk && off < 64) {
+                    while (p < dk && off < 64) {
                         const unsigned long long mm = m & (~0ull << off);
                         if (mm == 0) {
                             j = base + 64;  // the rest of the tile: rejected tries
@@ -296,7 +298,7 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
                 }
             }
             __syncthreads();
-            if (s_scan_p >= A.dk) break;
+            if (s_scan_p >= dk) break;
             if (hi[0] - sb[0] >= kLoopRing) {  // (uniform: every thread leaves here)
                 words_err = true;
                 break;
@@ -304,7 +306,7 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
             twist_block(ring[0][(hi[0] - 1) & 3], ring[0][hi[0] & 3]);
             hi[0]++;
         }
-        if (!words_err && tid < A.dk) {
+        if (!words_err && tid < dk) {
             const int32_t p = tid, r = s_r[p];
             const int32_t gi = (int32_t)(word(0, r) >> (32 - A.nbits));
             const int32_t atom = A.ident ? gi : A.gatoms[A.goff[gi]];
@@ -352,7 +354,7 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
     }
 
     // ------------------------------------------------ 4. the walk's and the streams' stores
-    if (A.wk > 0 && tid < walked) {
+    if (A.wk > 0 && tid < walked && A.glog) {
         A.glog[step0 + tid] = s_gid[tid];
         A.alog[step0 + tid] = tid == a ? 1 : 0;
         A.tlog[step0 + tid] = tid == a ? s_tse[tid] : cur;
@@ -395,12 +397,49 @@ __global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) {
         }
         L->py.pos = pos[0];
         L->np.pos = pos[1];
-        L->h.kw = A.dk;
+        L->h.kw = dk;
+        L->h.nwin = (A.reset ? 0 : s_head.nwin) + (dk > 0 ? 1 : 0);
+        L->h.neval = (A.reset ? 0 : s_head.neval) + dk;
         L->h.step = step;
         L->h.nacc = nacc;
         L->h.ntol = ntol;
         if (words_err) L->h.error = kLoopErrWords;
     }
+}
+
+__global__ void __launch_bounds__(kLoopThreads) k_loop_step(LoopArgs A) { loop_step_body(A); }
+
+// Step s of a batch of M windows for every replica (one workgroup each):
+// the replica's arguments are copied to LDS, with this step's window (walk
+// of window s-1, draw of window s) filled in.
+__global__ void __launch_bounds__(kLoopThreads) k_loop_step_batch(const LoopArgs *args, int32_t s, int32_t M) {
+    __shared__ LoopArgs sA;
+    constexpr int kWords = (int)(sizeof(LoopArgs) / 4);
+    static_assert(sizeof(LoopArgs) % 4 == 0 && kWords <= kLoopThreads, "LoopArgs copy");
+    const uint32_t *src = (const uint32_t *)(args + blockIdx.x);
+    if (threadIdx.x < kWords) ((uint32_t *)&sA)[threadIdx.x] = src[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sA.reset = s == 0 ? 1 : 0;
+        sA.wk = s > 0 ? sA.kr : 0;
+        sA.dk = s < M ? sA.kr : 0;
+        sA.leafmode = 0;
+        sA.win_dn = sA.dnb[(sA.c0 + s + 1) & 1];  // window s-1's buffer
+    }
+    __syncthreads();
+    loop_step_body(sA);
+}
+
+int launch_loop_step_batch(const LoopArgs *args, int32_t R, int32_t s, int32_t M, size_t shm, hipStream_t st) {
+    FRMC_REQUIRE(R >= 1 && s >= 0 && s <= M && M <= kLoopWindows, "bad loop batch step");
+    FRMC_REQUIRE(shm <= sizeof(float) * kLoopMaxK * kLoopResW, "bad loop batch LDS");
+    static const hipError_t attr = hipFuncSetAttribute(
+        (const void *)k_loop_step_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (int)(sizeof(float) * kLoopMaxK * (kLoopResW + kLoopNodes)));
+    FRMC_REQUIRE(attr == hipSuccess, "k_loop_step_batch: dynamic LDS not available");
+    k_loop_step_batch<<<(unsigned)R, kLoopThreads, std::max<size_t>(shm, 4), st>>>(args, s, M);
+    FRMC_CHECK_LAUNCH();
+    return FRMC_OK;
 }
 
 int launch_loop_step(const LoopArgs &A, hipStream_t st) {

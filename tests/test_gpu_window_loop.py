@@ -193,3 +193,53 @@ def test_window_loop_runs_split_over_calls(box500, loop_mode):
     E.run_device(3, k=24, log=dlog)
     assert 0 < w1 < loop_windows(E)
     assert_same_state(E, oe, dlog, olog, (pys, nps))
+
+
+@pytest.mark.parametrize("mode", [-1, 0])
+def test_batched_replicas_on_and_off_the_device_loop(loop_mode, mode):
+    """Batched replicas take the device loop by default (-1) and the host
+    walks with it off (0); both end every replica in the state of its own
+    single-engine run (host loop), tolerance above zero included, and
+    replica 0 in the oracle's."""
+    from fullrmc_amd.Engine import run_replicas
+    boxes = []
+    for i, n in enumerate((700, 400, 1100, 520)):
+        L = (n / 0.05) ** (1 / 3)
+        b = systems.random_box(n, L, 2.0, seed=140 + i)
+        b["exp"] = pdc_target(b, 0.1, 150 + i, 0.025, 0.05, 150)
+        boxes.append(b)
+
+    def engine(b):
+        E = device_engine(b, [PairDistributionConstraint(b["exp"])])
+        E.set_move_generator(TranslationGenerator(0.2))
+        E.set_tolerance(10.0)
+        return E
+
+    loop_mode(mode)
+    reps = [engine(b) for b in boxes]
+    seeds, nsteps = [160, 161, 162, 163], 900
+    stats = run_replicas(reps, nsteps, k=24, seeds=seeds)
+    assert all(st["generated"] == nsteps for st in stats)
+    windows = [loop_windows(E) for E in reps]
+    assert all(w > 0 for w in windows) if mode == -1 else all(w == 0 for w in windows)
+    loop_mode(0)
+    for E, b, seed in zip(reps, boxes, seeds):
+        E.sync_from_device()
+        random.seed(seed)
+        np.random.seed(seed)
+        S = engine(b)
+        S.run_device(nsteps, k=24)
+        assert E.realCoordinates.tobytes() == S.realCoordinates.tobytes()
+        assert E.boxCoordinates.tobytes() == S.boxCoordinates.tobytes()
+        assert np.float32(E.totalStandardError).tobytes() == np.float32(S.totalStandardError).tobytes()
+        assert (E.accepted, E.tolerated) == (S.accepted, S.tolerated)
+        for c, cs in zip(E.constraints, S.constraints):
+            assert c.data["intra"].tobytes() == cs.data["intra"].tobytes()
+            assert c.data["inter"].tobytes() == cs.data["inter"].tobytes()
+        assert E._replica_rng[0] == random.getstate()
+        assert np.array_equal(E._replica_rng[1][1], np.random.get_state()[1])
+    b = boxes[0]
+    oe, olog, pys, nps = oracle_run(b, [[i] for i in range(len(b["real"]))], oeng.Translation(0.2), _pdc(b),
+                                    nsteps, seeds[0], tolerance=10.0)
+    assert np.float32(reps[0].totalStandardError).tobytes() == np.float32(oe.tse).tobytes()
+    assert reps[0].realCoordinates.tobytes() == oe.s.real.tobytes()
