@@ -355,7 +355,7 @@ def replicas_section(args, boxes):
     sizes = [len(start["real"]) for start, _ in reps]
     out = {"workload": "%d independent single-element boxes, N log-uniform in [1000, 20000] (mean %.0f), density "
                        "0.05/A^3, PDC r[0,15]@0.05, translation 0.1 A, k<=32, own random streams per replica; "
-                       "all replicas' windows of a step in one launch per kernel"
+                       "all replicas' windows of a step in one launch per kernel, walks and draws on the GPU"
                        % (len(reps), float(np.mean(sizes))),
            "moves_per_s": round(moves / el, 1), "moves": moves, "seconds": round(el, 3),
            "accepted_fraction": round(sum(st["accepted"] for st in stats) / moves, 4),
@@ -764,7 +764,8 @@ def main():
     ap.add_argument("--eval-shape", type=int, default=-1,
                     help="force one window-evaluation shape (0..60 step 4; -1: tuned per window size)")
     ap.add_argument("--window-loop", type=int, default=-1, choices=(-1, 0, 1),
-                    help="device window loop for single-engine runs: -1 default (on), 0 off (host walks and draws), 1 on")
+                    help="where windows are walked and drawn: -1 default (host for a single engine, device for "
+                         "batched replicas), 0 host, 1 device")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()

@@ -498,11 +498,13 @@ int frmc_engine_delta_cells(frmc_engine *e, int32_t *nchunk_out, float *wcut_out
    windows of one size; the host synchronises once per batch and runs the
    last windows (fewer steps left than a window holds) itself.  Trajectory,
    logs, statistics and stream states are those of the host loop.
-   frmc_set_window_loop: 1 on, 0 or -1 (default) off: the host walks and
-   draws every window (for a single engine the device step costs as much
-   as the host's round trip between windows, about 7 us, and measured runs
-   were 2-6 % slower with it).  frmc_engine_loop_windows: windows the
-   device loop has run on the engine. */
+   frmc_engines_run_translation's batched replicas (single-atom groups)
+   take the device loop too: one step kernel per step walks and draws for
+   every replica.  frmc_set_window_loop: 1 on, 0 off, -1 (default) on for
+   batched replicas and off for a single engine (there the device step
+   costs as much as the host's round trip between windows, about 7 us, and
+   measured runs were 2-6 % slower with it).  frmc_engine_loop_windows:
+   windows the device loop has run on the engine. */
 int frmc_set_window_loop(int32_t mode);
 int frmc_engine_loop_windows(frmc_engine *e, int64_t *windows_out);
 /* window size of frmc_engine_run_translation: adaptive (default; k is the
