@@ -6,8 +6,9 @@ windows of 8 to 128 proposals, the tolerance branch, the four kinds of
 evaluation layout (64-Q units with k_eval_b, pairwise leaves combined by
 k_loop_step, with the split chain), a group count whose randrange rejects
 half of the words (many MT19937 blocks crossed per window), runs split over
-several calls with the host loop taking the last windows, and the host loop
-alone (the loop off)."""
+several calls with the host loop taking the last windows, the chunk walk
+with rebuilds between batches, groups out of atom order, the host loop alone
+(the loop off), and batched replicas on and off the device loop."""
 import ctypes
 import os
 import random
@@ -243,3 +244,49 @@ def test_batched_replicas_on_and_off_the_device_loop(loop_mode, mode):
                                     nsteps, seeds[0], tolerance=10.0)
     assert np.float32(reps[0].totalStandardError).tobytes() == np.float32(oe.tse).tobytes()
     assert reps[0].realCoordinates.tobytes() == oe.s.real.tobytes()
+
+
+def test_window_loop_with_the_chunk_walk(loop_mode):
+    """The device loop with k_delta's chunk walk (forced on for a 1,500-atom
+    periodic box): commits by the next k_delta keep the sorted copy, which the
+    host rebuilds between batches (every n/4 = 375 commits)."""
+    s = systems.random_box(1500, 26.0, 1.8, seed=171, species=("Si", "O"), fractions=(1 / 3, 2 / 3))
+    s["exp"] = pdc_target(s, 0.1, 172, 0.025, 0.05, 160)
+    n = len(s["real"])
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], oeng.Translation(0.2), _pdc(s), 2000, 173)
+    L = _lib.lib()
+    _lib.check(L.frmc_set_delta_cells(1))
+    try:
+        loop_mode(1)
+        random.seed(173)
+        np.random.seed(173)
+        E = device_engine(s, [PairDistributionConstraint(s["exp"])])
+        E.set_move_generator(TranslationGenerator(0.2))
+        dlog = []
+        E.run_device(2000, k=32, log=dlog)
+        nch = ctypes.c_int32(0)
+        _lib.check(L.frmc_engine_delta_cells(E.device_handle(), ctypes.byref(nch), None))
+        assert nch.value == (n + 255) // 256
+        assert loop_windows(E) > 0
+        assert_same_state(E, oe, dlog, olog, (pys, nps))
+        assert sum(a for _, a, _ in dlog) > 400
+    finally:
+        _lib.check(L.frmc_set_delta_cells(-1))
+
+
+def test_window_loop_groups_out_of_atom_order(loop_mode):
+    """Single-atom groups in a shuffled order over part of the atoms: the
+    draw looks each group's atom up in the group table."""
+    s = systems.random_box(600, 20.0, 1.8, seed=181)
+    s["exp"] = pdc_target(s, 0.1, 182, 0.025, 0.05, 150)
+    order = np.random.default_rng(183).permutation(600)[:450]
+    groups = [[int(i)] for i in order]
+    oe, olog, pys, nps = oracle_run(s, groups, oeng.Translation(0.2), _pdc(s), 1500, 184)
+    loop_mode(1)
+    random.seed(184)
+    np.random.seed(184)
+    E = device_engine(s, [PairDistributionConstraint(s["exp"])], groups=groups, gen=TranslationGenerator(0.2))
+    dlog = []
+    E.run_device(1500, k=32, log=dlog)
+    assert loop_windows(E) > 0
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
