@@ -1,5 +1,6 @@
 // window_loop.hip -- k_loop_step: one step of the device window loop for
-// single-atom translation runs, one workgroup.
+// single-atom translation runs, one workgroup (k_loop_step_batch: one per
+// replica of a batched run).
 //
 // The host queues, per window, k_loop_step, k_delta and the evaluation on the
 // engine stream, for a batch of windows of one size k, and synchronises once
@@ -15,16 +16,17 @@
 //      the last walked proposal's draws (the acceptance draw of an accepted
 //      proposal counts only if its total was worse), the state the
 //      sequential loop holds there, and
-//   3. draws the next window's k proposals from there exactly as the host
-//      loop does (RandomSelector.select_index, Selectors/RandomSelectors.py:
-//      87-94; generate_random_vector, Core/Collection.py:674-701; one
-//      acceptance draw each), and writes them into the proposal table
-//      k_delta reads.
+//   3. draws the next window's k proposals (fewer when fewer moves are
+//      left) from there exactly as the host loop does
+//      (RandomSelector.select_index, Selectors/RandomSelectors.py:87-94;
+//      generate_random_vector, Core/Collection.py:674-701; one acceptance
+//      draw each), and writes them into the proposal table k_delta reads.
 // Per stream the workgroup holds up to four consecutive state blocks in LDS
 // (block b+1 twisted out of block b, 3 dependent phases of independent
 // words).  The Python stream's word offsets depend on randrange's rejections:
-// wave 0 finds them with one ballot per proposal over 64 candidate words;
-// then each proposal is made by its own lane from its words.
+// wave 0 marks the usable words of 64-word tiles with one ballot each and
+// finds the proposals in a tile with scalar bit operations; then each
+// proposal is made by its own lane from its words.
 #include <algorithm>
 
 #include "draws.h"
@@ -53,9 +55,9 @@ __device__ __forceinline__ float res_value(const unsigned long long *w) {
 
 // Everything the step reads from memory is loaded in one parallel phase at
 // its start (state blocks, the walked window's results and proposals, the
-// combine programs) and once more for the drawn atoms (their coordinates):
-// the step sits between two windows on the stream, so its latency is what
-// it costs.
+// combine trees) and once more for the drawn atoms (their coordinates), and
+// everything it writes is stored at its end: the step sits between two
+// windows on the stream, so its latency is what it costs.
 __device__ void loop_step_body(const LoopArgs &A) {
     __shared__ uint32_t ring[2][kLoopRing][kMtN];
     extern __shared__ float s_res[];  // [wk][rs]: result words, then the combine tree's nodes
