@@ -268,7 +268,8 @@ constexpr int kDP = 1;
 // of the kernels in one copy.
 // other atoms per workgroup of k_delta_batch: 2,048 measured fastest on 256
 // replicas (3.65M moves/s; 512: 3.10M, 1,024: 3.62M, 4,096: 3.44M,
-// 8,192: 2.87M)
+// 8,192: 2.87M with host walks; again with walks and draws on the device:
+// 5.06M against 4.66M at 1,024 and 4.85M at 4,096)
 constexpr int32_t kChunk = 2048;
 // proposals per workgroup of k_delta_batch: 1 measured fastest on 256
 // replicas (3.21M moves/s; 2: 3.18M, 4: 2.64M -- fewer, longer workgroups
