@@ -271,10 +271,11 @@ constexpr int kDP = 1;
 // 8,192: 2.87M with host walks; again with walks and draws on the device:
 // 5.06M against 4.66M at 1,024 and 4.85M at 4,096)
 constexpr int32_t kChunk = 2048;
-// proposals per workgroup of k_delta_batch: 1 measured fastest on 256
-// replicas (3.21M moves/s; 2: 3.18M, 4: 2.64M -- fewer, longer workgroups
-// leave the latency-bound kernel short of waves)
-constexpr int32_t kBP = 1;
+// proposals per workgroup of k_delta_batch (sharing each other atom's
+// coordinate load): with walks and draws on the device, 256 replicas run
+// 5.41M moves/s at 2 against 5.05M at 1 and 4.68M at 4 (with host walks,
+// when the host bounded a step, 1 was fastest: 3.21M; 2: 3.18M; 4: 2.64M)
+constexpr int32_t kBP = 2;
 constexpr int32_t kBatchLdsInts = 8192;  // largest kBP delta blocks counted in LDS there (32 KB)
 struct BatchWin {
     int32_t rep;             // replica (index into the views)

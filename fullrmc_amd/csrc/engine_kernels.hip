@@ -1294,7 +1294,9 @@ __device__ __forceinline__ void delta_chunk_any(const View &V, const BatchWin &W
 
 // at most 128 VGPRs (four waves per SIMD): the unconstrained 130 allowed
 // three, and 256 replicas ran 8 % faster with four (3.57M against 3.30M
-// moves/s; five waves need a register spill)
+// moves/s; five waves need a register spill).  With two proposals per
+// workgroup the bound spills 57 VGPRs to scratch and still wins: 5.41M
+// against 4.99M moves/s with three waves and no spill.
 __global__ void __launch_bounds__(256, 4) k_delta_batch(BatchArgs A) {
     extern __shared__ int32_t bins[];  // kBP delta blocks (the largest that fits kBatchLdsInts)
     __shared__ BatchLds S;
