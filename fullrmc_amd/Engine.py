@@ -350,8 +350,14 @@ class Engine:
         if self.rigidConstraints:
             raise NotImplementedError("rigid (distance) constraints veto moves one at a time in Engine.run; "
                                       "the device engine evaluates the experimental pair constraints only")
-        if any(hasattr(g, "get_ready_for_move") or hasattr(g, "pick_from_list") for g in self._distinct_generators()):
-            raise NotImplementedError("swap and remove moves change their atoms per move; they run in Engine.run")
+        if any(hasattr(g, "pick_from_list") for g in self._distinct_generators()):
+            raise NotImplementedError("remove moves change the configuration's atoms; they run in Engine.run")
+        for grp, gen in zip(self.groups, self.generators):
+            if hasattr(gen, "get_ready_for_move") and (len(grp) != gen.swapLength or not len(gen.swapList)):
+                # Engine.run skips such a move without a draw (Engine.py:3192-3201), and
+                # with an empty swap list the group would be swapped with itself
+                raise NotImplementedError("a swap group whose size is not the swap length, or with an empty "
+                                          "swap list, runs in Engine.run")
         if self.collectedAtoms:
             raise NotImplementedError("a configuration with removed atoms runs in Engine.run")
         if self._dev is not None and self._dev_kmax >= kmax and self._dev_synced:
@@ -514,10 +520,18 @@ class Engine:
         step = 0
         windows = evaluated = accepted = 0
         C = len(self.constraints)
+        # swap moves act on the group plus drawn partners (Engine.py:3192-3201):
+        # their windows name each proposal's atoms
+        lists = any(hasattr(g, "get_ready_for_move") for g in self._distinct_generators())
+
         def propose():
             gi = self.selector.select_index(len(self.groups))
-            before = np.array(real[self.groups[gi]], dtype=FLOAT_TYPE)
-            return gi, self.generators[gi].move(before)
+            gen = self.generators[gi]
+            atoms = self.groups[gi]
+            if hasattr(gen, "get_ready_for_move"):
+                atoms = gen.get_ready_for_move(atoms, self)
+            before = np.array(real[atoms], dtype=FLOAT_TYPE)
+            return gi, atoms, gen.move(before)
 
         gen0 = acc0 = 0
         while step < nsteps:
@@ -526,21 +540,29 @@ class Engine:
             p_acc = (acc0 + 1.0) / (gen0 + 2.0)
             kw = min(k, nsteps - step, max(1, int(np.ceil(1.0 / p_acc))))
             gids = np.empty(kw, np.int32)
-            boxes, reals, u = [], [], []
+            boxes, reals, u, moved = [], [], [], []
             # the streams are saved once per window; an accepted proposal's
             # position is reached again by replaying the window's draws
             st0 = (random.getstate(), np.random.get_state())
             for p in range(kw):
-                gi, mr = propose()
+                gi, atoms, mr = propose()
                 mb = box_coordinates(self.reciprocalBasisVectors, mr)
                 u.append(random.random())
                 gids[p] = gi
                 boxes.append(mb)
                 reals.append(mr)
+                moved.append(np.asarray(atoms, dtype=np.int32))
             nb = np.ascontiguousarray(np.concatenate(boxes), dtype=np.float32)
             tse = np.empty(kw, np.float32)
-            _lib.check(L.frmc_engine_eval(h, kw, gids.ctypes.data_as(_lib.c_i32p), nb.ctypes.data_as(_lib.c_f32p),
-                                          tse.ctypes.data_as(_lib.c_f32p), None))
+            if lists:
+                counts = np.array([len(a) for a in moved], dtype=np.int32)
+                flat = np.ascontiguousarray(np.concatenate(moved), dtype=np.int32)
+                _lib.check(L.frmc_engine_eval_atoms(h, kw, counts.ctypes.data_as(_lib.c_i32p),
+                                                    flat.ctypes.data_as(_lib.c_i32p), nb.ctypes.data_as(_lib.c_f32p),
+                                                    tse.ctypes.data_as(_lib.c_f32p), None))
+            else:
+                _lib.check(L.frmc_engine_eval(h, kw, gids.ctypes.data_as(_lib.c_i32p),
+                                              nb.ctypes.data_as(_lib.c_f32p), tse.ctypes.data_as(_lib.c_f32p), None))
             windows += 1
             evaluated += kw
             cur = np.float32(self.totalStandardError)
@@ -574,7 +596,7 @@ class Engine:
                 self.accepted += 1
                 accepted += 1
                 self.totalStandardError = tse[acc]
-                real[self.groups[gids[acc]]] = reals[acc]
+                real[moved[acc]] = reals[acc]
         # the native mirror of real coordinates is refreshed from this loop
         r = np.ascontiguousarray(real, dtype=np.float32)
         _lib.check(L.frmc_engine_set_real(h, r.ctypes.data_as(_lib.c_f32p)))

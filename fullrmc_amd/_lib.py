@@ -120,6 +120,7 @@ _PROTOS = {
     "frmc_engine_compute_data": [c_vp, c_f32p, c_f32p],
     "frmc_engine_refresh": [c_vp, c_f32p, c_f32p],
     "frmc_engine_eval": [c_vp, c_i32, c_i32p, c_f32p, c_f32p, c_f32p],
+    "frmc_engine_eval_atoms": [c_vp, c_i32, c_i32p, c_i32p, c_f32p, c_f32p, c_f32p],
     "frmc_engine_commit": [c_vp, c_i32],
     "frmc_engine_sync": [c_vp],
     "frmc_engine_get_state": [c_vp, c_f32p, c_i32, c_i32p, c_i32p, c_f32p, c_f32p],

@@ -59,12 +59,14 @@ struct DevC {
 struct Stage {
     int32_t *pgroup;  // [kmax]
     int32_t *poff;    // [kmax+1]
-    float *pold;      // [kmax*gmax*3]
-    float *pnew;      // [kmax*gmax*3]
+    int32_t *patoms;  // [kmax*lcap] the moved atoms of each proposal (windows of atom lists)
+    float *pold;      // [kmax*lcap*3]
+    float *pnew;      // [kmax*lcap*3]
     void *host = nullptr, *dev = nullptr;  // the two blocks
     size_t bytes = 0;
-    int32_t *d_pgroup = nullptr, *d_poff = nullptr;
+    int32_t *d_pgroup = nullptr, *d_poff = nullptr, *d_patoms = nullptr;
     float *d_pold = nullptr, *d_pnew = nullptr;
+    bool lists = false;  // the staged window names its atoms per proposal (patoms), not by group
 };
 
 struct Prop {  // one single-atom proposal: atom, element, old and new box coordinates
@@ -85,6 +87,7 @@ struct View {
     int32_t Hmax, Mmax, maxLeaves, nunits;
     // window
     const int32_t *pgroup, *poff;  // staging (host-mapped)
+    const int32_t *patoms;         // the moved atoms per proposal (at poff), or null: the group's
     const float *pold, *pnew;
     int32_t *dn;                   // delta buffer of this window
     int32_t *dn_zero;              // the other buffer, zeroed during this window's evaluation
@@ -321,6 +324,7 @@ struct frmc_engine {
     hipStream_t st = nullptr;
     bool own_stream = false;
     int32_t n = 0, nel = 0, T = 0, C = 0, G = 0, isPBC = 0, ortho = 0, kmax = 0, gmax = 0, total_gatoms = 0;
+    int32_t lcap = 0;  // atoms per proposal the staging holds: min(kMaxG, 2 gmax) (a swap moves group + partner)
     int32_t ncu = 256;
     int device = 0;                  // the HIP device the engine's memory and stream live on
     frmc::Basis B;

@@ -50,6 +50,7 @@ View make_view(frmc_engine *e, int s) {
     V.nunits = e->nunits;
     V.pgroup = e->stage[s].d_pgroup;
     V.poff = e->stage[s].d_poff;
+    V.patoms = e->stage[s].lists ? e->stage[s].d_patoms : nullptr;
     V.pold = e->stage[s].d_pold;
     V.pnew = e->stage[s].d_pnew;
     V.dn = e->dnbuf[e->cur_dn];
@@ -70,7 +71,7 @@ View make_view(frmc_engine *e, int s) {
     V.cbox = e->cbox;
     V.spos = e->spos;
     V.nchunk = e->nchunk;
-    V.prune = sorted ? 1 : 0;
+    V.prune = sorted && !e->stage[s].lists ? 1 : 0;  // the chunk walk takes single-atom groups
     for (int d = 0; d < 3; d++) V.hsp[d] = e->hsp[d];
     V.wcut = e->wcut;
     V.pend_dev = nullptr;
@@ -633,6 +634,7 @@ static int run_window(frmc_engine *e, int s, int32_t k, const Inline *I, const C
 static int stage_host(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box, int &s) {
     s = 1 - e->cur_stage;
     Stage &S = e->stage[s];
+    S.lists = false;
     int32_t off = 0;
     for (int32_t p = 0; p < k; p++) {
         const int32_t gi = group_ids[p];
@@ -648,6 +650,36 @@ static int stage_host(frmc_engine *e, int32_t k, const int32_t *group_ids, const
     }
     S.poff[k] = off;
     memcpy(S.pnew, new_box, sizeof(float) * 3 * (size_t)off);
+    e->cur_stage = s;
+    return FRMC_OK;
+}
+
+// Staging for proposals that name their moved atoms (counts[p] atoms each,
+// listed in order in atoms): moves whose atoms are not one group, such as a
+// swap (the group and its drawn partners).
+static int stage_host_atoms(frmc_engine *e, int32_t k, const int32_t *counts, const int32_t *atoms,
+                            const float *new_box, int &s) {
+    s = 1 - e->cur_stage;
+    Stage &S = e->stage[s];
+    int32_t off = 0;
+    for (int32_t p = 0; p < k; p++) {
+        const int32_t g = counts[p];
+        FRMC_REQUIRE(g >= 1 && g <= e->lcap, "proposal %d: %d atoms (at most %d)", p, g, e->lcap);
+        S.pgroup[p] = -1;
+        S.poff[p] = off;
+        for (int32_t a = 0; a < g; a++) {
+            const int32_t id = atoms[off + a];
+            FRMC_REQUIRE(id >= 0 && id < e->n, "proposal %d: atom %d out of range [0,%d)", p, id, e->n);
+            for (int32_t b = 0; b < a; b++)
+                FRMC_REQUIRE(atoms[off + b] != id, "proposal %d: atom %d listed twice", p, id);
+            S.patoms[off + a] = id;
+            for (int d = 0; d < 3; d++) S.pold[3 * (size_t)(off + a) + d] = e->h_box[3 * (size_t)id + d];
+        }
+        off += g;
+    }
+    S.poff[k] = off;
+    memcpy(S.pnew, new_box, sizeof(float) * 3 * (size_t)off);
+    S.lists = true;
     e->cur_stage = s;
     return FRMC_OK;
 }
@@ -795,9 +827,9 @@ static int stage_and_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, c
 static void note_commit(frmc_engine *e, int32_t p) {
     const Stage &S = e->stage[e->cur_stage];
     const int32_t gi = S.pgroup[p], a0 = S.poff[p];
-    const int32_t g0 = e->h_goff[gi], g = e->h_goff[gi + 1] - g0;
+    const int32_t g0 = S.lists ? 0 : e->h_goff[gi], g = S.lists ? S.poff[p + 1] - a0 : e->h_goff[gi + 1] - g0;
     for (int32_t a = 0; a < g; a++) {
-        const size_t id = (size_t)e->h_gatoms[g0 + a];
+        const size_t id = (size_t)(S.lists ? S.patoms[a0 + a] : e->h_gatoms[g0 + a]);
         for (int d = 0; d < 3; d++) {
             const float x = S.pnew[3 * (size_t)(a0 + a) + d];
             e->h_box[3 * id + d] = x;
@@ -1231,6 +1263,7 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             if (g < 1 || g > kMaxG) EFAIL(FRMC_EINVAL, "group %d has %d atoms (1..%d supported)", gi, g, kMaxG);
             e->gmax = std::max(e->gmax, g);
         }
+        e->lcap = std::min(kMaxG, 2 * e->gmax);
         e->total_gatoms = e->h_goff[d->ngroups];
         e->h_gatoms.assign(d->group_atoms, d->group_atoms + e->total_gatoms);
         for (int32_t a : e->h_gatoms)
@@ -1364,7 +1397,10 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             ECHK(dalloc(e, &e->dnbuf[b], K * e->dnStride));
             ECHK(hipMemset(e->dnbuf[b], 0, sizeof(int32_t) * K * e->dnStride));
             Stage &S = e->stage[b];
-            const size_t ni = 2 * K + 2, nf = 2 * K * (size_t)e->gmax * 3;  // K+1 offsets padded to even
+            // group ids, K+1 offsets (padded to even), the atom lists, old and
+            // new box coordinates
+            const size_t L = (size_t)e->lcap;
+            const size_t ni = 2 * K + 2 + ((K * L + 1) & ~(size_t)1), nf = 2 * K * L * 3;
             S.bytes = 4 * (ni + nf);
             char *hb = nullptr, *db = nullptr;
             ECHK(halloc(e, &hb, S.bytes));
@@ -1373,12 +1409,14 @@ int frmc_engine_create(frmc_engine **out, const frmc_engine_desc *d) {
             S.dev = db;
             S.pgroup = (int32_t *)hb;
             S.poff = S.pgroup + K;
+            S.patoms = S.poff + K + 2;
             S.pold = (float *)(hb + 4 * ni);
-            S.pnew = S.pold + K * (size_t)e->gmax * 3;
+            S.pnew = S.pold + K * L * 3;
             S.d_pgroup = (int32_t *)db;
             S.d_poff = S.d_pgroup + K;
+            S.d_patoms = S.d_poff + K + 2;
             S.d_pold = (float *)(db + 4 * ni);
-            S.d_pnew = S.d_pold + K * (size_t)e->gmax * 3;
+            S.d_pnew = S.d_pold + K * L * 3;
         }
         ECHK(dalloc(e, &e->sqbuf, K * e->C * e->Mmax));
         ECHK(dalloc(e, &e->chi2d, K * e->C));
@@ -1460,6 +1498,31 @@ int frmc_engine_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const 
     FRMC_REQUIRE(group_ids && new_box && tse_out, "NULL argument");
     if (int rc0 = flush_commit(e)) return rc0;
     int rc = stage_and_eval(e, k, group_ids, new_box);
+    if (rc) return rc;
+    memcpy(tse_out, e->h_tse.data(), sizeof(float) * k);
+    if (chi2_out) memcpy(chi2_out, e->h_chi2.data(), sizeof(float) * k * e->C);
+    return FRMC_OK;
+}
+
+int frmc_engine_eval_atoms(frmc_engine *e, int32_t k, const int32_t *counts, const int32_t *atoms,
+                           const float *new_box, float *tse_out, float *chi2_out) {
+    FRMC_REQUIRE(e, "NULL engine");
+    FRMC_REQUIRE(k >= 1 && k <= e->kmax, "k=%d out of range [1,%d]", k, e->kmax);
+    FRMC_REQUIRE(counts && atoms && new_box && tse_out, "NULL argument");
+    if (int rc0 = flush_commit(e)) return rc0;
+    if (int rc0 = update_cells(e)) return rc0;
+    int s = 0;
+    if (int rc0 = stage_host_atoms(e, k, counts, atoms, new_box, s)) return rc0;
+    Stage &S = e->stage[s];
+    FRMC_CHECK_HIP(hipMemcpyAsync(S.dev, S.host, S.bytes, hipMemcpyHostToDevice, e->st));
+    Inline &I = e->last_inline;
+    I.k = 0;
+    e->win_dn = e->dnbuf[e->cur_dn];
+    e->deferred = false;
+    e->device_decided = false;
+    int rc = launch_window(e, s, k, &I);
+    if (rc) return rc;
+    rc = complete_eval(e, k);
     if (rc) return rc;
     memcpy(tse_out, e->h_tse.data(), sizeof(float) * k);
     if (chi2_out) memcpy(chi2_out, e->h_chi2.data(), sizeof(float) * k * e->C);

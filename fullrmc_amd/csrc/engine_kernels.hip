@@ -75,7 +75,9 @@ __device__ __forceinline__ void load_group(const View &V, int32_t p, GroupLds &G
     const int32_t gi = load_dev(V.pgroup + p);
     const int32_t a0 = load_dev(V.poff + p);
     const int32_t g = load_dev(V.poff + p + 1) - a0;
-    const int32_t *atoms = V.gatoms + V.goff[gi];
+    // the proposal's own atom list (moves of atoms that are not one group),
+    // or its group's atoms
+    const int32_t *atoms = V.patoms ? V.patoms + a0 : V.gatoms + V.goff[gi];
     if (threadIdx.x == 0) G.g = g;
     for (int32_t a = threadIdx.x; a < g; a += blockDim.x) {
         const int32_t id = atoms[a];

@@ -394,6 +394,16 @@ int frmc_engine_refresh(frmc_engine *e, float *chi2_out, float *tse_out);
    chi-squares (chi2_out[k][C], may be NULL).  Synchronous. */
 int frmc_engine_eval(frmc_engine *e, int32_t k, const int32_t *group_ids, const float *new_box, float *tse_out,
                      float *chi2_out);
+/* A window whose proposals name their moved atoms (no reference
+   counterpart as one call: the moves of Engine.run whose atoms are not one
+   group, the swap's group plus its drawn partners, Engine.py:3192-3201 and
+   Core/MoveGenerator.py:502-529, evaluated as frmc_engine_eval evaluates
+   group moves).  Proposal p moves counts[p] distinct atoms (at most twice
+   the largest group, 64 at most), listed in order in atoms, to the box
+   coordinates at the same place in new_box; frmc_engine_commit commits one
+   of them. */
+int frmc_engine_eval_atoms(frmc_engine *e, int32_t k, const int32_t *counts, const int32_t *atoms,
+                           const float *new_box, float *tse_out, float *chi2_out);
 /* accept proposal p of the last evaluation (accept_move, PDC:1131-1152) */
 int frmc_engine_commit(frmc_engine *e, int32_t p);
 int frmc_engine_sync(frmc_engine *e);

@@ -371,8 +371,40 @@ def test_swap_moves_in_dropin_loop():
     assert_same_state(E, oe, dlog, olog, (pys, nps))
     swaps = [a for g, a, _ in dlog if el[g] == "Fe"]
     assert len(swaps) > 20 and any(swaps)
-    with pytest.raises(NotImplementedError):
-        E.run_device(10)
+
+
+@pytest.mark.parametrize("cells", [0, 1])
+def test_swap_moves_on_the_device_engine(cells):
+    """The same swap and translation moves in speculative windows on the
+    device engine: each proposal names its moved atoms (a swap's group plus
+    its drawn partner), evaluated and committed there.  cells=1 keeps k_delta's
+    sorted atom copy (kept by the commits; the windows of atom lists take the
+    plain k_delta)."""
+    from fullrmc_amd import _lib
+    s = systems.b2_alloy(cells=(5, 5, 5), a=3.01, sigma=0.08, seed=13)
+    exp = pdc_target(s, 0.08, 14, 0.01, 0.02, 300)
+    n = len(s["real"])
+    el = s["elements"]
+    al = [[j] for j in range(n) if el[j] == "Al"]
+    gp = [SwapPositionsGenerator(swapList=al) if el[i] == "Fe" else TranslationGenerator(0.1) for i in range(n)]
+    go = [oeng.Swap(al) if el[i] == "Fe" else oeng.Translation(0.1) for i in range(n)]
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], go, _pdc_cons_of({"exp": exp}), 600, 22,
+                                    tolerance=5.0)
+    L = _lib.lib()
+    _lib.check(L.frmc_set_delta_cells(cells))
+    try:
+        random.seed(22)
+        np.random.seed(22)
+        E = device_engine(s, [PairDistributionConstraint(exp)], groups=[[i] for i in range(n)], gen=gp)
+        E.set_tolerance(5.0)
+        dlog = []
+        st = E.run_device(600, k=16, log=dlog)
+    finally:
+        _lib.check(L.frmc_set_delta_cells(-1))
+    assert st["windows"] < 600
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    swaps = [a for g, a, _ in dlog if el[g] == "Fe"]
+    assert len(swaps) > 50 and sum(swaps) > 3
 
 
 def test_remove_moves_amputation_in_dropin_loop():
