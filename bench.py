@@ -693,6 +693,44 @@ def headline(args):
     return res
 
 
+def swap_moves_record(start, pdc, sfc):
+    """Swap moves on the same B2 box: the first element's atoms exchange
+    positions with a randomly drawn atom of the second (SwapPositionsGenerator),
+    the second element's atoms translate by 0.1 A.  The device engine in
+    speculative windows of Python-drawn proposals (each naming its moved
+    atoms), and the reference's loop over the drop-in calls."""
+    import torch
+    from fullrmc_amd.Generators import SwapPositionsGenerator, TranslationGenerator
+    els = start["elements"]
+    n = len(els)
+    first, second = sorted(set(els))[:2]
+    partners = [[j] for j in range(n) if els[j] == second]
+
+    def engine():
+        E = make_engine(start, pdc, sfc, seed=2)
+        sw, tr = SwapPositionsGenerator(swapList=partners), TranslationGenerator(0.1)
+        E.set_groups([[i] for i in range(n)], [sw if els[i] == first else tr for i in range(n)])
+        return E
+
+    E = engine()
+    E.run_device(200, k=16, pull=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = E.run_device(2000, k=16, pull=False)
+    torch.cuda.synchronize()
+    dev = 2000 / (time.perf_counter() - t0)
+    E._drop_device()
+    E2 = engine()
+    E2.run(2)
+    t1 = time.perf_counter()
+    E2.run(30)
+    dropin = 30 / (time.perf_counter() - t1)
+    return {"workload": "B2 alloy 13500 atoms: %s atoms swap positions with a drawn %s atom, %s atoms translate "
+                        "0.1 A; PDC + SFC as above" % (first, second, second),
+            "device_engine_moves_per_s": round(dev, 1), "windows": st["windows"],
+            "dropin_per_move_loop_moves_per_s": round(dropin, 1)}
+
+
 def b2_section(args):
     """The 13,500-atom B2 alloy: engine moves (k <= 64), plain form, the
     reference's loop over the drop-in calls, full histogram, CPU oracle."""
@@ -726,6 +764,8 @@ def b2_section(args):
     t2 = time.perf_counter()
     E2.run(30)
     out["dropin_per_move_loop_moves_per_s"] = round(30 / (time.perf_counter() - t2), 1)
+    E2._drop_device()
+    out["swap_moves"] = swap_moves_record(start, pdc, sfc)
     if not args.no_cpu:
         oe = cpu_engine(start, pdc, sfc, 0.02)
         cps, cn, cel = cpu_moves_per_s(oe, args.cpu_seconds / 2)
