@@ -350,9 +350,11 @@ class Engine:
         if self.rigidConstraints:
             raise NotImplementedError("rigid (distance) constraints veto moves one at a time in Engine.run; "
                                       "the device engine evaluates the experimental pair constraints only")
-        if any(hasattr(g, "pick_from_list") for g in self._distinct_generators()):
+        distinct = self._distinct_generators()
+        if any(hasattr(g, "pick_from_list") for g in distinct):
             raise NotImplementedError("remove moves change the configuration's atoms; they run in Engine.run")
-        for grp, gen in zip(self.groups, self.generators):
+        swaps = any(hasattr(g, "get_ready_for_move") for g in distinct)
+        for grp, gen in zip(self.groups, self.generators) if swaps else ():
             if hasattr(gen, "get_ready_for_move") and (len(grp) != gen.swapLength or not len(gen.swapList)):
                 # Engine.run skips such a move without a draw (Engine.py:3192-3201), and
                 # with an empty swap list the group would be swapped with itself
