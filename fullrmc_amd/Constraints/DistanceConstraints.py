@@ -202,6 +202,11 @@ class _MolecularDistanceConstraint:
             return random.random() < self.rejectProbability
         return bool(np.any(self.activeAtomsDataAfterMove["number"] > self.activeAtomsDataBeforeMove["number"]))
 
+    def rejection_draws(self, standardError):
+        """The Python random draws should_step_get_rejected makes for this
+        standard error: one when a flexible constraint's error grows."""
+        return 1 if self.flexible and not standardError <= self.standardError else 0
+
     def accept_move(self, realIndexes, relativeIndexes):
         """:739-758"""
         self.data = self._newData

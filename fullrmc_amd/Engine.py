@@ -347,9 +347,6 @@ class Engine:
         """Create (or reuse) the device engine holding the current state."""
         from . import _device
         _device.require_gpu()
-        if self.rigidConstraints:
-            raise NotImplementedError("rigid (distance) constraints veto moves one at a time in Engine.run; "
-                                      "the device engine evaluates the experimental pair constraints only")
         distinct = self._distinct_generators()
         if any(hasattr(g, "pick_from_list") for g in distinct):
             raise NotImplementedError("remove moves change the configuration's atoms; they run in Engine.run")
@@ -462,15 +459,20 @@ class Engine:
 
         With translation moves on every group (the default generator) the
         whole loop, random streams included, runs in the native library;
-        otherwise proposals come from the Python generators (the reference's
-        own NumPy calls) and only the evaluation runs on the device.
+        otherwise, or with rigid (distance) constraints, proposals come from
+        the Python generators (the reference's own NumPy calls), the rigid
+        constraints veto them in Python, and the evaluation runs on the
+        device.
         ``pull=False`` leaves the state on the device (the host mirror is
         refreshed by the next :meth:`run`, :meth:`sync_from_device` or a
         pulling call).  Returns a dict of counters and timings."""
         h = self._device_engine(max(k, 1))
         distinct = self._distinct_generators()
         gen0 = distinct[0]
-        nat = native and len(distinct) == 1 and getattr(gen0, "native_kind", None) == "translation"
+        # rigid constraints veto moves with draws of their own, so their runs
+        # take the Python-drawn windows
+        nat = (native and not self.rigidConstraints and len(distinct) == 1
+               and getattr(gen0, "native_kind", None) == "translation")
         t0 = time.perf_counter()
         if nat:
             stats = self._run_native(h, int(numberOfSteps), int(k), gen0, log)
@@ -516,12 +518,21 @@ class Engine:
         return dict(generated=st.generated, accepted=st.accepted, windows=st.windows, evaluated=st.evaluated)
 
     def _run_python(self, h, nsteps, k, log):
-        """Speculative windows with proposals from the Python generators."""
+        """Speculative windows with proposals from the Python generators.
+
+        Each proposal reserves the acceptance draw a rejected worse move
+        makes (Engine.py:3300-3310).  Rigid constraints veto a proposal
+        before it is tried (Engine.py:3278-3296): the veto is decided while
+        the window is drawn, in order and on the live streams, since it
+        depends only on the configuration the window starts from; a vetoed
+        proposal reserves no acceptance draw, and the draws a flexible veto
+        makes are replayed with the proposal's own."""
         L = _lib.lib()
+        self._pull_device()
         real = self.realCoordinates
         step = 0
         windows = evaluated = accepted = 0
-        C = len(self.constraints)
+        rigid = self.rigidConstraints
         # swap moves act on the group plus drawn partners (Engine.py:3192-3201):
         # their windows name each proposal's atoms
         lists = any(hasattr(g, "get_ready_for_move") for g in self._distinct_generators())
@@ -535,6 +546,20 @@ class Engine:
             before = np.array(real[atoms], dtype=FLOAT_TYPE)
             return gi, atoms, gen.move(before)
 
+        def veto(atoms, mb):
+            """The rigid constraints in turn; returns (vetoed, constraints
+            evaluated, Python random draws made)."""
+            draws = 0
+            for ci, c in enumerate(rigid):
+                c.compute_before_move(realIndexes=atoms, relativeIndexes=atoms)
+                c.compute_after_move(realIndexes=atoms, relativeIndexes=atoms, movedBoxCoordinates=mb)
+                draws += c.rejection_draws(c.afterMoveStandardError)
+                if c.should_step_get_rejected(c.afterMoveStandardError):
+                    for c2 in rigid:
+                        c2.reject_move(realIndexes=atoms, relativeIndexes=atoms)
+                    return True, ci + 1, draws
+            return False, len(rigid), draws
+
         gen0 = acc0 = 0
         while step < nsteps:
             # proposals drawn by the Python generators cost more than a window
@@ -542,14 +567,17 @@ class Engine:
             p_acc = (acc0 + 1.0) / (gen0 + 2.0)
             kw = min(k, nsteps - step, max(1, int(np.ceil(1.0 / p_acc))))
             gids = np.empty(kw, np.int32)
-            boxes, reals, u, moved = [], [], [], []
+            boxes, reals, u, moved, vetoed = [], [], [], [], []
             # the streams are saved once per window; an accepted proposal's
             # position is reached again by replaying the window's draws
             st0 = (random.getstate(), np.random.get_state())
+            tried0 = [c.tried for c in rigid]
             for p in range(kw):
                 gi, atoms, mr = propose()
                 mb = box_coordinates(self.reciprocalBasisVectors, mr)
-                u.append(random.random())
+                v = veto(atoms, mb) if rigid else (False, 0, 0)
+                vetoed.append(v)
+                u.append(None if v[0] else random.random())
                 gids[p] = gi
                 boxes.append(mb)
                 reals.append(mr)
@@ -568,11 +596,18 @@ class Engine:
             windows += 1
             evaluated += kw
             cur = np.float32(self.totalStandardError)
-            acc = -1
+            acc = last = -1
             for p in range(kw):
                 self.generated += 1
-                self.tried += 1
                 gen0 += 1
+                step += 1
+                last = p
+                if vetoed[p][0]:
+                    # never tried (Engine.py:3290-3296)
+                    if log is not None:
+                        log.append((int(gids[p]), False, float(cur)))
+                    continue
+                self.tried += 1
                 ok, drew = True, False
                 if tse[p] > cur:
                     drew = True
@@ -580,25 +615,39 @@ class Engine:
                         ok = False
                     else:
                         self.tolerated += 1
-                step += 1
                 if log is not None:
                     log.append((int(gids[p]), ok, float(tse[p] if ok else cur)))
                 if ok:
                     acc = p
                     acc0 += 1
                     break
+            # each rigid constraint counts the proposals it evaluated up to
+            # the last one walked, as the per-move loop would
+            for ci, c in enumerate(rigid):
+                c.tried = tried0[ci] + sum(1 for p in range(last + 1) if vetoed[p][1] > ci)
             if acc >= 0:
                 random.setstate(st0[0])
                 np.random.set_state(st0[1])
                 for p in range(acc + 1):
                     propose()
-                    if p < acc or drew:
+                    for _ in range(vetoed[p][2]):
                         random.random()
+                    if (p < acc and not vetoed[p][0]) or (p == acc and drew):
+                        random.random()
+                atoms = moved[acc]
+                for c in rigid:
+                    # the accepted move's data, then the rigid constraints take it
+                    t = c.tried
+                    c.compute_before_move(realIndexes=atoms, relativeIndexes=atoms)
+                    c.compute_after_move(realIndexes=atoms, relativeIndexes=atoms, movedBoxCoordinates=boxes[acc])
+                    c.tried = t
+                    c.accept_move(realIndexes=atoms, relativeIndexes=atoms)
                 _lib.check(L.frmc_engine_commit(h, acc))
                 self.accepted += 1
                 accepted += 1
                 self.totalStandardError = tse[acc]
-                real[moved[acc]] = reals[acc]
+                real[atoms] = reals[acc]
+                self.boxCoordinates[atoms] = boxes[acc]
         # the native mirror of real coordinates is refreshed from this loop
         r = np.ascontiguousarray(real, dtype=np.float32)
         _lib.check(L.frmc_engine_set_real(h, r.ctypes.data_as(_lib.c_f32p)))
@@ -663,6 +712,8 @@ def run_replicas(engines, numberOfSteps=100, k=32, seeds=None, batched=True):
         g0 = distinct[0]
         if not (len(distinct) == 1 and getattr(g0, "native_kind", None) == "translation"):
             raise ValueError("replicas need translation moves on every group")
+        if E.rigidConstraints:
+            raise ValueError("replicas run the native loop, which has no rigid (distance) constraints")
         if (float(g0.amplitude[0]), float(g0.amplitude[1])) != amp or float(E.tolerance) != tol:
             raise ValueError("replicas need the same move amplitude and tolerance")
     R = len(engines)

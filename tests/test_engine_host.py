@@ -86,3 +86,21 @@ def test_rotation_of_one_atom_raises_before_any_draw():
     else:
         raise AssertionError("a rotation on a one-atom group must be refused")
     E.set_groups([[0, 1], [1, 2]], RotationGenerator(10))
+
+
+def test_replicas_refuse_rigid_constraints():
+    """Replicas run the native translation loop, which has no veto: an engine
+    with a distance constraint is refused before anything runs."""
+    from fullrmc_amd import systems
+    from fullrmc_amd.Constraints import InterMolecularDistanceConstraint
+
+    s = systems.random_box(20, 10.0, 1.5, seed=3)
+    E = eng.Engine()
+    E.set_system(s["real"], s["elements"], s["molecules"], s["basis"], s["recip"])
+    E.add_constraints([InterMolecularDistanceConstraint(1.5)])
+    try:
+        eng.run_replicas([E], 10)
+    except ValueError as e:
+        assert "rigid" in str(e)
+    else:
+        raise AssertionError("run_replicas accepted an engine with a rigid constraint")

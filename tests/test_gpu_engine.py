@@ -313,8 +313,67 @@ def test_intermolecular_distance_constraint_vetoes_in_dropin_loop(small_triclini
     assert np.float32(c.standardError).tobytes() == np.float32(oc.sd).tobytes()
     assert E.tried < E.generated  # some moves were vetoed before being tried
     assert oc.sd > 0
-    with pytest.raises(NotImplementedError):
-        E.run_device(10)
+
+
+@pytest.mark.parametrize("kind,flexible,rp", [("inter", True, 1.0), ("inter", True, 0.5), ("inter", False, 1.0),
+                                               ("both", True, 0.5), ("both", False, 1.0)])
+def test_distance_constraints_veto_on_the_device_engine(small_triclinic, kind, flexible, rp):
+    """Rigid distance constraints on the device engine: the veto is decided
+    in Python as each window is drawn (a flexible constraint whose error
+    grows draws once, and with rejectProbability 0.5 half of those moves go
+    on to the accept rule and its own draw; a strict one draws nothing), the
+    experimental constraints are evaluated on the device.  The trajectory,
+    streams and final state equal the oracle's, and the counters equal the
+    per-move drop-in loop's."""
+    s = small_triclinic
+    exp = pdc_target(s, 0.1, 31, 0.025, 0.05, 200)
+    n = len(s["real"])
+    inter_lim = {("C", "H"): 2.2, ("H", "H"): 2.4}
+    intra_lim = {("H", "H"): 1.7, ("C", "H"): 1.05}
+
+    def rigid_oracle(o):
+        r = [oeng.IMD(o, 2.6, inter_lim, flexible=flexible, rejectProbability=rp)]
+        if kind == "both":
+            r.append(oeng.IMD(o, 1.0, intra_lim, flexible=flexible, rejectProbability=rp, inter=False))
+        return r
+
+    def rigid_engine():
+        r = [InterMolecularDistanceConstraint(2.6, pairsDistanceDefinition=[(a, b, d) for (a, b), d in inter_lim.items()],
+                                              flexible=flexible, rejectProbability=rp)]
+        if kind == "both":
+            r.append(IntraMolecularDistanceConstraint(1.0, typeDefinition="element",
+                                                      pairsDistanceDefinition=[(a, b, d) for (a, b), d in
+                                                                               intra_lim.items()],
+                                                      flexible=flexible, rejectProbability=rp))
+        return r
+
+    nsteps = 300
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], oeng.Translation(0.4), _pdc_cons_of({"exp": exp}),
+                                    nsteps, 23, rigid_fn=rigid_oracle)
+    random.seed(23)
+    np.random.seed(23)
+    E = device_engine(s, [PairDistributionConstraint(exp)] + rigid_engine())
+    E.set_move_generator(TranslationGenerator(0.4))
+    dlog = []
+    st = E.run_device(nsteps, k=16, log=dlog)
+    assert st["windows"] < nsteps
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    for c, oc in zip(E.rigidConstraints, oe.rigid):
+        assert c.data["number"].tobytes() == oc.number.tobytes()
+        assert c.data["distanceSum"].tobytes() == oc.dsum.tobytes()
+        assert np.float32(c.standardError).tobytes() == np.float32(oc.sd).tobytes()
+    # the per-move drop-in loop from the same start: the same counters
+    random.seed(23)
+    np.random.seed(23)
+    P = device_engine(s, [PairDistributionConstraint(exp)] + rigid_engine())
+    P.set_move_generator(TranslationGenerator(0.4))
+    plog = []
+    P.run(nsteps, log=plog)
+    assert plog == dlog
+    assert (E.generated, E.tried, E.accepted, E.tolerated) == (P.generated, P.tried, P.accepted, P.tolerated)
+    assert [(c.tried, c.accepted) for c in E.rigidConstraints] == [(c.tried, c.accepted) for c in P.rigidConstraints]
+    assert E.tried < E.generated  # some moves were vetoed before being tried
+    assert E.accepted > 10
 
 
 @pytest.mark.parametrize("flexible", [True, False])
