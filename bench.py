@@ -445,6 +445,7 @@ def molecular_section(args):
                        "r[0,15]@0.05, molecule translation 0.2 A + rotation 10 deg (Python generators), k<=16",
            "moves_per_s": round(args.molecular_moves / el, 1), "moves": args.molecular_moves,
            "accepted_fraction": round(st["accepted"] / args.molecular_moves, 4), "windows": st["windows"]}
+    out["distance_veto"] = distance_veto_record(args, start, groups, pdc)
     if not args.no_cpu:
         random.seed(5)
         np.random.seed(5)
@@ -459,6 +460,52 @@ def molecular_section(args):
         out["speedup_vs_cpu"] = round(out["moves_per_s"] / rate, 1)
     print("bench: molecular done (%.1f moves/s)" % out["moves_per_s"], file=sys.stderr, flush=True)
     return out
+
+
+def distance_veto_record(args, start, groups, pdc):
+    """The molecular workload with an inter-molecular distance constraint
+    (2.4 A between atoms of different molecules, flexible): every proposal
+    is vetoed or passed in Python (the constraint's own drop-in atomic
+    distance calls) as its window is drawn, the PDC evaluated on the device;
+    beside it the per-move drop-in loop on the same start."""
+    import torch
+    from fullrmc_amd.Constraints import InterMolecularDistanceConstraint, PairDistributionConstraint
+    from fullrmc_amd.Engine import Engine
+    from fullrmc_amd.Generators import MoveGeneratorCombinator, RotationGenerator, TranslationGenerator
+
+    def engine():
+        E = Engine()
+        E.set_system(start["real"], start["elements"], start["molecules"], start["basis"], start["recip"])
+        E.set_groups(groups, MoveGeneratorCombinator([TranslationGenerator(0.2), RotationGenerator(10)]))
+        E.add_constraints([PairDistributionConstraint(pdc), InterMolecularDistanceConstraint(2.4)])
+        return E
+
+    n_dev = max(1, args.molecular_moves // 4)
+    n_loop = max(1, args.molecular_moves // 10)
+    random.seed(6)
+    np.random.seed(6)
+    E = engine()
+    E.run_device(20, k=16, pull=False)
+    torch.cuda.synchronize()
+    g0, t0_ = E.generated, E.tried
+    t0 = time.perf_counter()
+    st = E.run_device(n_dev, k=16, pull=False)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    vetoed = (E.generated - g0) - (E.tried - t0_)
+    random.seed(6)
+    np.random.seed(6)
+    P = engine()
+    P.run(20)
+    t0 = time.perf_counter()
+    P.run(n_loop)
+    el_p = time.perf_counter() - t0
+    print("bench: distance veto done (%.1f moves/s)" % (n_dev / el), file=sys.stderr, flush=True)
+    return {"workload": "the molecular workload plus InterMolecularDistanceConstraint(2.4 A, flexible): the veto "
+                        "in Python per proposal as each window is drawn, the PDC on the device",
+            "moves_per_s": round(n_dev / el, 1), "moves": n_dev, "vetoed_fraction": round(vetoed / n_dev, 4),
+            "accepted_fraction": round(st["accepted"] / n_dev, 4), "windows": st["windows"],
+            "dropin_per_move_loop_moves_per_s": round(n_loop / el_p, 1), "dropin_moves": n_loop}
 
 
 def sq_transform_section(reps=50):

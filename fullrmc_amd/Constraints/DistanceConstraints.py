@@ -202,6 +202,13 @@ class _MolecularDistanceConstraint:
             return random.random() < self.rejectProbability
         return bool(np.any(self.activeAtomsDataAfterMove["number"] > self.activeAtomsDataBeforeMove["number"]))
 
+    def after_move_state(self):
+        """What accept_move takes from the last compute_after_move."""
+        return self._newData, self.afterMoveStandardError
+
+    def set_after_move_state(self, state):
+        self._newData, self.afterMoveStandardError = state
+
     def rejection_draws(self, standardError):
         """The Python random draws should_step_get_rejected makes for this
         standard error: one when a flexible constraint's error grows."""

@@ -548,7 +548,8 @@ class Engine:
 
         def veto(atoms, mb):
             """The rigid constraints in turn; returns (vetoed, constraints
-            evaluated, Python random draws made)."""
+            evaluated, Python random draws made, the after-move states a
+            passed proposal's acceptance takes)."""
             draws = 0
             for ci, c in enumerate(rigid):
                 c.compute_before_move(realIndexes=atoms, relativeIndexes=atoms)
@@ -557,8 +558,8 @@ class Engine:
                 if c.should_step_get_rejected(c.afterMoveStandardError):
                     for c2 in rigid:
                         c2.reject_move(realIndexes=atoms, relativeIndexes=atoms)
-                    return True, ci + 1, draws
-            return False, len(rigid), draws
+                    return True, ci + 1, draws, None
+            return False, len(rigid), draws, [c.after_move_state() for c in rigid]
 
         gen0 = acc0 = 0
         while step < nsteps:
@@ -566,6 +567,12 @@ class Engine:
             # on the device: about one expected acceptance per window
             p_acc = (acc0 + 1.0) / (gen0 + 2.0)
             kw = min(k, nsteps - step, max(1, int(np.ceil(1.0 / p_acc))))
+            if rigid:
+                # a veto in Python (four drop-in calls per constraint) costs
+                # more than a device window, so a window ends at its first
+                # proposal that passes them: no veto is evaluated for a
+                # proposal past the window's acceptance
+                kw = min(k, nsteps - step)
             gids = np.empty(kw, np.int32)
             boxes, reals, u, moved, vetoed = [], [], [], [], []
             # the streams are saved once per window; an accepted proposal's
@@ -575,16 +582,22 @@ class Engine:
             for p in range(kw):
                 gi, atoms, mr = propose()
                 mb = box_coordinates(self.reciprocalBasisVectors, mr)
-                v = veto(atoms, mb) if rigid else (False, 0, 0)
+                v = veto(atoms, mb) if rigid else (False, 0, 0, None)
                 vetoed.append(v)
                 u.append(None if v[0] else random.random())
                 gids[p] = gi
                 boxes.append(mb)
                 reals.append(mr)
                 moved.append(np.asarray(atoms, dtype=np.int32))
+                if rigid and not v[0]:
+                    break
+            kw = len(boxes)
+            gids = gids[:kw]
             nb = np.ascontiguousarray(np.concatenate(boxes), dtype=np.float32)
             tse = np.empty(kw, np.float32)
-            if lists:
+            if all(v[0] for v in vetoed):
+                pass  # every proposal vetoed: nothing to evaluate
+            elif lists:
                 counts = np.array([len(a) for a in moved], dtype=np.int32)
                 flat = np.ascontiguousarray(np.concatenate(moved), dtype=np.int32)
                 _lib.check(L.frmc_engine_eval_atoms(h, kw, counts.ctypes.data_as(_lib.c_i32p),
@@ -635,12 +648,9 @@ class Engine:
                     if (p < acc and not vetoed[p][0]) or (p == acc and drew):
                         random.random()
                 atoms = moved[acc]
-                for c in rigid:
-                    # the accepted move's data, then the rigid constraints take it
-                    t = c.tried
-                    c.compute_before_move(realIndexes=atoms, relativeIndexes=atoms)
-                    c.compute_after_move(realIndexes=atoms, relativeIndexes=atoms, movedBoxCoordinates=boxes[acc])
-                    c.tried = t
+                for c, state in zip(rigid, vetoed[acc][3] or ()):
+                    # the accepted move's after-move data, kept from its veto
+                    c.set_after_move_state(state)
                     c.accept_move(realIndexes=atoms, relativeIndexes=atoms)
                 _lib.check(L.frmc_engine_commit(h, acc))
                 self.accepted += 1
