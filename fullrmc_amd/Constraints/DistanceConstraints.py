@@ -14,6 +14,9 @@ import warnings
 
 import numpy as np
 
+from .. import _device as D
+from .._lib import check, lib
+from ..Core.atomic_distances import _flags as atomic_distance_flags
 from ..Core.atomic_distances import full_atomic_distances_coords, multiple_atomic_distances_coords
 from ..Globals import FLOAT_TYPE, INT_TYPE
 
@@ -56,6 +59,7 @@ class _MolecularDistanceConstraint:
         self.activeAtomsDataAfterMove = None
         self.tried = 0
         self.accepted = 0
+        self._res = None  # resident device copies while the device engine's windows run
 
     # ------------------------------------------------------------- setup
     def _attach(self, engine):
@@ -134,6 +138,8 @@ class _MolecularDistanceConstraint:
                                                         self.lowerLimitArray, self.upperLimitArray, **self._kw()))
 
     def _group(self, rel):
+        if self._res is not None:
+            return self._group_resident(rel)
         e = self.engine
         kw = self._kw()
         nM, dM = self._pick(*multiple_atomic_distances_coords(
@@ -143,6 +149,65 @@ class _MolecularDistanceConstraint:
             reduceDistanceToUpper=kw["reduceDistanceToUpper"], reduceDistanceToLower=kw["reduceDistanceToLower"],
             reduceDistance=kw["reduceDistance"], allAtoms=True))
         nF, dF = self._full(e.boxCoordinates[rel], e.moleculesIndex[rel], self.typesIndex[rel])
+        return {"number": nM - nF, "distanceSum": dM - dF}
+
+    # ------------------------------------------- resident device copies
+    def bind_resident(self):
+        """Device copies of the engine's box coordinates, molecule and type
+        indexes, basis and limits, held while the device engine's windows
+        run: an evaluation then uploads only the group's rows.  It runs the
+        drop-in calls' kernels on the same inputs, so its counts and float32
+        sums are theirs."""
+        e = self.engine
+        self._res = {"box": D.upload(np.asarray(e.boxCoordinates, np.float32)),
+                     "mol": D.upload(np.asarray(e.moleculesIndex, np.int32)),
+                     "types": D.upload(np.asarray(self.typesIndex, np.int32)),
+                     "basis": D.upload(np.asarray(e.basisVectors, np.float32)),
+                     "lo": D.upload(self.lowerLimitArray), "up": D.upload(self.upperLimitArray),
+                     "flags": atomic_distance_flags(**self._kw()), "pbc": int(bool(e.isPBC))}
+
+    def unbind_resident(self):
+        self._res = None
+
+    def resident_commit(self, rel, boxRows):
+        """An accepted move's new box coordinates into the device copy."""
+        if self._res is not None:
+            idx = D.upload(np.asarray(rel, np.int32)).long()
+            self._res["box"][idx] = D.upload(np.asarray(boxRows, np.float32))
+
+    def _group_resident(self, rel):
+        """_group on the resident copies: the group's current rows (moved,
+        inside compute_after_move) patched in for the group-vs-all call and
+        restored after it; the group-alone call on those rows."""
+        t = D.torch()
+        r, e = self._res, self.engine
+        rel = np.asarray(rel, np.int32)
+        g, T, n = len(rel), self.numberOfTypes, r["box"].shape[0]
+        rows = D.upload(np.asarray(e.boxCoordinates[rel], np.float32))
+        idx = D.upload(rel)
+        idx64 = idx.long()
+        saved = r["box"][idx64]
+        r["box"][idx64] = rows
+        # outputs from zeros, as the drop-in calls take them: group vs all
+        # (intra, inter), then the group alone (intra, inter)
+        cnt = D.zeros((4, T, T), np.int32)
+        dst = D.zeros((4, T, T), np.float32)
+        st = D.stream()
+        L = lib()
+        check(L.frmc_multiple_atomic_distances_coords(
+            idx.data_ptr(), g, r["box"].data_ptr(), n, r["basis"].data_ptr(), r["pbc"], r["mol"].data_ptr(),
+            r["types"].data_ptr(), T, r["lo"].data_ptr(), r["up"].data_ptr(), r["flags"], 1, cnt[0].data_ptr(),
+            dst[0].data_ptr(), cnt[1].data_ptr(), dst[1].data_ptr(), st))
+        mol_g, types_g = r["mol"][idx64], r["types"][idx64]
+        ar = t.arange(g, dtype=t.int32, device="cuda")
+        check(L.frmc_multiple_atomic_distances_coords(
+            ar.data_ptr(), g, rows.data_ptr(), g, r["basis"].data_ptr(), r["pbc"], mol_g.data_ptr(),
+            types_g.data_ptr(), T, r["lo"].data_ptr(), r["up"].data_ptr(), r["flags"], 0, cnt[2].data_ptr(),
+            dst[2].data_ptr(), cnt[3].data_ptr(), dst[3].data_ptr(), st))
+        r["box"][idx64] = saved
+        c, d = D.download(cnt).reshape(4, T, T, 1), D.download(dst).reshape(4, T, T, 1)
+        nM, dM = self._pick(c[0], d[0], c[1], d[1])
+        nF, dF = self._pick(c[2], d[2], c[3], d[3])
         return {"number": nM - nF, "distanceSum": dM - dF}
 
     def _get_constraint_value(self, data=None):

@@ -530,8 +530,6 @@ class Engine:
         L = _lib.lib()
         self._pull_device()
         real = self.realCoordinates
-        step = 0
-        windows = evaluated = accepted = 0
         rigid = self.rigidConstraints
         # swap moves act on the group plus drawn partners (Engine.py:3192-3201):
         # their windows name each proposal's atoms
@@ -561,6 +559,25 @@ class Engine:
                     return True, ci + 1, draws, None
             return False, len(rigid), draws, [c.after_move_state() for c in rigid]
 
+        resident = [c for c in rigid if hasattr(c, "bind_resident")]
+        for c in resident:
+            c.bind_resident()
+        try:
+            stats = self._python_windows(h, nsteps, k, log, rigid, resident, lists, propose, veto)
+        finally:
+            for c in resident:
+                c.unbind_resident()
+        # the native mirror of real coordinates is refreshed from this loop
+        r = np.ascontiguousarray(self.realCoordinates, dtype=np.float32)
+        _lib.check(L.frmc_engine_set_real(h, r.ctypes.data_as(_lib.c_f32p)))
+        return stats
+
+    def _python_windows(self, h, nsteps, k, log, rigid, resident, lists, propose, veto):
+        """The window loop of _run_python."""
+        L = _lib.lib()
+        real = self.realCoordinates
+        step = 0
+        windows = evaluated = accepted = 0
         gen0 = acc0 = 0
         while step < nsteps:
             # proposals drawn by the Python generators cost more than a window
@@ -658,9 +675,8 @@ class Engine:
                 self.totalStandardError = tse[acc]
                 real[atoms] = reals[acc]
                 self.boxCoordinates[atoms] = boxes[acc]
-        # the native mirror of real coordinates is refreshed from this loop
-        r = np.ascontiguousarray(real, dtype=np.float32)
-        _lib.check(L.frmc_engine_set_real(h, r.ctypes.data_as(_lib.c_f32p)))
+                for c in resident:
+                    c.resident_commit(atoms, boxes[acc])
         return dict(generated=step, accepted=accepted, windows=windows, evaluated=evaluated)
 
 
