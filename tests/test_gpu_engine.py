@@ -466,6 +466,37 @@ def test_swap_moves_on_the_device_engine(cells):
     assert len(swaps) > 50 and sum(swaps) > 3
 
 
+def test_swap_moves_with_a_distance_constraint_on_the_device_engine():
+    """Swaps and translations of the B2 alloy with an inter-molecular
+    distance constraint (every atom its own molecule, 2.45 A, strict): the veto
+    evaluates the group and its drawn partner, a vetoed swap makes no
+    acceptance draw, and the trajectory, streams and final state equal the
+    oracle's."""
+    s = systems.b2_alloy(cells=(5, 5, 5), a=3.01, sigma=0.08, seed=13)
+    exp = pdc_target(s, 0.08, 14, 0.01, 0.02, 300)
+    n = len(s["real"])
+    el = s["elements"]
+    al = [[j] for j in range(n) if el[j] == "Al"]
+    gp = [SwapPositionsGenerator(swapList=al) if el[i] == "Fe" else TranslationGenerator(0.3) for i in range(n)]
+    go = [oeng.Swap(al) if el[i] == "Fe" else oeng.Translation(0.3) for i in range(n)]
+    oe, olog, pys, nps = oracle_run(s, [[i] for i in range(n)], go, _pdc_cons_of({"exp": exp}), 300, 24,
+                                    tolerance=5.0, rigid_fn=lambda o: [oeng.IMD(o, 2.45, flexible=False)])
+    random.seed(24)
+    np.random.seed(24)
+    E = device_engine(s, [PairDistributionConstraint(exp), InterMolecularDistanceConstraint(2.45, flexible=False)],
+                      groups=[[i] for i in range(n)], gen=gp)
+    E.set_tolerance(5.0)
+    dlog = []
+    E.run_device(300, k=16, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    c, oc = E.rigidConstraints[0], oe.rigid[0]
+    assert c.data["number"].tobytes() == oc.number.tobytes()
+    assert c.data["distanceSum"].tobytes() == oc.dsum.tobytes()
+    assert E.tried < E.generated
+    swaps = [a for g, a, _ in dlog if el[g] == "Fe"]
+    assert len(swaps) > 20 and any(swaps)
+
+
 def test_remove_moves_amputation_in_dropin_loop():
     """AtomsRemoveGenerator groups beside translations on a B2 alloy with a
     G(r), an S(Q) and a distance constraint: each removal is evaluated as if
