@@ -176,39 +176,57 @@ class _MolecularDistanceConstraint:
             self._res["box"][idx] = D.upload(np.asarray(boxRows, np.float32))
 
     def _group_resident(self, rel):
-        """_group on the resident copies: the group's current rows (moved,
-        inside compute_after_move) patched in for the group-vs-all call and
-        restored after it; the group-alone call on those rows."""
+        """_group on the resident copies, at the group's current host rows
+        (the moved ones inside compute_after_move)."""
+        e = self.engine
+        rel = np.asarray(rel, np.int32)
+        return self._groups_resident(rel, None, np.asarray(e.boxCoordinates[rel], np.float32))[0]
+
+    def _groups_resident(self, rel, moved, rows=None):
+        """_group before the move (the resident rows, or rows) and, with
+        moved, after it: for each state the group-vs-all call with the
+        group's rows patched in (restored after it) and the group-alone call
+        on those rows, outputs from zeros as the drop-in calls take them,
+        downloaded once."""
         t = D.torch()
-        r, e = self._res, self.engine
+        r = self._res
         rel = np.asarray(rel, np.int32)
         g, T, n = len(rel), self.numberOfTypes, r["box"].shape[0]
-        rows = D.upload(np.asarray(e.boxCoordinates[rel], np.float32))
         idx = D.upload(rel)
         idx64 = idx.long()
-        saved = r["box"][idx64]
-        r["box"][idx64] = rows
-        # outputs from zeros, as the drop-in calls take them: group vs all
-        # (intra, inter), then the group alone (intra, inter)
-        cnt = D.zeros((4, T, T), np.int32)
-        dst = D.zeros((4, T, T), np.float32)
-        st = D.stream()
-        L = lib()
-        check(L.frmc_multiple_atomic_distances_coords(
-            idx.data_ptr(), g, r["box"].data_ptr(), n, r["basis"].data_ptr(), r["pbc"], r["mol"].data_ptr(),
-            r["types"].data_ptr(), T, r["lo"].data_ptr(), r["up"].data_ptr(), r["flags"], 1, cnt[0].data_ptr(),
-            dst[0].data_ptr(), cnt[1].data_ptr(), dst[1].data_ptr(), st))
+        resident = r["box"][idx64]
+        states = [resident if rows is None else D.upload(rows)]
+        if moved is not None:
+            states.append(D.upload(np.asarray(moved, np.float32)))
+        cnt = D.zeros((4 * len(states), T, T), np.int32)
+        dst = D.zeros((4 * len(states), T, T), np.float32)
         mol_g, types_g = r["mol"][idx64], r["types"][idx64]
         ar = t.arange(g, dtype=t.int32, device="cuda")
-        check(L.frmc_multiple_atomic_distances_coords(
-            ar.data_ptr(), g, rows.data_ptr(), g, r["basis"].data_ptr(), r["pbc"], mol_g.data_ptr(),
-            types_g.data_ptr(), T, r["lo"].data_ptr(), r["up"].data_ptr(), r["flags"], 0, cnt[2].data_ptr(),
-            dst[2].data_ptr(), cnt[3].data_ptr(), dst[3].data_ptr(), st))
-        r["box"][idx64] = saved
-        c, d = D.download(cnt).reshape(4, T, T, 1), D.download(dst).reshape(4, T, T, 1)
-        nM, dM = self._pick(c[0], d[0], c[1], d[1])
-        nF, dF = self._pick(c[2], d[2], c[3], d[3])
-        return {"number": nM - nF, "distanceSum": dM - dF}
+        st = D.stream()
+        L = lib()
+        for si, rows_d in enumerate(states):
+            o = 4 * si
+            if rows_d is not resident:
+                r["box"][idx64] = rows_d
+            check(L.frmc_multiple_atomic_distances_coords(
+                idx.data_ptr(), g, r["box"].data_ptr(), n, r["basis"].data_ptr(), r["pbc"], r["mol"].data_ptr(),
+                r["types"].data_ptr(), T, r["lo"].data_ptr(), r["up"].data_ptr(), r["flags"], 1,
+                cnt[o].data_ptr(), dst[o].data_ptr(), cnt[o + 1].data_ptr(), dst[o + 1].data_ptr(), st))
+            if rows_d is not resident:
+                r["box"][idx64] = resident
+            check(L.frmc_multiple_atomic_distances_coords(
+                ar.data_ptr(), g, rows_d.data_ptr(), g, r["basis"].data_ptr(), r["pbc"], mol_g.data_ptr(),
+                types_g.data_ptr(), T, r["lo"].data_ptr(), r["up"].data_ptr(), r["flags"], 0, cnt[o + 2].data_ptr(),
+                dst[o + 2].data_ptr(), cnt[o + 3].data_ptr(), dst[o + 3].data_ptr(), st))
+        c = D.download(cnt).reshape(-1, T, T, 1)
+        d = D.download(dst).reshape(-1, T, T, 1)
+        out = []
+        for si in range(len(states)):
+            o = 4 * si
+            nM, dM = self._pick(c[o], d[o], c[o + 1], d[o + 1])
+            nF, dF = self._pick(c[o + 2], d[o + 2], c[o + 3], d[o + 3])
+            out.append({"number": nM - nF, "distanceSum": dM - dF})
+        return out
 
     def _get_constraint_value(self, data=None):
         """Mean counted distance per type pair (:415-428)."""
@@ -252,11 +270,25 @@ class _MolecularDistanceConstraint:
         e.boxCoordinates[relativeIndexes] = movedBoxCoordinates
         self.activeAtomsDataAfterMove = self._group(relativeIndexes)
         e.boxCoordinates[relativeIndexes] = saved
+        self._after_move_totals()
+
+    def _after_move_totals(self):
         b, a = self.activeAtomsDataBeforeMove, self.activeAtomsDataAfterMove
         self._newData = {"number": self.data["number"] - b["number"] + a["number"],
                          "distanceSum": self.data["distanceSum"] - b["distanceSum"] + a["distanceSum"]}
         self.afterMoveStandardError = self._compute_standard_error(self._get_constraint_value(self._newData))
         self.tried += 1
+
+    def compute_move(self, realIndexes, relativeIndexes, movedBoxCoordinates):
+        """compute_before_move then compute_after_move; on the resident
+        copies both in one pass with one download."""
+        if self._res is None:
+            self.compute_before_move(realIndexes, relativeIndexes)
+            self.compute_after_move(realIndexes, relativeIndexes, movedBoxCoordinates)
+            return
+        self.activeAtomsDataBeforeMove, self.activeAtomsDataAfterMove = self._groups_resident(
+            relativeIndexes, movedBoxCoordinates)
+        self._after_move_totals()
 
     def should_step_get_rejected(self, standardError):
         """:361-383; flexible: Core/Constraint.py:2083-2099 (one Python

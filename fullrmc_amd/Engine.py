@@ -550,8 +550,11 @@ class Engine:
             passed proposal's acceptance takes)."""
             draws = 0
             for ci, c in enumerate(rigid):
-                c.compute_before_move(realIndexes=atoms, relativeIndexes=atoms)
-                c.compute_after_move(realIndexes=atoms, relativeIndexes=atoms, movedBoxCoordinates=mb)
+                if hasattr(c, "compute_move"):
+                    c.compute_move(realIndexes=atoms, relativeIndexes=atoms, movedBoxCoordinates=mb)
+                else:
+                    c.compute_before_move(realIndexes=atoms, relativeIndexes=atoms)
+                    c.compute_after_move(realIndexes=atoms, relativeIndexes=atoms, movedBoxCoordinates=mb)
                 draws += c.rejection_draws(c.afterMoveStandardError)
                 if c.should_step_get_rejected(c.afterMoveStandardError):
                     for c2 in rigid:
