@@ -15,6 +15,7 @@ Two ways to run the same trajectory:
 Both use fullrmc's two global streams, Python's ``random`` and NumPy's
 legacy ``numpy.random``, exactly where the reference draws from them.
 """
+import collections
 import ctypes
 import random
 import time
@@ -545,9 +546,7 @@ class Engine:
             return gi, atoms, gen.move(before)
 
         def veto(atoms, mb):
-            """The rigid constraints in turn; returns (vetoed, constraints
-            evaluated, Python random draws made, the after-move states a
-            passed proposal's acceptance takes)."""
+            """The rigid constraints in turn (a _Veto)."""
             draws = 0
             for ci, c in enumerate(rigid):
                 if hasattr(c, "compute_move"):
@@ -559,8 +558,8 @@ class Engine:
                 if c.should_step_get_rejected(c.afterMoveStandardError):
                     for c2 in rigid:
                         c2.reject_move(realIndexes=atoms, relativeIndexes=atoms)
-                    return True, ci + 1, draws, None
-            return False, len(rigid), draws, [c.after_move_state() for c in rigid]
+                    return _Veto(True, ci + 1, draws, None)
+            return _Veto(False, len(rigid), draws, [c.after_move_state() for c in rigid])
 
         resident = [c for c in rigid if hasattr(c, "bind_resident")]
         for c in resident:
@@ -594,7 +593,7 @@ class Engine:
                 # proposal past the window's acceptance
                 kw = min(k, nsteps - step)
             gids = np.empty(kw, np.int32)
-            boxes, reals, u, moved, vetoed = [], [], [], [], []
+            boxes, reals, u, moved, outcome = [], [], [], [], []
             # the streams are saved once per window; an accepted proposal's
             # position is reached again by replaying the window's draws
             st0 = (random.getstate(), np.random.get_state())
@@ -602,20 +601,20 @@ class Engine:
             for p in range(kw):
                 gi, atoms, mr = propose()
                 mb = box_coordinates(self.reciprocalBasisVectors, mr)
-                v = veto(atoms, mb) if rigid else (False, 0, 0, None)
-                vetoed.append(v)
-                u.append(None if v[0] else random.random())
+                v = veto(atoms, mb) if rigid else _NO_VETO
+                outcome.append(v)
+                u.append(None if v.vetoed else random.random())
                 gids[p] = gi
                 boxes.append(mb)
                 reals.append(mr)
                 moved.append(np.asarray(atoms, dtype=np.int32))
-                if rigid and not v[0]:
+                if rigid and not v.vetoed:
                     break
             kw = len(boxes)
             gids = gids[:kw]
             nb = np.ascontiguousarray(np.concatenate(boxes), dtype=np.float32)
             tse = np.empty(kw, np.float32)
-            if all(v[0] for v in vetoed):
+            if all(v.vetoed for v in outcome):
                 pass  # every proposal vetoed: nothing to evaluate
             elif lists:
                 counts = np.array([len(a) for a in moved], dtype=np.int32)
@@ -635,7 +634,7 @@ class Engine:
                 gen0 += 1
                 step += 1
                 last = p
-                if vetoed[p][0]:
+                if outcome[p].vetoed:
                     # never tried (Engine.py:3290-3296)
                     if log is not None:
                         log.append((int(gids[p]), False, float(cur)))
@@ -657,18 +656,18 @@ class Engine:
             # each rigid constraint counts the proposals it evaluated up to
             # the last one walked, as the per-move loop would
             for ci, c in enumerate(rigid):
-                c.tried = tried0[ci] + sum(1 for p in range(last + 1) if vetoed[p][1] > ci)
+                c.tried = tried0[ci] + sum(1 for p in range(last + 1) if outcome[p].evaluated > ci)
             if acc >= 0:
                 random.setstate(st0[0])
                 np.random.set_state(st0[1])
                 for p in range(acc + 1):
                     propose()
-                    for _ in range(vetoed[p][2]):
+                    for _ in range(outcome[p].draws):
                         random.random()
-                    if (p < acc and not vetoed[p][0]) or (p == acc and drew):
+                    if (p < acc and not outcome[p].vetoed) or (p == acc and drew):
                         random.random()
                 atoms = moved[acc]
-                for c, state in zip(rigid, vetoed[acc][3] or ()):
+                for c, state in zip(rigid, outcome[acc].states or ()):
                     # the accepted move's after-move data, kept from its veto
                     c.set_after_move_state(state)
                     c.accept_move(realIndexes=atoms, relativeIndexes=atoms)
@@ -681,6 +680,13 @@ class Engine:
                 for c in resident:
                     c.resident_commit(atoms, boxes[acc])
         return dict(generated=step, accepted=accepted, windows=windows, evaluated=evaluated)
+
+
+# A proposal's outcome under the rigid constraints: vetoed, how many of them
+# evaluated it, the Python random draws they made, and (when it passed) their
+# after-move states for its acceptance.
+_Veto = collections.namedtuple("_Veto", "vetoed evaluated draws states")
+_NO_VETO = _Veto(False, 0, 0, None)
 
 
 # ---------------------------------------------------------------- replicas
