@@ -349,8 +349,6 @@ class Engine:
         from . import _device
         _device.require_gpu()
         distinct = self._distinct_generators()
-        if any(hasattr(g, "pick_from_list") for g in distinct):
-            raise NotImplementedError("remove moves change the configuration's atoms; they run in Engine.run")
         swaps = any(hasattr(g, "get_ready_for_move") for g in distinct)
         for grp, gen in zip(self.groups, self.generators) if swaps else ():
             if hasattr(gen, "get_ready_for_move") and (len(grp) != gen.swapLength or not len(gen.swapList)):
@@ -358,8 +356,6 @@ class Engine:
                 # with an empty swap list the group would be swapped with itself
                 raise NotImplementedError("a swap group whose size is not the swap length, or with an empty "
                                           "swap list, runs in Engine.run")
-        if self.collectedAtoms:
-            raise NotImplementedError("a configuration with removed atoms runs in Engine.run")
         if self._dev is not None and self._dev_kmax >= kmax and self._dev_synced:
             return self._dev
         self._pull_device()
@@ -391,10 +387,17 @@ class Engine:
         for q in range(9):
             d.basis[q] = float(self.basisVectors.reshape(-1)[q])
             d.recip[q] = float(self.reciprocalBasisVectors.reshape(-1)[q])
-        offs = np.zeros(len(self.groups) + 1, dtype=np.int32)
-        offs[1:] = np.cumsum([len(g) for g in self.groups])
-        d.ngroups = len(self.groups)
-        d.group_offsets, d.group_atoms = i32p(offs), i32p(np.concatenate(self.groups))
+        groups = self.groups
+        if self.collectedAtoms:
+            # the remaining atoms of each group, by position in the arrays; the
+            # windows of such an engine name their atoms, so emptied groups
+            # leave the table
+            groups = [self.relative_indexes(g[[not self.is_collected(int(i)) for i in g]]) for g in groups]
+            groups = [g for g in groups if len(g)] or [np.zeros(1, np.int32)]
+        offs = np.zeros(len(groups) + 1, dtype=np.int32)
+        offs[1:] = np.cumsum([len(g) for g in groups])
+        d.ngroups = len(groups)
+        d.group_offsets, d.group_atoms = i32p(offs), i32p(np.concatenate(groups))
         d.nconstraints = len(self.constraints)
         for ci, c in enumerate(self.constraints):
             a = c._device_arrays()
@@ -472,7 +475,7 @@ class Engine:
         gen0 = distinct[0]
         # rigid constraints veto moves with draws of their own, so their runs
         # take the Python-drawn windows
-        nat = (native and not self.rigidConstraints and len(distinct) == 1
+        nat = (native and not self.rigidConstraints and not self.collectedAtoms and len(distinct) == 1
                and getattr(gen0, "native_kind", None) == "translation")
         t0 = time.perf_counter()
         if nat:
@@ -527,37 +530,53 @@ class Engine:
         the window is drawn, in order and on the live streams, since it
         depends only on the configuration the window starts from; a vetoed
         proposal reserves no acceptance draw, and the draws a flexible veto
-        makes are replayed with the proposal's own."""
-        L = _lib.lib()
+        makes are replayed with the proposal's own.  A removal
+        (Engine.py:3231-3275) ends its window: it is evaluated on the host
+        with the constraints' amputation calls once the moves before it are
+        walked, and an accepted one rebuilds the device engine on the
+        remaining atoms."""
         self._pull_device()
-        real = self.realCoordinates
         rigid = self.rigidConstraints
-        # swap moves act on the group plus drawn partners (Engine.py:3192-3201):
-        # their windows name each proposal's atoms
-        lists = any(hasattr(g, "get_ready_for_move") for g in self._distinct_generators())
+        distinct = self._distinct_generators()
+        removals = any(hasattr(g, "pick_from_list") for g in distinct)
+        # swap moves act on the group plus drawn partners (Engine.py:3192-3201),
+        # and with removals the device's group table no longer matches the
+        # groups: windows then name each proposal's atoms
+        lists = removals or bool(self.collectedAtoms) or any(hasattr(g, "get_ready_for_move") for g in distinct)
 
         def propose():
+            """Engine.run's selection and move (Engine.py:3160-3210)."""
             gi = self.selector.select_index(len(self.groups))
             gen = self.generators[gi]
-            atoms = self.groups[gi]
-            if hasattr(gen, "get_ready_for_move"):
-                atoms = gen.get_ready_for_move(atoms, self)
-            before = np.array(real[atoms], dtype=FLOAT_TYPE)
-            return gi, atoms, gen.move(before)
+            if hasattr(gen, "pick_from_list"):
+                real = gen.pick_from_list(self)
+                if not len(real):
+                    return _Proposal(gi, "none", None, None, None)
+                return _Proposal(gi, "remove", real, self.relative_indexes(real), None)
+            real = self.groups[gi]
+            if self.collectedAtoms:
+                real = real[[not self.is_collected(int(i)) for i in real]]
+            if hasattr(gen, "get_ready_for_move") and len(real):
+                real = gen.get_ready_for_move(real, self) if len(real) == gen.swapLength else real[:0]
+            if not len(real):
+                return _Proposal(gi, "none", None, None, None)
+            rel = self.relative_indexes(real)
+            before = np.array(self.realCoordinates[rel], dtype=FLOAT_TYPE)
+            return _Proposal(gi, "move", real, rel, gen.move(before))
 
-        def veto(atoms, mb):
+        def veto(real, rel, mb):
             """The rigid constraints in turn (a _Veto)."""
             draws = 0
             for ci, c in enumerate(rigid):
                 if hasattr(c, "compute_move"):
-                    c.compute_move(realIndexes=atoms, relativeIndexes=atoms, movedBoxCoordinates=mb)
+                    c.compute_move(realIndexes=real, relativeIndexes=rel, movedBoxCoordinates=mb)
                 else:
-                    c.compute_before_move(realIndexes=atoms, relativeIndexes=atoms)
-                    c.compute_after_move(realIndexes=atoms, relativeIndexes=atoms, movedBoxCoordinates=mb)
+                    c.compute_before_move(realIndexes=real, relativeIndexes=rel)
+                    c.compute_after_move(realIndexes=real, relativeIndexes=rel, movedBoxCoordinates=mb)
                 draws += c.rejection_draws(c.afterMoveStandardError)
                 if c.should_step_get_rejected(c.afterMoveStandardError):
                     for c2 in rigid:
-                        c2.reject_move(realIndexes=atoms, relativeIndexes=atoms)
+                        c2.reject_move(realIndexes=real, relativeIndexes=rel)
                     return _Veto(True, ci + 1, draws, None)
             return _Veto(False, len(rigid), draws, [c.after_move_state() for c in rigid])
 
@@ -565,90 +584,120 @@ class Engine:
         for c in resident:
             c.bind_resident()
         try:
-            stats = self._python_windows(h, nsteps, k, log, rigid, resident, lists, propose, veto)
+            stats = self._python_windows(h, nsteps, k, log, rigid, resident, removals, lists, propose, veto)
         finally:
             for c in resident:
                 c.unbind_resident()
         # the native mirror of real coordinates is refreshed from this loop
-        r = np.ascontiguousarray(self.realCoordinates, dtype=np.float32)
-        _lib.check(L.frmc_engine_set_real(h, r.ctypes.data_as(_lib.c_f32p)))
+        self._push_real()
         return stats
 
-    def _python_windows(self, h, nsteps, k, log, rigid, resident, lists, propose, veto):
+    def _push_real(self):
+        if self._dev is not None:
+            r = np.ascontiguousarray(self.realCoordinates, dtype=np.float32)
+            _lib.check(_lib.lib().frmc_engine_set_real(self._dev, r.ctypes.data_as(_lib.c_f32p)))
+
+    def _python_windows(self, h, nsteps, k, log, rigid, resident, removals, lists, propose, veto):
         """The window loop of _run_python."""
         L = _lib.lib()
-        real = self.realCoordinates
         step = 0
         windows = evaluated = accepted = 0
         gen0 = acc0 = 0
         while step < nsteps:
+            if h is None:
+                # after an accepted removal: the engine on the remaining atoms
+                h = self._device_engine(k)
             # proposals drawn by the Python generators cost more than a window
             # on the device: about one expected acceptance per window
             p_acc = (acc0 + 1.0) / (gen0 + 2.0)
             kw = min(k, nsteps - step, max(1, int(np.ceil(1.0 / p_acc))))
-            if rigid:
+            if rigid or removals:
                 # a veto in Python (four drop-in calls per constraint) costs
                 # more than a device window, so a window ends at its first
                 # proposal that passes them: no veto is evaluated for a
-                # proposal past the window's acceptance
+                # proposal past the window's acceptance; a removal ends it too
                 kw = min(k, nsteps - step)
-            gids = np.empty(kw, np.int32)
-            boxes, reals, u, moved, outcome = [], [], [], [], []
+            props, boxes, u, outcome = [], [], [], []
             # the streams are saved once per window; an accepted proposal's
             # position is reached again by replaying the window's draws
             st0 = (random.getstate(), np.random.get_state())
             tried0 = [c.tried for c in rigid]
             for p in range(kw):
-                gi, atoms, mr = propose()
-                mb = box_coordinates(self.reciprocalBasisVectors, mr)
-                v = veto(atoms, mb) if rigid else _NO_VETO
+                pr = propose()
+                v = _NO_VETO
+                if pr.kind == "move":
+                    mb = box_coordinates(self.reciprocalBasisVectors, pr.moved)
+                    if rigid:
+                        v = veto(pr.real, pr.rel, mb)
+                else:
+                    # no move for the device: atom 0 where it is
+                    mb = np.array(self.boxCoordinates[:1], dtype=np.float32)
+                props.append(pr)
                 outcome.append(v)
-                u.append(None if v.vetoed else random.random())
-                gids[p] = gi
+                u.append(None if v.vetoed or pr.kind == "none" else random.random())
                 boxes.append(mb)
-                reals.append(mr)
-                moved.append(np.asarray(atoms, dtype=np.int32))
-                if rigid and not v.vetoed:
+                if pr.kind == "remove" or (rigid and pr.kind == "move" and not v.vetoed):
                     break
-            kw = len(boxes)
-            gids = gids[:kw]
-            nb = np.ascontiguousarray(np.concatenate(boxes), dtype=np.float32)
+            kw = len(props)
             tse = np.empty(kw, np.float32)
-            if all(v.vetoed for v in outcome):
-                pass  # every proposal vetoed: nothing to evaluate
-            elif lists:
-                counts = np.array([len(a) for a in moved], dtype=np.int32)
-                flat = np.ascontiguousarray(np.concatenate(moved), dtype=np.int32)
-                _lib.check(L.frmc_engine_eval_atoms(h, kw, counts.ctypes.data_as(_lib.c_i32p),
-                                                    flat.ctypes.data_as(_lib.c_i32p), nb.ctypes.data_as(_lib.c_f32p),
-                                                    tse.ctypes.data_as(_lib.c_f32p), None))
-            else:
-                _lib.check(L.frmc_engine_eval(h, kw, gids.ctypes.data_as(_lib.c_i32p),
-                                              nb.ctypes.data_as(_lib.c_f32p), tse.ctypes.data_as(_lib.c_f32p), None))
+            moves = [p for p in range(kw) if props[p].kind == "move" and not outcome[p].vetoed]
+            if moves:
+                nb = np.ascontiguousarray(np.concatenate(boxes), dtype=np.float32)
+                if lists:
+                    atoms = [pr.rel if pr.kind == "move" else np.zeros(1, np.int32) for pr in props]
+                    counts = np.array([len(a) for a in atoms], dtype=np.int32)
+                    flat = np.ascontiguousarray(np.concatenate(atoms), dtype=np.int32)
+                    _lib.check(L.frmc_engine_eval_atoms(h, kw, counts.ctypes.data_as(_lib.c_i32p),
+                                                        flat.ctypes.data_as(_lib.c_i32p),
+                                                        nb.ctypes.data_as(_lib.c_f32p),
+                                                        tse.ctypes.data_as(_lib.c_f32p), None))
+                else:
+                    gids = np.array([pr.gi for pr in props], dtype=np.int32)
+                    _lib.check(L.frmc_engine_eval(h, kw, gids.ctypes.data_as(_lib.c_i32p),
+                                                  nb.ctypes.data_as(_lib.c_f32p), tse.ctypes.data_as(_lib.c_f32p),
+                                                  None))
             windows += 1
-            evaluated += kw
+            evaluated += len(moves)
             cur = np.float32(self.totalStandardError)
             acc = last = -1
+            amputated = None
             for p in range(kw):
+                pr = props[p]
                 self.generated += 1
                 gen0 += 1
                 step += 1
                 last = p
-                if outcome[p].vetoed:
-                    # never tried (Engine.py:3290-3296)
+                if pr.kind == "none" or outcome[p].vetoed:
+                    # nothing to try, or never tried (Engine.py:3290-3296)
                     if log is not None:
-                        log.append((int(gids[p]), False, float(cur)))
+                        log.append((int(pr.gi), False, float(cur)))
                     continue
                 self.tried += 1
+                if pr.kind == "remove":
+                    # the constraints as if the atom were gone, from the state
+                    # the window started from (Engine.py:3231-3246)
+                    self.removed[0] += 1
+                    self._push_real()
+                    self._dev_dirty = True
+                    self._pull_device()
+                    for c in self.constraints:
+                        c.compute_as_if_amputated(realIndex=pr.real, relativeIndex=pr.rel)
+                    amputated = self.compute_total_standard_error(self.constraints, current="amputationStandardError")
+                    val = amputated
+                else:
+                    val = tse[p]
                 ok, drew = True, False
-                if tse[p] > cur:
+                if val > cur:
                     drew = True
                     if u[p] > self.tolerance:
                         ok = False
                     else:
                         self.tolerated += 1
+                if pr.kind == "remove" and not ok:
+                    for c in self.constraints:
+                        c.reject_amputation(realIndex=pr.real, relativeIndex=pr.rel)
                 if log is not None:
-                    log.append((int(gids[p]), ok, float(tse[p] if ok else cur)))
+                    log.append((int(pr.gi), ok, float(val if ok else cur)))
                 if ok:
                     acc = p
                     acc0 += 1
@@ -657,31 +706,49 @@ class Engine:
             # the last one walked, as the per-move loop would
             for ci, c in enumerate(rigid):
                 c.tried = tried0[ci] + sum(1 for p in range(last + 1) if outcome[p].evaluated > ci)
-            if acc >= 0:
-                random.setstate(st0[0])
-                np.random.set_state(st0[1])
-                for p in range(acc + 1):
-                    propose()
-                    for _ in range(outcome[p].draws):
-                        random.random()
-                    if (p < acc and not outcome[p].vetoed) or (p == acc and drew):
-                        random.random()
-                atoms = moved[acc]
-                for c, state in zip(rigid, outcome[acc].states or ()):
-                    # the accepted move's after-move data, kept from its veto
-                    c.set_after_move_state(state)
-                    c.accept_move(realIndexes=atoms, relativeIndexes=atoms)
-                _lib.check(L.frmc_engine_commit(h, acc))
-                self.accepted += 1
-                accepted += 1
-                self.totalStandardError = tse[acc]
-                real[atoms] = reals[acc]
-                self.boxCoordinates[atoms] = boxes[acc]
+            if acc < 0:
+                continue
+            random.setstate(st0[0])
+            np.random.set_state(st0[1])
+            for p in range(acc + 1):
+                propose()
+                for _ in range(outcome[p].draws):
+                    random.random()
+                if (p < acc and u[p] is not None) or (p == acc and drew):
+                    random.random()
+            pr = props[acc]
+            self.accepted += 1
+            accepted += 1
+            if pr.kind == "remove":
+                # Engine.py:3260-3275: the atom leaves the configuration, and
+                # the device engine is rebuilt on the remaining atoms
+                self.removed[1] += 1
+                self.totalStandardError = amputated
+                for c in self.constraints + rigid:
+                    c.accept_amputation(realIndex=pr.real, relativeIndex=pr.rel)
                 for c in resident:
-                    c.resident_commit(atoms, boxes[acc])
+                    c.unbind_resident()
+                self._collect_atom(int(pr.real[0]), int(pr.rel[0]))
+                for c in resident:
+                    c.bind_resident()
+                h = None
+                continue
+            for c, state in zip(rigid, outcome[acc].states or ()):
+                # the accepted move's after-move data, kept from its veto
+                c.set_after_move_state(state)
+                c.accept_move(realIndexes=pr.real, relativeIndexes=pr.rel)
+            _lib.check(L.frmc_engine_commit(h, acc))
+            self.totalStandardError = tse[acc]
+            self.realCoordinates[pr.rel] = pr.moved
+            self.boxCoordinates[pr.rel] = boxes[acc]
+            for c in resident:
+                c.resident_commit(pr.rel, boxes[acc])
+        if h is None:
+            self._device_engine(k)
         return dict(generated=step, accepted=accepted, windows=windows, evaluated=evaluated)
 
 
+_Proposal = collections.namedtuple("_Proposal", "gi kind real rel moved")
 # A proposal's outcome under the rigid constraints: vetoed, how many of them
 # evaluated it, the Python random draws they made, and (when it passed) their
 # after-move states for its acceptance.
@@ -749,6 +816,8 @@ def run_replicas(engines, numberOfSteps=100, k=32, seeds=None, batched=True):
             raise ValueError("replicas need translation moves on every group")
         if E.rigidConstraints:
             raise ValueError("replicas run the native loop, which has no rigid (distance) constraints")
+        if E.collectedAtoms:
+            raise ValueError("replicas run the native loop, whose group table holds every original atom")
         if (float(g0.amplitude[0]), float(g0.amplitude[1])) != amp or float(E.tolerance) != tol:
             raise ValueError("replicas need the same move amplitude and tolerance")
     R = len(engines)

@@ -497,12 +497,17 @@ def test_swap_moves_with_a_distance_constraint_on_the_device_engine():
     assert len(swaps) > 20 and any(swaps)
 
 
-def test_remove_moves_amputation_in_dropin_loop():
+@pytest.mark.parametrize("device", [False, True])
+def test_remove_moves_amputation(device):
     """AtomsRemoveGenerator groups beside translations on a B2 alloy with a
     G(r), an S(Q) and a distance constraint: each removal is evaluated as if
     amputated (atom counts, weights, number density without the atom) and,
     accepted, takes the atom out of every array; later moves run on the
-    smaller system.  Same trajectory as the oracle."""
+    smaller system.  Same trajectory as the oracle, through the per-move
+    drop-in loop and on the device engine (a removal ends its window, is
+    evaluated on the host, and an accepted one rebuilds the device engine on
+    the remaining atoms; the run is split in two calls, so the second starts
+    with atoms already removed)."""
     s = systems.b2_alloy(cells=(5, 5, 5), a=3.01, sigma=0.08, seed=17)
     pdc_exp = pdc_target(s, 0.08, 18, 0.01, 0.02, 300)
     q = np.linspace(0.5, 20, 80).astype(F32)
@@ -528,15 +533,19 @@ def test_remove_moves_amputation_in_dropin_loop():
                           InterMolecularDistanceConstraint(2.3)], groups=groups, gen=gp)
     E.set_tolerance(20.0)
     dlog = []
-    E.run(300, log=dlog)
+    if device:
+        E.run_device(150, k=16, log=dlog)
+        assert 0 < len(E.collectedAtoms)
+        E.run_device(150, k=16, log=dlog)
+    else:
+        E.run(300, log=dlog)
     assert_same_state(E, oe, dlog, olog, (pys, nps))
     c, oc = E.rigidConstraints[0], oe.rigid[0]
     assert c.data["number"].tobytes() == oc.number.tobytes()
     assert c.data["distanceSum"].tobytes() == oc.dsum.tobytes()
     assert 0 < len(E.collectedAtoms) <= 12 and E.numberOfAtoms == n - len(E.collectedAtoms)
     assert sorted(E.collectedAtoms) == sorted(oe.s.collected)
-    with pytest.raises(NotImplementedError):
-        E.run_device(10)
+    assert E.removed[0] >= E.removed[1] == len(E.collectedAtoms)
 
 
 def test_molecular_triclinic_translation_plus_rotation(small_triclinic):
