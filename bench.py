@@ -778,6 +778,47 @@ def swap_moves_record(start, pdc, sfc):
             "dropin_per_move_loop_moves_per_s": round(dropin, 1)}
 
 
+def removal_moves_record(start, pdc, sfc):
+    """Removals on the same B2 box: 500 extra groups remove a drawn atom of
+    the second element (AtomsRemoveGenerator, at most 20 in all) beside the
+    translations of every atom.  The device engine (a removal evaluated on
+    the host at the end of its window, an accepted one rebuilding the engine
+    on the remaining atoms), and the reference's loop over the drop-in
+    calls."""
+    import torch
+    from fullrmc_amd.Generators import AtomsRemoveGenerator, TranslationGenerator
+    els = start["elements"]
+    n = len(els)
+    second = sorted(set(els))[1]
+    pool = [j for j in range(n) if els[j] == second]
+
+    def engine():
+        E = make_engine(start, pdc, sfc, seed=3)
+        rm, tr = AtomsRemoveGenerator(maximumCollected=20, atomsList=pool), TranslationGenerator(0.1)
+        E.set_groups([[i] for i in range(n)] + [[0]] * 500, [tr] * n + [rm] * 500)
+        return E
+
+    E = engine()
+    E.run_device(100, k=16, pull=False)
+    torch.cuda.synchronize()
+    r0 = E.removed[1]
+    t0 = time.perf_counter()
+    st = E.run_device(1000, k=16, pull=False)
+    torch.cuda.synchronize()
+    dev = 1000 / (time.perf_counter() - t0)
+    removed = E.removed[1] - r0
+    E._drop_device()
+    E2 = engine()
+    E2.run(2)
+    t1 = time.perf_counter()
+    E2.run(30)
+    dropin = 30 / (time.perf_counter() - t1)
+    return {"workload": "B2 alloy 13500 atoms: translations of 0.1 A, and 500 groups removing a drawn %s atom "
+                        "(at most 20); PDC + SFC as above" % second,
+            "device_engine_moves_per_s": round(dev, 1), "windows": st["windows"], "removed_in_timed_moves": removed,
+            "dropin_per_move_loop_moves_per_s": round(dropin, 1)}
+
+
 def b2_section(args):
     """The 13,500-atom B2 alloy: engine moves (k <= 64), plain form, the
     reference's loop over the drop-in calls, full histogram, CPU oracle."""
@@ -813,6 +854,7 @@ def b2_section(args):
     out["dropin_per_move_loop_moves_per_s"] = round(30 / (time.perf_counter() - t2), 1)
     E2._drop_device()
     out["swap_moves"] = swap_moves_record(start, pdc, sfc)
+    out["removal_moves"] = removal_moves_record(start, pdc, sfc)
     if not args.no_cpu:
         oe = cpu_engine(start, pdc, sfc, 0.02)
         cps, cn, cel = cpu_moves_per_s(oe, args.cpu_seconds / 2)

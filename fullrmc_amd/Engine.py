@@ -584,7 +584,7 @@ class Engine:
         for c in resident:
             c.bind_resident()
         try:
-            stats = self._python_windows(h, nsteps, k, log, rigid, resident, removals, lists, propose, veto)
+            stats = self._python_windows(h, nsteps, k, log, rigid, resident, lists, propose, veto)
         finally:
             for c in resident:
                 c.unbind_resident()
@@ -597,7 +597,7 @@ class Engine:
             r = np.ascontiguousarray(self.realCoordinates, dtype=np.float32)
             _lib.check(_lib.lib().frmc_engine_set_real(self._dev, r.ctypes.data_as(_lib.c_f32p)))
 
-    def _python_windows(self, h, nsteps, k, log, rigid, resident, removals, lists, propose, veto):
+    def _python_windows(self, h, nsteps, k, log, rigid, resident, lists, propose, veto):
         """The window loop of _run_python."""
         L = _lib.lib()
         step = 0
@@ -611,11 +611,11 @@ class Engine:
             # on the device: about one expected acceptance per window
             p_acc = (acc0 + 1.0) / (gen0 + 2.0)
             kw = min(k, nsteps - step, max(1, int(np.ceil(1.0 / p_acc))))
-            if rigid or removals:
+            if rigid:
                 # a veto in Python (four drop-in calls per constraint) costs
                 # more than a device window, so a window ends at its first
                 # proposal that passes them: no veto is evaluated for a
-                # proposal past the window's acceptance; a removal ends it too
+                # proposal past the window's acceptance
                 kw = min(k, nsteps - step)
             props, boxes, u, outcome = [], [], [], []
             # the streams are saved once per window; an accepted proposal's
@@ -636,6 +636,7 @@ class Engine:
                 outcome.append(v)
                 u.append(None if v.vetoed or pr.kind == "none" else random.random())
                 boxes.append(mb)
+                # a removal is evaluated on the host after the moves before it
                 if pr.kind == "remove" or (rigid and pr.kind == "move" and not v.vetoed):
                     break
             kw = len(props)
