@@ -104,3 +104,24 @@ def test_replicas_refuse_rigid_constraints():
         assert "rigid" in str(e)
     else:
         raise AssertionError("run_replicas accepted an engine with a rigid constraint")
+
+
+def test_replicas_refuse_removed_atoms():
+    """An engine with atoms already removed keeps its groups as original
+    indexes, which the native loop's group table cannot hold: refused before
+    anything runs."""
+    from fullrmc_amd import systems
+    from fullrmc_amd.Constraints import PairDistributionConstraint
+
+    s = systems.random_box(20, 10.0, 1.5, seed=4)
+    E = eng.Engine()
+    E.set_system(s["real"], s["elements"], s["molecules"], s["basis"], s["recip"])
+    x = (0.05 + 0.1 * np.arange(40)).astype(np.float32)
+    E.add_constraints([PairDistributionConstraint(np.stack([x, np.zeros_like(x)], 1))])
+    E.collectedAtoms.append(3)
+    try:
+        eng.run_replicas([E], 10)
+    except ValueError as e:
+        assert "removed" in str(e) or "original" in str(e)
+    else:
+        raise AssertionError("run_replicas accepted an engine with removed atoms")
