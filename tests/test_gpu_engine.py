@@ -566,6 +566,39 @@ def test_molecular_triclinic_translation_plus_rotation(small_triclinic):
     assert_same_state(E, oe, dlog, olog, (pys, nps))
 
 
+@pytest.mark.parametrize("flexible,rp", [(True, 0.5), (False, 1.0)])
+def test_molecular_moves_with_a_distance_constraint_on_the_device_engine(small_triclinic, flexible, rp):
+    """Whole-molecule translation + rotation windows with an inter-molecular
+    distance constraint on the device engine: every veto evaluates the
+    13-atom group against all atoms and the group alone on the resident
+    copies; trajectory, streams, state and the constraint's data equal the
+    oracle's."""
+    s = small_triclinic
+    exp = pdc_target(s, 0.05, 8, 0.025, 0.05, 140)
+    mols = {}
+    for i, m in enumerate(s["molecules"]):
+        mols.setdefault(int(m), []).append(i)
+    groups = list(mols.values())
+    lim = {("C", "H"): 2.2, ("H", "H"): 2.4}
+    ogen = oeng.Combined([oeng.Translation(0.4), oeng.Rotation(20)])
+    oe, olog, pys, nps = oracle_run(s, groups, ogen, lambda osys, w: [oeng.PDC(osys, exp, w)], 150, 9,
+                                    rigid_fn=lambda o: [oeng.IMD(o, 2.6, lim, flexible=flexible, rejectProbability=rp)])
+    random.seed(9)
+    np.random.seed(9)
+    gen = MoveGeneratorCombinator([TranslationGenerator(0.4), RotationGenerator(20)])
+    imd = InterMolecularDistanceConstraint(2.6, pairsDistanceDefinition=[(a, b, d) for (a, b), d in lim.items()],
+                                           flexible=flexible, rejectProbability=rp)
+    E = device_engine(s, [PairDistributionConstraint(exp), imd], groups=groups, gen=gen)
+    dlog = []
+    E.run_device(150, k=8, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    oc = oe.rigid[0]
+    assert imd.data["number"].tobytes() == oc.number.tobytes()
+    assert imd.data["distanceSum"].tobytes() == oc.dsum.tobytes()
+    assert np.float32(imd.standardError).tobytes() == np.float32(oc.sd).tobytes()
+    assert E.tried < E.generated and E.accepted > 5
+
+
 def test_b2_13500_size_properties():
     """B2 alloy at full size (13,500 atoms, PDC + SFC): after a device run the
     engine's histograms equal a fresh full recompute in the symmetrised sums
