@@ -930,7 +930,11 @@ __device__ __forceinline__ void eval_pc_body(const View &V, int32_t p, int32_t u
     float *scratch = sm + Hp + 2 * 2 * kPcGroups * 64 * 4;
     int32_t *lprog = (int32_t *)(scratch + V.maxLeaves + kStack);
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
-    for (int64_t s = b * blockDim.x + tid; s < V.zero_len; s += nblk * blockDim.x) V.dn_zero[s] = 0;
+    // the other delta buffer is cleared for the next window after this
+    // block's last barrier, so that no barrier waits for these stores
+    auto clear_next = [&] {
+        for (int64_t s = b * blockDim.x + tid; s < V.zero_len; s += nblk * blockDim.x) V.dn_zero[s] = 0;
+    };
     int32_t ci = 0;
     while (ci + 1 < V.C && u >= V.c[ci + 1].uOff) ci++;
     const DevC c = V.c[ci];
@@ -944,6 +948,7 @@ __device__ __forceinline__ void eval_pc_body(const View &V, int32_t p, int32_t u
             V.chi2d[(int64_t)p * V.C + ci] = chi;
             publish(V.out_res + (int64_t)p * V.resW + 1 + ci, chi, V.seq);
         }
+        clear_next();
         return;
     }
     const int32_t leaf = u - c.uOff;
@@ -1063,6 +1068,7 @@ __device__ __forceinline__ void eval_pc_body(const View &V, int32_t p, int32_t u
     // the leaf's sum in NumPy's order (the host combines the leaves)
     __syncthreads();
     if (tid == 0) publish(V.out_res + (int64_t)p * V.resW + c.leafOff + leaf, leaf_sum(scratch, len), V.seq);
+    clear_next();
 }
 
 // grid as k_eval_a's (units x proposals, or XCD-aligned), blocks of
