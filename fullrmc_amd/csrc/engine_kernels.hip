@@ -648,8 +648,13 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
     float *scratch = Gs + Hp;                    // [maxLeaves + kStack] (S(Q): the leaf's 128 deviations)
     int32_t *lprog = (int32_t *)(scratch + V.maxLeaves + kStack);  // [2*maxLeaves]
     const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
-    if (V.leafmode)
-        for (int64_t s = b * blockDim.x + tid; s < V.zero_len; s += nblk * blockDim.x) V.dn_zero[s] = 0;
+    // leaf layout: the other delta buffer is cleared for the next window
+    // after this block's last barrier, so that no barrier waits for these
+    // stores
+    auto clear_next = [&] {
+        if (V.leafmode)
+            for (int64_t s = b * blockDim.x + tid; s < V.zero_len; s += nblk * blockDim.x) V.dn_zero[s] = 0;
+    };
     int32_t ci = 0;
     while (ci + 1 < V.C && u >= V.c[ci + 1].uOff) ci++;
     // constraint fields in registers: keeps every load below unconditional
@@ -723,6 +728,7 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
             for (int64_t s = (int64_t)p * blockDim.x + tid; s < F->zero_len; s += (int64_t)F->nb * blockDim.x)
                 F->dn_zero[s] = 0;
         }
+        clear_next();
         return;
     }
     if (chain_wave) {
@@ -805,6 +811,7 @@ __device__ __forceinline__ void eval_a_body(const View &V, const int32_t *dnw, i
     // the leaf's sum in NumPy's order (the host combines the leaves)
     __syncthreads();
     if (tid == 0) publish(V.out_res + (int64_t)p * V.resW + c.leafOff + leaf, leaf_sum(scratch, len), V.seq);
+    clear_next();
 }
 
 // --------------------------------------------------------------- k_eval_b
