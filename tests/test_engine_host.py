@@ -125,3 +125,29 @@ def test_replicas_refuse_removed_atoms():
         assert "removed" in str(e) or "original" in str(e)
     else:
         raise AssertionError("run_replicas accepted an engine with removed atoms")
+
+
+def test_rejection_draws_counts_the_draws_of_the_veto():
+    """The device engine's windows replay a veto's Python random draws from
+    rejection_draws: it must equal the draws should_step_get_rejected makes
+    (one for a flexible constraint whose error grows, none otherwise), for
+    every outcome."""
+    from fullrmc_amd.Constraints import InterMolecularDistanceConstraint
+
+    for flexible in (True, False):
+        for rp in (0.0, 0.5, 1.0):
+            c = InterMolecularDistanceConstraint(2.0, flexible=flexible, rejectProbability=rp)
+            c.standardError = np.float32(1.0)
+            # strict vetoes read the counts of the move's pairs
+            c.activeAtomsDataBeforeMove = {"number": np.zeros((1, 1, 1), np.int32)}
+            c.activeAtomsDataAfterMove = {"number": np.ones((1, 1, 1), np.int32)}
+            for after in (np.float32(0.5), np.float32(1.0), np.float32(1.5)):
+                for seed in range(20):
+                    random.seed(seed)
+                    before = random.getstate()
+                    for _ in range(c.rejection_draws(after)):
+                        random.random()
+                    replayed = random.getstate()
+                    random.setstate(before)
+                    c.should_step_get_rejected(after)
+                    assert random.getstate() == replayed, (flexible, rp, after)
