@@ -463,10 +463,11 @@ class Engine:
 
         With translation moves on every group (the default generator) the
         whole loop, random streams included, runs in the native library;
-        otherwise, or with rigid (distance) constraints, proposals come from
-        the Python generators (the reference's own NumPy calls), the rigid
-        constraints veto them in Python, and the evaluation runs on the
-        device.
+        otherwise, or with rigid (distance) constraints or removed atoms,
+        proposals come from the Python generators (the reference's own NumPy
+        calls), the rigid constraints veto them in Python, removals are
+        evaluated on the host at the end of their window, and the moves are
+        evaluated on the device.
         ``pull=False`` leaves the state on the device (the host mirror is
         refreshed by the next :meth:`run`, :meth:`sync_from_device` or a
         pulling call).  Returns a dict of counters and timings."""
