@@ -497,6 +497,42 @@ def test_swap_moves_with_a_distance_constraint_on_the_device_engine():
     assert len(swaps) > 20 and any(swaps)
 
 
+def test_swaps_removals_and_a_distance_constraint_together_on_the_device_engine():
+    """Swaps, translations and removals of the B2 alloy with a strict
+    distance constraint, all on the device engine: atom-list windows, vetoed
+    proposals without draws, removals (one rejected, eight accepted, each
+    rebuilding the engine and the constraint's resident copies); the same
+    trajectory, streams and final state as the oracle."""
+    s = systems.b2_alloy(cells=(5, 5, 5), a=3.01, sigma=0.08, seed=13)
+    exp = pdc_target(s, 0.08, 14, 0.01, 0.02, 300)
+    n = len(s["real"])
+    el = s["elements"]
+    al = [j for j in range(n) if el[j] == "Al"]
+    elw = {e: WEIGHTS[e] for e in sorted(set(el))}
+    groups = [[i] for i in range(n)] + [[0]] * 40
+    gp = ([SwapPositionsGenerator(swapList=[[j] for j in al]) if el[i] == "Fe" else TranslationGenerator(0.3)
+           for i in range(n)] + [AtomsRemoveGenerator(maximumCollected=8, atomsList=al)] * 40)
+    go = ([oeng.Swap([[j] for j in al]) if el[i] == "Fe" else oeng.Translation(0.3) for i in range(n)]
+          + [oeng.Remove(al, 8)] * 40)
+    oe, olog, pys, nps = oracle_run(s, groups, go, lambda osys, w: [oeng.PDC(osys, exp, w, elementWeights=elw)],
+                                    300, 25, tolerance=5.0, rigid_fn=lambda o: [oeng.IMD(o, 2.45, flexible=False)])
+    random.seed(25)
+    np.random.seed(25)
+    E = device_engine(s, [PairDistributionConstraint(exp), InterMolecularDistanceConstraint(2.45, flexible=False)],
+                      groups=groups, gen=gp)
+    E.set_tolerance(5.0)
+    dlog = []
+    E.run_device(300, k=16, log=dlog)
+    assert_same_state(E, oe, dlog, olog, (pys, nps))
+    c, oc = E.rigidConstraints[0], oe.rigid[0]
+    assert c.data["number"].tobytes() == oc.number.tobytes()
+    assert c.data["distanceSum"].tobytes() == oc.dsum.tobytes()
+    assert sorted(E.collectedAtoms) == sorted(oe.s.collected) and len(E.collectedAtoms) == 8
+    assert E.removed[0] > E.removed[1] == 8
+    swaps = [a for g, a, _ in dlog if g < n and el[g] == "Fe"]
+    assert sum(swaps) > 5 and E.tried < E.generated
+
+
 @pytest.mark.parametrize("device", [False, True])
 def test_remove_moves_amputation(device):
     """AtomsRemoveGenerator groups beside translations on a B2 alloy with a
