@@ -21,7 +21,9 @@ against the device's (``parity_checked_moves``).
 
 Also reported: the plain one-proposal-per-window form on the same box; the
 13,500-atom B2 alloy (G(r) + S(Q)), the 256 batched replicas and the 6,500-
-atom molecular triclinic box with their own CPU baselines; the S(Q)
+atom molecular triclinic box with their own CPU baselines; swap moves and
+removals on the B2 box and the molecular box with a distance constraint,
+each beside the reference's per-move loop over the drop-in calls; the S(Q)
 transform entry; the roofline of the dominant kernel.
 """
 import argparse
